@@ -1,0 +1,239 @@
+/*
+ * vsc.h — C ABI of the MI355X (gfx950) per-frame consistency hot path.
+ *
+ * Drop-in boundary for two parts of MaxReimann/video-stream-consistency:
+ *   1. the ONNX Runtime custom ops "Correlation" and "Warp" of
+ *      src/ort_custom_ops (same tensor layouts, attributes and output shapes);
+ *   2. the per-frame stabilization entry points of src/stabilization
+ *      (get_warp_result, get_adap_comb, get_consist_wt, get_consist_out,
+ *      get_bilinear, perform_consistency, the GPUImage u8<->float conversions
+ *      and VideoStabilizer::doOneStep with its hyperParams).
+ *
+ * Conventions (differences from the reference are deliberate, see each entry):
+ *   - every entry point returns int status: VSC_OK (0) or a negative VSC_E_*;
+ *     nothing throws or calls exit() across this boundary.  vsc_last_error()
+ *     returns a thread-local message describing the last failure.
+ *   - all pointers are device pointers unless stated otherwise; the caller owns
+ *     every buffer it passes.  All work is enqueued asynchronously on the
+ *     caller's hipStream_t (NULL = the null stream); nothing synchronises the
+ *     device except where stated.
+ *   - images are HWC interleaved float, element (x,y,c) at
+ *     data[(y*width + x)*channels + c] (the GPUImageWrapper layout,
+ *     src/stabilization/gpuimagewrapper.h:8-25).
+ */
+#ifndef VSC_H
+#define VSC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t without pulling in the HIP headers. */
+typedef struct ihipStream_t* vsc_hip_stream;
+
+#define VSC_OK 0
+#define VSC_E_INVALID (-1)   /* bad shape, attribute, null pointer or parameter   */
+#define VSC_E_DTYPE (-2)     /* unsupported element type                         */
+#define VSC_E_HIP (-3)       /* a HIP runtime call or kernel launch failed        */
+#define VSC_E_NOMEM (-4)     /* device allocation failed                          */
+#define VSC_E_UNSUPPORTED (-5) /* valid request this build does not implement     */
+
+typedef enum {
+    VSC_F32 = 0,
+    VSC_F64 = 1,
+    VSC_BF16 = 2,
+    VSC_F16 = 3,
+    VSC_FP8E4M3 = 4
+} vsc_dtype;
+
+/* Device image view: replaces GPUImage / GPUImageWrapper
+ * (src/stabilization/gpuimage.h:8-31, gpuimagewrapper.h:8-25). */
+typedef struct {
+    float* data;
+    int width;
+    int height;
+    int channels;
+} vsc_image;
+
+/* Consistency hyper-parameters: replaces struct hyperParams
+ * (src/stabilization/videostabilizer.h:38-46). */
+typedef struct {
+    float alpha;        /* photometric weight of the adaptive combination     */
+    float beta;         /* photometric weight of the consistency weights      */
+    float gamma;        /* consistency strength (the interactive flicker knob) */
+    int pyramid_levels; /* solver pyramid depth (1..VSC_MAX_LEVELS)            */
+    int num_iter;       /* sweeps at level 0; level j runs num_iter/(j+1)      */
+    float step_size;    /* gradient step                                      */
+    float mom_fac;      /* momentum factor                                    */
+} vsc_params;
+
+#define VSC_MAX_LEVELS 4
+
+const char* vsc_last_error(void);
+const char* vsc_version(void);
+
+/* Defaults of VideoStabilizer::initHyperParams (videostabilizer.cpp:104-113):
+ * alpha = beta = 6800, gamma = 2, 2 levels, 150 iterations, step = mom = 0.15. */
+int vsc_default_params(vsc_params* out);
+
+/* ---------------------------------------------------------------------------
+ * ORT custom op "Correlation".
+ * Replaces CorrelationKernel::Compute (ort_custom_ops/opticalflow/correlation.h:35-50)
+ * -> ComputeCPU (src/opticalflow/correlation.cc:45-143) / ComputeCUDA
+ * (src/opticalflow/correlation_cuda.cc:29-138).
+ * f1, f2: NCHW [N,C,H,W] of in_dtype.  out: FP32 (FP64 when in_dtype is F64)
+ *   non-legacy: [N, 2md+1, 2md+1, H, W], out = sum_c f1 * f2(shifted), 0 outside;
+ *   legacy:     [N, (2md+1)^2, H, W],  out = (sum_c f1 * f2(shifted)) / C,
+ *               zero padding, md <= 4 (correlation_cuda.cu:182-331,359-360).
+ * in_dtype F32 accumulates sequentially in c in FP32 (the reference CPU op's
+ * order); BF16 / F16 / FP8E4M3 inputs are NCHW arrays of that element type
+ * (raw bit patterns, 2/2/1 bytes) multiplied on MFMA with FP32 accumulation.
+ * chan_scale (FP8 only, nullable = 1): per-channel dequantisation scale s_c so
+ *   that out = sum_c s_c * q1_c * q2_c (SmoothQuant-style equalised operands).
+ * Errors: VSC_E_INVALID for N,C,H,W < 1, md < 0, legacy with md > 4;
+ *         VSC_E_DTYPE for an unknown in_dtype (the reference throws
+ *         "Unsupported input type", correlation.cc:140-141).                  */
+int vsc_correlation_forward(const void* f1, const void* f2, void* out, int N, int C, int H, int W,
+                            int max_displacement, int legacy, int in_dtype, const float* chan_scale,
+                            vsc_hip_stream stream);
+
+/* Host-side quantiser for the FP8 e4m3 path: SmoothQuant equalisation
+ * r_c = sqrt(max|f2_c| / max|f1_c|), f1' = f1*r_c, f2' = f2/r_c, then one
+ * per-tensor e4m3 scale per operand.  Writes q1, q2 (device, 1 byte/elem) and
+ * chan_scale[c] = s1*s2 (device, C floats).  Device-side, asynchronous.    */
+int vsc_quantize_fp8_pair(const float* f1, const float* f2, uint8_t* q1, uint8_t* q2, float* chan_scale,
+                          int N, int C, int H, int W, vsc_hip_stream stream);
+
+/* Device-side converters for the 16-bit paths (round to nearest even). */
+int vsc_convert_f32(const float* in, void* out, size_t count, int out_dtype, vsc_hip_stream stream);
+
+/* ---------------------------------------------------------------------------
+ * ORT custom op "Warp".
+ * Replaces WarpKernel::Compute (ort_custom_ops/opticalflow/warp.h:11-36) ->
+ * ComputeCPU (src/opticalflow/warp.cc:25-134) / ComputeCUDA (warp_cuda.cc:28-77).
+ * in NCHW [N,C,H,W], flow [N,2,H,W] (x then y displacement), out NCHW.
+ * Bilinear backward warp with zero padding; a sample whose in-bounds bilinear
+ * mass is <= 0.999 is 0.  dtype: VSC_F32 (FP32 weights) or VSC_F64.        */
+int vsc_warp_forward(const void* in, const void* flow, void* out, int N, int C, int H, int W, int dtype,
+                     vsc_hip_stream stream);
+
+/* ---------------------------------------------------------------------------
+ * Stabilization stages (src/stabilization/flowconsistency.cuh:3-50).
+ * All images HWC float; sizes must match where the reference reads them
+ * together.  The reference synchronises the device after each stage
+ * (flowconsistency.cu:285-373); these entry points do not.                  */
+
+/* get_warp_result (flowconsistency.cu:77-114,301-312): out = in sampled at
+ * clamp(x+flow, 0, W-3/H-3), bilinear, 3 channels; flow has 2 or 3 channels. */
+int vsc_stab_warp(vsc_image in, vsc_image flow, vsc_image out, vsc_hip_stream stream);
+
+/* get_adap_comb (flowconsistency.cu:116-166,314-333). */
+int vsc_stab_adap_comb(vsc_image crnt_in, vsc_image crnt_pr, vsc_image prev_warp_in, vsc_image prev_warp_pr,
+                       vsc_image next_warp_in, vsc_image next_warp_pr, vsc_image adap_cmb_in,
+                       vsc_image adap_cmb_pr, vsc_image last_stab_warp, float alpha, vsc_hip_stream stream);
+
+/* get_consist_wt (flowconsistency.cu:168-191,335-348). */
+int vsc_stab_consist_wt(vsc_image adap_cmb_in, vsc_image crnt_in, vsc_image consist_wt, float beta, float gamma,
+                        vsc_hip_stream stream);
+
+/* get_consist_out (flowconsistency.cu:193-258,350-374): num_iter gradient
+ * sweeps with momentum; consist_out holds the initial solution on entry and
+ * the result on return.  Jacobi semantics (each sweep reads only the previous
+ * sweep) — the reference updates in place, which is schedule-dependent.
+ * workspace: device scratch of vsc_stab_consist_out_workspace_size() bytes, or
+ * NULL to let the call allocate it stream-ordered (hipMallocAsync/FreeAsync). */
+size_t vsc_stab_consist_out_workspace_size(int width, int height);
+int vsc_stab_consist_out(vsc_image crnt_pr, vsc_image prev_stab_warp, vsc_image consist_wt, int num_iter,
+                         float step_size, float mom_fac, vsc_image consist_out, void* workspace,
+                         vsc_hip_stream stream);
+
+/* get_bilinear (flowconsistency.cu:50-75,288-299): resample in -> out. */
+int vsc_stab_bilinear(vsc_image in, vsc_image out, vsc_hip_stream stream);
+
+/* perform_consistency (flowconsistency.cu:35-48,269-286): out = 0.5*prev + 0.5*next
+ * of the processed frames (the reference declares more arguments but only
+ * these three are read; it has no caller). */
+int vsc_average_frames(vsc_image processed_prev, vsc_image processed_next, vsc_image out, vsc_hip_stream stream);
+
+/* char_to_cuda_float_image / kernel_to_float_image (gpuimage.cu:39-51,70-104):
+ * RGBA8 (device, width*height*4 bytes) -> 3-channel float, v/255; alpha dropped. */
+int vsc_rgba_to_float(const uint8_t* rgba, vsc_image out, vsc_hip_stream stream);
+
+/* cuda_float_to_char_image / kernel_to_char_image (gpuimage.cu:54-67,107-135):
+ * 3-channel float -> RGBA8, u8 = (uint8)floor(|v|*255) with uint32 saturation
+ * then truncation mod 256 (the reference's wrap for v >= 256/255); alpha = 1. */
+int vsc_float_to_rgba(vsc_image in, uint8_t* rgba, vsc_hip_stream stream);
+
+/* ---------------------------------------------------------------------------
+ * Per-stream stabilizer: VideoStabilizer (videostabilizer.h:48-110) without
+ * the Qt frame lists.  Owns the device-resident, double-buffered state (the
+ * previous stabilized frame and every intermediate image); nothing is
+ * allocated per frame.  One vsc_stream may be driven by one host thread at a
+ * time; different streams are independent.                                 */
+typedef struct vsc_stream vsc_stream;
+
+/* flow_width/flow_height: resolution the flows arrive at (== width/height,
+ * or the FLOWDOWNSCALE resolution round(W/f) x round(H/f), in which case
+ * vsc_stab_frame upsamples them with get_bilinear as flowmodel.cpp:158-161). */
+int vsc_stream_create(int width, int height, int flow_width, int flow_height, vsc_stream** out);
+int vsc_stream_destroy(vsc_stream* s);
+/* Device bytes held by a stream of this size (for admission control). */
+size_t vsc_stream_footprint(int width, int height, int flow_width, int flow_height);
+
+/* preloadProcessedFrames (videostabilizer.cpp:136-153): the first `last`
+ * stabilized frame is processed frame 2k+B-1 (RGBA8 device image). */
+int vsc_stream_set_last_rgba(vsc_stream* s, const uint8_t* processed_rgba, vsc_hip_stream stream);
+/* Set `last` from a 3-channel float HWC device image. */
+int vsc_stream_set_last(vsc_stream* s, vsc_image last, vsc_hip_stream stream);
+/* Copy the current `last` stabilized frame into a 3-channel HWC float image
+ * (stream-ordered). */
+int vsc_stream_copy_last(vsc_stream* s, vsc_image out, vsc_hip_stream stream);
+
+typedef struct {
+    /* RGBA8 device frames (width*height*4 bytes): the k=1 window. */
+    const uint8_t* orig_prev;
+    const uint8_t* orig_cur;
+    const uint8_t* orig_next;
+    const uint8_t* proc_prev;
+    const uint8_t* proc_cur;
+    const uint8_t* proc_next;
+    /* HWC float device flows at flow_width x flow_height, flow_channels (2|3):
+     * flow_fwd maps cur -> next, flow_bwd maps cur -> prev. */
+    const float* flow_fwd;
+    const float* flow_bwd;
+    int flow_channels;
+} vsc_frame_in;
+
+#define VSC_FRAME_FUSED 0     /* fused front pass + temporally blocked solver */
+#define VSC_FRAME_STAGED 1    /* one launch per reference stage (parity aid) */
+
+/* Stabilization body of VideoStabilizer::doOneStep (videostabilizer.cpp:177-238):
+ * 5 warps, adaptive combination (alpha), consistency weights (beta, gamma),
+ * pyramid solve, RGBA8 output; then last <- stabilized frame.
+ * out_rgba: device RGBA8 (may be NULL); out_float: optional HWC float copy.
+ * params: NULL = defaults; read at call time (the interactive knob applies
+ * from the next frame, as videostabilizer.cpp:192). */
+int vsc_stab_frame(vsc_stream* s, const vsc_frame_in* in, const vsc_params* params, uint8_t* out_rgba,
+                   float* out_float, int mode, vsc_hip_stream stream);
+
+/* ---------------------------------------------------------------------------
+ * Per-kernel timing (replaces the per-stage QElapsedTimer accounting of
+ * VideoStabilizer, videostabilizer.h:72-76 / videostabilizer.cpp:175,240-262,
+ * with device-side timing).  Between begin and end, every launch of the
+ * instrumented kernels ("consist_blocked": the temporally blocked solver,
+ * "front_pass": the fused front pass) is bracketed by HIP events recorded on
+ * the launch stream, up to `capacity` launches.  vsc_profile_end synchronises
+ * those events and returns, for one kernel name, the number of launches, their
+ * summed duration and their summed algorithmic bytes (tensors crossing the
+ * launch boundary, each counted once).  Host-thread-global; not reentrant.  */
+int vsc_profile_begin(int capacity);
+int vsc_profile_end(const char* kernel, int* launches, double* total_ms, double* algorithmic_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VSC_H */

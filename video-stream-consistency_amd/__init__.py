@@ -1,0 +1,453 @@
+"""MI355X (gfx950) per-frame consistency hot path of video-stream-consistency.
+
+Python mirror of the reference interfaces over the C ABI of ``libvsc.so``
+(``include/vsc.h``):
+
+* the ONNX Runtime custom ops ``Correlation`` and ``Warp``
+  (src/ort_custom_ops/include/ort_custom_ops/opticalflow/correlation.h:10-84,
+  warp.h:11-60) as :func:`correlation` and :func:`warp`;
+* the stabilization stage functions of src/stabilization/flowconsistency.cuh:3-50
+  (``get_warp_result``, ``get_adap_comb``, ``get_consist_wt``,
+  ``get_consist_out``, ``get_bilinear``, ``perform_consistency``) on
+  :class:`GPUImage` objects;
+* :class:`VideoStabilizer` with ``hyperParams`` and ``doOneStep``
+  (src/stabilization/videostabilizer.h:38-110).
+
+torch supplies device memory and the current HIP stream; every computation
+runs in the hand-written HIP kernels of ``libvsc.so``.  There is no fallback:
+if the library is missing, loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvsc.so")
+SYNTH_PATH = os.path.join(_HERE, "libvsc_synth.so")
+
+VSC_OK = 0
+VSC_F32, VSC_F64, VSC_BF16, VSC_F16, VSC_FP8E4M3 = 0, 1, 2, 3, 4
+VSC_FRAME_FUSED, VSC_FRAME_STAGED = 0, 1
+VSC_MAX_LEVELS = 4
+
+#: every function the C header declares (checked against the built library)
+EXPORTED = (
+    "vsc_last_error", "vsc_version", "vsc_default_params", "vsc_correlation_forward",
+    "vsc_quantize_fp8_pair", "vsc_convert_f32", "vsc_warp_forward", "vsc_stab_warp",
+    "vsc_stab_adap_comb", "vsc_stab_consist_wt", "vsc_stab_consist_out_workspace_size",
+    "vsc_stab_consist_out", "vsc_stab_bilinear", "vsc_average_frames", "vsc_rgba_to_float",
+    "vsc_float_to_rgba", "vsc_stream_create", "vsc_stream_destroy", "vsc_stream_footprint",
+    "vsc_stream_set_last_rgba", "vsc_stream_set_last", "vsc_stream_copy_last", "vsc_stab_frame",
+    "vsc_profile_begin", "vsc_profile_end",
+)
+
+
+class VscImage(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("channels", ctypes.c_int)]
+
+
+class VscParams(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_float), ("beta", ctypes.c_float), ("gamma", ctypes.c_float),
+                ("pyramid_levels", ctypes.c_int), ("num_iter", ctypes.c_int),
+                ("step_size", ctypes.c_float), ("mom_fac", ctypes.c_float)]
+
+
+class VscFrameIn(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("orig_prev", "orig_cur", "orig_next", "proc_prev",
+                                                "proc_cur", "proc_next", "flow_fwd", "flow_bwd")] + \
+               [("flow_channels", ctypes.c_int)]
+
+
+class SynthCfg(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("nframes", ctypes.c_int),
+                ("nblobs", ctypes.c_int), ("sigma_min", ctypes.c_float), ("sigma_max", ctypes.c_float),
+                ("motion", ctypes.c_float), ("npatches", ctypes.c_int), ("patch_size", ctypes.c_int),
+                ("gain_min", ctypes.c_float), ("gain_max", ctypes.c_float), ("offset_max", ctypes.c_float),
+                ("noise_sigma", ctypes.c_float), ("seed", ctypes.c_ulonglong)]
+
+
+class VscError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vsc error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_synth_lib = None
+
+
+def lib():
+    """Load libvsc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(make -C video-stream-consistency_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+        img = VscImage
+        L.vsc_last_error.restype = ctypes.c_char_p
+        L.vsc_version.restype = ctypes.c_char_p
+        L.vsc_default_params.argtypes = [ctypes.POINTER(VscParams)]
+        L.vsc_correlation_forward.argtypes = [vp, vp, vp, i, i, i, i, i, i, i, vp, vp]
+        L.vsc_quantize_fp8_pair.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, vp]
+        L.vsc_convert_f32.argtypes = [vp, vp, ctypes.c_size_t, i, vp]
+        L.vsc_warp_forward.argtypes = [vp, vp, vp, i, i, i, i, i, vp]
+        L.vsc_stab_warp.argtypes = [img, img, img, vp]
+        L.vsc_stab_adap_comb.argtypes = [img] * 9 + [f, vp]
+        L.vsc_stab_consist_wt.argtypes = [img, img, img, f, f, vp]
+        L.vsc_stab_consist_out_workspace_size.argtypes = [i, i]
+        L.vsc_stab_consist_out_workspace_size.restype = ctypes.c_size_t
+        L.vsc_stab_consist_out.argtypes = [img, img, img, i, f, f, img, vp, vp]
+        L.vsc_stab_bilinear.argtypes = [img, img, vp]
+        L.vsc_average_frames.argtypes = [img, img, img, vp]
+        L.vsc_rgba_to_float.argtypes = [vp, img, vp]
+        L.vsc_float_to_rgba.argtypes = [img, vp, vp]
+        L.vsc_stream_create.argtypes = [i, i, i, i, ctypes.POINTER(vp)]
+        L.vsc_stream_destroy.argtypes = [vp]
+        L.vsc_stream_footprint.argtypes = [i, i, i, i]
+        L.vsc_stream_footprint.restype = ctypes.c_size_t
+        L.vsc_stream_set_last_rgba.argtypes = [vp, vp, vp]
+        L.vsc_stream_set_last.argtypes = [vp, img, vp]
+        L.vsc_stream_copy_last.argtypes = [vp, img, vp]
+        L.vsc_stab_frame.argtypes = [vp, ctypes.POINTER(VscFrameIn), ctypes.POINTER(VscParams), vp, vp, i, vp]
+        L.vsc_profile_begin.argtypes = [i]
+        L.vsc_profile_end.argtypes = [ctypes.c_char_p, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != VSC_OK:
+        raise VscError(rc, lib().vsc_last_error().decode())
+    return rc
+
+
+def version():
+    return lib().vsc_version().decode()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+# ---------------------------------------------------------------------------
+# ORT custom ops
+# ---------------------------------------------------------------------------
+_DT = None
+
+
+def _dtype_code(t):
+    torch = _torch()
+    global _DT
+    if _DT is None:
+        _DT = {torch.float32: VSC_F32, torch.float64: VSC_F64, torch.bfloat16: VSC_BF16,
+               torch.float16: VSC_F16}
+        if hasattr(torch, "float8_e4m3fn"):
+            _DT[torch.float8_e4m3fn] = VSC_FP8E4M3
+    if t.dtype not in _DT:
+        raise VscError(-2, f"unsupported input type {t.dtype}")
+    return _DT[t.dtype]
+
+
+def correlation(input1, input2, max_displacement: int = 4, legacy: int = 0, chan_scale=None, out=None):
+    """``custom::Correlation`` (correlation.h:60-84; correlation.cc:45-143).
+
+    input1, input2: NCHW device tensors of float32 / float64 / bfloat16 /
+    float16 / float8_e4m3fn.  Returns [N, 2md+1, 2md+1, H, W] (legacy=0) or
+    [N, (2md+1)^2, H, W] (legacy=1), float32 (float64 for float64 inputs)."""
+    torch = _torch()
+    if input1.shape != input2.shape or input1.dim() != 4:
+        raise VscError(-1, "Correlation expects two NCHW tensors of the same shape")
+    a = input1.contiguous()
+    b = input2.contiguous()
+    N, C, H, W = a.shape
+    P = 2 * int(max_displacement) + 1
+    odt = torch.float64 if a.dtype == torch.float64 else torch.float32
+    shape = (N, P * P, H, W) if legacy else (N, P, P, H, W)
+    if out is None:
+        out = torch.empty(shape, dtype=odt, device=a.device)
+    cs = None if chan_scale is None else _ptr(chan_scale.contiguous())
+    check(lib().vsc_correlation_forward(_ptr(a), _ptr(b), _ptr(out), N, C, H, W, int(max_displacement),
+                                        int(bool(legacy)), _dtype_code(a), cs, _stream()))
+    return out
+
+
+def warp(input, flow, out=None):
+    """``custom::Warp`` (warp.h:38-60; warp.cc:25-134): NCHW input, N2HW flow."""
+    torch = _torch()
+    a = input.contiguous()
+    fl = flow.contiguous()
+    N, C, H, W = a.shape
+    if tuple(fl.shape) != (N, 2, H, W):
+        raise VscError(-1, "Warp expects flow of shape [N, 2, H, W]")
+    if out is None:
+        out = torch.empty_like(a)
+    check(lib().vsc_warp_forward(_ptr(a), _ptr(fl), _ptr(out), N, C, H, W, _dtype_code(a), _stream()))
+    return out
+
+
+def quantize_fp8_pair(f1, f2):
+    """SmoothQuant-equalised e4m3 operands and per-channel scales for the FP8
+    correlation (returns q1, q2 as uint8 tensors of e4m3 bits, scale [C])."""
+    torch = _torch()
+    a = f1.contiguous().float()
+    b = f2.contiguous().float()
+    N, C, H, W = a.shape
+    q1 = torch.empty(a.shape, dtype=torch.uint8, device=a.device)
+    q2 = torch.empty(a.shape, dtype=torch.uint8, device=a.device)
+    sc = torch.empty(C, dtype=torch.float32, device=a.device)
+    check(lib().vsc_quantize_fp8_pair(_ptr(a), _ptr(b), _ptr(q1), _ptr(q2), _ptr(sc), N, C, H, W, _stream()))
+    return q1, q2, sc
+
+
+def correlation_fp8(q1, q2, chan_scale, max_displacement=4, legacy=0):
+    """Correlation of e4m3 operands given as uint8 bit patterns."""
+    torch = _torch()
+    N, C, H, W = q1.shape
+    P = 2 * int(max_displacement) + 1
+    shape = (N, P * P, H, W) if legacy else (N, P, P, H, W)
+    out = torch.empty(shape, dtype=torch.float32, device=q1.device)
+    check(lib().vsc_correlation_forward(_ptr(q1.contiguous()), _ptr(q2.contiguous()), _ptr(out), N, C, H, W,
+                                        int(max_displacement), int(bool(legacy)), VSC_FP8E4M3,
+                                        _ptr(chan_scale.contiguous()), _stream()))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# GPUImage and the stabilization stages
+# ---------------------------------------------------------------------------
+class GPUImage:
+    """Owning HWC float32 device image (gpuimage.h:8-31).  ``data`` is a torch
+    tensor of shape (height, width, channels) on the current device."""
+
+    def __init__(self, width: int, height: int, channels: int, data=None):
+        torch = _torch()
+        self.width, self.height, self.channels = int(width), int(height), int(channels)
+        if data is None:
+            data = torch.empty((self.height, self.width, self.channels), dtype=torch.float32, device="cuda")
+        if tuple(data.shape) != (self.height, self.width, self.channels) or data.dtype != torch.float32:
+            raise VscError(-1, "GPUImage data must be float32 (height, width, channels)")
+        self.data = data.contiguous()
+
+    @classmethod
+    def from_tensor(cls, t):
+        h, w, c = t.shape
+        return cls(w, h, c, t)
+
+    @classmethod
+    def from_rgba(cls, rgba):
+        """copyFromQImage (gpuimage.cpp:103-124): RGBA8 (H, W, 4) uint8 device tensor."""
+        h, w, _ = rgba.shape
+        im = cls(w, h, 3)
+        check(lib().vsc_rgba_to_float(_ptr(rgba.contiguous()), im.c(), _stream()))
+        return im
+
+    def to_rgba(self):
+        """copyToQImage (gpuimage.cpp:127-135): RGBA8 (H, W, 4) uint8 device tensor."""
+        torch = _torch()
+        out = torch.empty((self.height, self.width, 4), dtype=torch.uint8, device=self.data.device)
+        check(lib().vsc_float_to_rgba(self.c(), _ptr(out), _stream()))
+        return out
+
+    def copy_from(self, other: "GPUImage"):
+        if (other.width, other.height, other.channels) != (self.width, self.height, self.channels):
+            raise VscError(-1, "Invalid dimensions")
+        self.data.copy_(other.data)
+
+    def c(self):
+        return VscImage(self.data.data_ptr(), self.width, self.height, self.channels)
+
+
+def get_warp_result(input: GPUImage, flow: GPUImage, input_warp: GPUImage):
+    check(lib().vsc_stab_warp(input.c(), flow.c(), input_warp.c(), _stream()))
+
+
+def get_adap_comb(crnt_in, crnt_pr, prev_warp_in, prev_warp_pr, next_warp_in, next_warp_pr, adap_cmb_in,
+                  adap_cmb_pr, last_stab_warp, alpha):
+    check(lib().vsc_stab_adap_comb(crnt_in.c(), crnt_pr.c(), prev_warp_in.c(), prev_warp_pr.c(),
+                                   next_warp_in.c(), next_warp_pr.c(), adap_cmb_in.c(), adap_cmb_pr.c(),
+                                   last_stab_warp.c(), float(alpha), _stream()))
+
+
+def get_consist_wt(adap_cmb_in, crnt_in, consis_wt, beta, gamma):
+    check(lib().vsc_stab_consist_wt(adap_cmb_in.c(), crnt_in.c(), consis_wt.c(), float(beta), float(gamma),
+                                    _stream()))
+
+
+def get_consist_out(crnt_pr, prev_stab_warp, cons_wt, num_iter, step_size, mom_fac, consis_out, workspace=None):
+    ws = None if workspace is None else _ptr(workspace)
+    check(lib().vsc_stab_consist_out(crnt_pr.c(), prev_stab_warp.c(), cons_wt.c(), int(num_iter),
+                                     float(step_size), float(mom_fac), consis_out.c(), ws, _stream()))
+
+
+def consist_out_workspace_bytes(width, height):
+    return lib().vsc_stab_consist_out_workspace_size(int(width), int(height))
+
+
+def get_bilinear(input: GPUImage, output: GPUImage):
+    check(lib().vsc_stab_bilinear(input.c(), output.c(), _stream()))
+
+
+def perform_consistency(original_curr, processed_prev, processed_curr, processed_next, flow_fwd, flow_bwd,
+                        stabilized_prev, stabilized_out):
+    """perform_consistency (flowconsistency.cu:269-286): only the average of the
+    previous and next processed frames is computed, as in the reference."""
+    check(lib().vsc_average_frames(processed_prev.c(), processed_next.c(), stabilized_out.c(), _stream()))
+
+
+# ---------------------------------------------------------------------------
+# VideoStabilizer
+# ---------------------------------------------------------------------------
+@dataclass
+class HyperParams:
+    """struct hyperParams (videostabilizer.h:38-46) with initHyperParams
+    defaults (videostabilizer.cpp:104-113)."""
+    alpha: float = 6800.0
+    beta: float = 6800.0
+    gamma: float = 2.0
+    pyramidLevels: int = 2
+    numIter: int = 150
+    stepSize: float = 0.15
+    momFac: float = 0.15
+
+    def c(self):
+        return VscParams(self.alpha, self.beta, self.gamma, self.pyramidLevels, self.numIter, self.stepSize,
+                         self.momFac)
+
+
+class VideoStabilizer:
+    """Per-stream stabilizer (videostabilizer.h:48-110) over the device-resident
+    state of ``vsc_stream``.  Frames are RGBA8 device tensors (H, W, 4); flows
+    are HWC float32 device tensors (fh, fw, 2|3), fwd = cur->next, bwd = cur->prev.
+    ``flow_width``/``flow_height`` below the frame size select the
+    FLOWDOWNSCALE path (flows upsampled with get_bilinear)."""
+
+    k = 1
+
+    def __init__(self, width, height, flow_width=None, flow_height=None, mode=VSC_FRAME_FUSED):
+        self.width, self.height = int(width), int(height)
+        self.flow_width = int(flow_width or width)
+        self.flow_height = int(flow_height or height)
+        self.mode = mode
+        self.controlParameters = HyperParams()
+        h = ctypes.c_void_p()
+        check(lib().vsc_stream_create(self.width, self.height, self.flow_width, self.flow_height, ctypes.byref(h)))
+        self._h = h
+
+    def getHyperParams(self) -> HyperParams:
+        return self.controlParameters
+
+    @staticmethod
+    def footprint(width, height, flow_width=None, flow_height=None):
+        return int(lib().vsc_stream_footprint(width, height, flow_width or width, flow_height or height))
+
+    def set_last_rgba(self, processed_rgba):
+        """preloadProcessedFrames: last <- processed frame 2k+B-1."""
+        check(lib().vsc_stream_set_last_rgba(self._h, _ptr(processed_rgba), _stream()))
+
+    def set_last(self, img: GPUImage):
+        check(lib().vsc_stream_set_last(self._h, img.c(), _stream()))
+
+    def last(self) -> GPUImage:
+        im = GPUImage(self.width, self.height, 3)
+        check(lib().vsc_stream_copy_last(self._h, im.c(), _stream()))
+        return im
+
+    def do_one_step(self, orig, proc, flow_fwd, flow_bwd, out_rgba=None, out_float=None, params=None):
+        """The stabilization body of doOneStep (videostabilizer.cpp:177-238).
+        orig, proc: sequences (prev, cur, next) of RGBA8 device tensors."""
+        torch = _torch()
+        fc = int(flow_fwd.shape[2])
+        fin = VscFrameIn(*(t.data_ptr() for t in (orig[0], orig[1], orig[2], proc[0], proc[1], proc[2],
+                                                     flow_fwd, flow_bwd)), fc)
+        if out_rgba is None:
+            out_rgba = torch.empty((self.height, self.width, 4), dtype=torch.uint8, device=orig[1].device)
+        prm = (params or self.controlParameters).c()
+        check(lib().vsc_stab_frame(self._h, ctypes.byref(fin), ctypes.byref(prm), _ptr(out_rgba),
+                                   None if out_float is None else _ptr(out_float), int(self.mode), _stream()))
+        return out_rgba
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vsc_stream_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def profile_begin(capacity=4096):
+    """Start timing the instrumented kernels with HIP events (see vsc.h)."""
+    check(lib().vsc_profile_begin(int(capacity)))
+
+
+def profile_end(kernel="consist_blocked"):
+    """Return (launches, total_ms, algorithmic_bytes) for one kernel."""
+    n = ctypes.c_int()
+    ms = ctypes.c_double()
+    by = ctypes.c_double()
+    check(lib().vsc_profile_end(kernel.encode(), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)))
+    return n.value, ms.value, by.value
+
+
+# ---------------------------------------------------------------------------
+# synthetic inputs
+# ---------------------------------------------------------------------------
+def _synth():
+    global _synth_lib
+    if _synth_lib is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise RuntimeError(f"{SYNTH_PATH} is missing: build with make -C video-stream-consistency_amd")
+        L = ctypes.CDLL(SYNTH_PATH)
+        L.vsc_synth_generate.argtypes = [ctypes.POINTER(SynthCfg), ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]
+        _synth_lib = L
+    return _synth_lib
+
+
+#: seed-generated workloads: small plain stabilization (640x360, 40 frames)
+#: and the full 1080p form (1920x1080, 300 frames)
+SYNTH_640x360 = dict(width=640, height=360, nframes=40, nblobs=200, sigma_min=5.0, sigma_max=40.0, motion=3.0,
+                     npatches=1, patch_size=64, gain_min=0.8, gain_max=1.2, offset_max=20.0, noise_sigma=4.0,
+                     seed=0)
+SYNTH_1080P = dict(width=1920, height=1080, nframes=300, nblobs=800, sigma_min=5.0, sigma_max=40.0, motion=8.0,
+                   npatches=4, patch_size=64, gain_min=0.8, gain_max=1.2, offset_max=20.0, noise_sigma=4.0,
+                   seed=2)
+
+
+def synth_video(**kw):
+    """Generate (orig, proc, fwd, bwd) numpy arrays: orig/proc uint8
+    [N, H, W, 4] RGBA, fwd/bwd float32 [N, H, W, 2] (fwd[t] maps t -> t+1,
+    bwd[t] maps t -> t-1; zero at the sequence ends)."""
+    cfg = dict(SYNTH_640x360)
+    cfg.update(kw)
+    c = SynthCfg(**cfg)
+    n, h, w = c.nframes, c.height, c.width
+    orig = np.empty((n, h, w, 4), np.uint8)
+    proc = np.empty((n, h, w, 4), np.uint8)
+    fwd = np.empty((n, h, w, 2), np.float32)
+    bwd = np.empty((n, h, w, 2), np.float32)
+    rc = _synth().vsc_synth_generate(ctypes.byref(c), orig.ctypes.data, proc.ctypes.data, fwd.ctypes.data,
+                                     bwd.ctypes.data)
+    if rc != 0:
+        raise VscError(rc, "synth_video: invalid configuration")
+    return orig, proc, fwd, bwd

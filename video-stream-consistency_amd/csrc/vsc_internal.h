@@ -1,0 +1,221 @@
+// Internal declarations shared by the HIP translation units of libvsc.
+// Parity-critical arithmetic: every TU is compiled with -ffp-contract=off and
+// additionally pins the contraction policy with the pragma below, so that no
+// a*b+c is fused into an FMA (the CPU oracle follows the same policy).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/vsc.h"
+
+#pragma clang fp contract(off)
+
+namespace vsc {
+
+// ---------------------------------------------------------------------------
+// error state (thread-local message behind vsc_last_error)
+// ---------------------------------------------------------------------------
+int set_error(int code, const std::string& msg);
+int check_launch(const char* what);
+int check_hip(hipError_t e, const char* what);
+
+inline hipStream_t as_stream(vsc_hip_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+// kernel timing (vsc_profile_begin/end): returns a slot index or -1 when off
+enum ProfKernel { PROF_CONSIST_BLOCKED = 0, PROF_FRONT_PASS = 1, PROF_NKERNELS = 2 };
+int prof_start(ProfKernel k, double algorithmic_bytes, hipStream_t s);
+void prof_stop(int slot, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// strided float image view: element (x, y, c) at d[c*cs + (y*w + x)*ps].
+//   HWC interleaved (the reference GPUImage layout): cs = 1, ps = channels
+//   planar (one plane per channel, used for solver state): cs = w*h, ps = 1
+// ---------------------------------------------------------------------------
+struct View {
+    float* d;
+    int w, h, ch;
+    long long cs, ps;
+    __host__ __device__ inline long long idx(int x, int y, int c) const {
+        return (long long)c * cs + ((long long)y * w + x) * ps;
+    }
+    __device__ inline float rd(int x, int y, int c) const { return d[idx(x, y, c)]; }
+    __device__ inline void wr(int x, int y, int c, float v) const { d[idx(x, y, c)] = v; }
+};
+
+inline View hwc(const vsc_image& im) {
+    return View{im.data, im.width, im.height, im.channels, 1, (long long)im.channels};
+}
+inline View planar(float* d, int w, int h, int ch) {
+    return View{d, w, h, ch, (long long)w * h, 1};
+}
+
+// RGBA8 source image: channel c of pixel (x, y) at p[(y*w + x)*4 + c], value
+// converted exactly as kernel_to_float_image (gpuimage.cu:49): the float
+// quotient v/255.0f equals (float)((double)v/255.0) for all 256 byte values.
+struct Rgba {
+    const uint8_t* p;
+    int w, h;
+    __device__ inline float rd(int x, int y, int c) const {
+        return (float)p[((long long)y * w + x) * 4 + c] / 255.0f;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// exp: Cody & Waite (1980) reduction x = k*ln2 + r with a two-part ln2, then a
+// degree-7 Taylor polynomial for e^r (|r| <= ~0.35), Horner form, and the
+// scale 2^k through the exponent field.  Only binary32 +, -, *, rint: identical
+// results on every IEEE device that does not contract (see the pragma above).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline float vexp(float x)
+{
+    if (x != x) return x;
+    if (x > 88.72283935546875f) return __builtin_inff();
+    if (x < -103.97208404541015625f) return 0.0f;
+    const float log2e = 1.44269502162933349609375f;
+    const float ln2_hi = 0.693145751953125f;
+    const float ln2_lo = 1.428606765330187045e-06f;
+    float kf = __builtin_rintf(x * log2e);
+    int k = (int)kf;
+    float r = x - kf * ln2_hi;
+    r = r - kf * ln2_lo;
+    float p = 1.0f / 5040.0f;
+    p = p * r + 1.0f / 720.0f;
+    p = p * r + 1.0f / 120.0f;
+    p = p * r + 1.0f / 24.0f;
+    p = p * r + 1.0f / 6.0f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    if (k > 127) { p = p * 2.0f; k -= 1; }
+    if (k < -126) { p = p * 5.42101086242752217003726400434970855712890625e-20f; k += 64; }
+    return p * __builtin_bit_cast(float, (uint32_t)(k + 127) << 23);
+}
+
+// ---------------------------------------------------------------------------
+// Stage formulas, one pixel at a time, in the reference's operation order.
+// ---------------------------------------------------------------------------
+
+// kernel_warp (flowconsistency.cu:85-102): sample position, clamped to
+// [0, W-3] x [0, H-3], split into integer corner and fraction.
+struct WarpTap {
+    int ix, iy;
+    float fx, fy;
+};
+__device__ inline WarpTap warp_tap(int x, int y, float flo_x, float flo_y, int W, int H)
+{
+    const float mfx = fmaxf(0.0f, fminf((float)x + flo_x, (float)(W - 3)));
+    const float mfy = fmaxf(0.0f, fminf((float)y + flo_y, (float)(H - 3)));
+    WarpTap t;
+    t.ix = (int)floorf(mfx);
+    t.iy = (int)floorf(mfy);
+    t.fx = mfx - (float)t.ix;
+    t.fy = mfy - (float)t.iy;
+    return t;
+}
+
+// kernel_warp (flowconsistency.cu:104-113): bilinear from the four corners.
+template <class Img>
+__device__ inline float warp_sample(const Img& im, const WarpTap& t, int c)
+{
+    const float t1 = im.rd(t.ix, t.iy, c) * (1.0f - t.fx) + im.rd(t.ix + 1, t.iy, c) * t.fx;
+    const float t2 = im.rd(t.ix, t.iy + 1, c) * (1.0f - t.fx) + im.rd(t.ix + 1, t.iy + 1, c) * t.fx;
+    return t1 * (1.0f - t.fy) + t2 * t.fy;
+}
+
+// kernel_adap_comb (flowconsistency.cu:145-161) for one channel value.
+__device__ inline void adap_comb(float ci, float cp, float pi, float pp, float ni, float np, float ls, float alpha,
+                                 float& adIn, float& adPr)
+{
+    float wp = vexp(((-alpha) * (ci - pi)) * (ci - pi));
+    float wn = vexp(((-alpha) * (ci - ni)) * (ci - ni));
+    if (wp > 0.45f) wp = 0.45f;
+    if (wn > 0.3f) wn = 0.3f;
+    if (wp < 0.001f) wp = 0.0f;
+    if (wn < 0.001f) wn = 0.0f;
+    adIn = (wp * pi + wn * ni) + (1.0f - (wp + wn)) * ci;
+    float ap = (wp * pp + wn * np) + (1.0f - (wp + wn)) * cp;
+    adPr = wp * ls + (1.0f - wp) * ap;
+}
+
+// kernel_consist_wt (flowconsistency.cu:177-188).
+__device__ inline float consist_wt(float ci, float adIn, float beta, float gamma)
+{
+    float w = gamma * vexp(((-beta) * (ci - adIn)) * (ci - adIn));
+    if (w < 0.001f) w = 0.0f;
+    return w;
+}
+
+// kernel_bilinear (flowconsistency.cu:58-74): source coordinate and fraction.
+struct BiTap {
+    int ix, iy, ix1, iy1;
+    float fx, fy;
+};
+__device__ inline BiTap bilinear_tap(int ox, int oy, int inW, int inH, int outW, int outH)
+{
+    const float xx = ((float)ox * (float)inW) / (float)outW;
+    const float yy = ((float)oy * (float)inH) / (float)outH;
+    BiTap t;
+    t.ix = (int)floorf(xx);
+    t.iy = (int)floorf(yy);
+    t.fx = xx - (float)t.ix;
+    t.fy = yy - (float)t.iy;
+    t.ix1 = min(t.ix + 1, inW - 1);
+    t.iy1 = min(t.iy + 1, inH - 1);
+    return t;
+}
+template <class Img>
+__device__ inline float bilinear_sample(const Img& im, const BiTap& t, int c)
+{
+    const float v00 = im.rd(t.ix, t.iy, c);
+    const float v10 = im.rd(t.ix1, t.iy, c);
+    const float v01 = im.rd(t.ix, t.iy1, c);
+    const float v11 = im.rd(t.ix1, t.iy1, c);
+    return (((v00 * (1.0f - t.fx)) * (1.0f - t.fy) + (v10 * t.fx) * (1.0f - t.fy)) + (v01 * (1.0f - t.fx)) * t.fy) +
+           (v11 * t.fx) * t.fy;
+}
+
+// kernel_to_char_image (gpuimage.cu:62-65): floor(|v|*255) to uint32 with
+// saturation (NaN -> 0), then the unsigned char cast keeps the low byte.
+__device__ inline uint8_t to_u8(float v)
+{
+    const float f = fabsf(v) * 255.0f;
+    uint32_t u;
+    if (!(f == f)) u = 0u;
+    else if (f >= 4294967296.0f) u = 0xFFFFFFFFu;
+    else u = (uint32_t)floorf(f);
+    return (uint8_t)(u & 0xFFu);
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers implemented in stab.hip and used by the stream driver
+// ---------------------------------------------------------------------------
+int launch_stab_warp(View in_f, const uint8_t* in_rgba, View flow, View out, hipStream_t s);
+int launch_bilinear(View in, View out, hipStream_t s);
+int launch_consist_sweep(View P, View A, View Wt, View Oin, View Oout, View U, float step, float mom, int use_mom,
+                         hipStream_t s);
+int launch_consist_blocked(const float* P, const float* A, const float* Wt, const float* Oin, float* Oout,
+                           const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step,
+                           float mom, hipStream_t s);
+int consist_blocked_halo();
+int launch_to_rgba(View in, uint8_t* out, hipStream_t s);
+int launch_rgba_to_view(const uint8_t* in, View out, hipStream_t s);
+int launch_copy_view(View in, View out, hipStream_t s);
+
+struct FrontArgs {
+    Rgba oPrev, oCur, oNext, pPrev, pCur, pNext;
+    View last;      // previous stabilized frame (planar)
+    View fwd, bwd;  // flows (HWC, 2|3 channels) at frame size
+    View P0, A0, W0;          // level-0 outputs (planar)
+    View P1, A1, W1, O1;      // level-1 outputs (planar), written when decimate != 0
+    float alpha, beta, gamma;
+    int decimate;
+};
+int launch_front_pass(const FrontArgs& a, hipStream_t s);
+
+// reduced-precision (BF16 / F16 / FP8 e4m3) correlation on MFMA, corr_mfma.hip
+int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
+                       int in_dtype, const float* chan_scale, hipStream_t s);
+
+}  // namespace vsc
