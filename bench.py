@@ -34,7 +34,36 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 PSNR")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy kernel of known size: HBM traffic counters read for it
+    give the scale factor between counter units and bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(4):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4, "copies": 4}), flush=True)
+    return 0
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (copy-calibrated) for this workload, from the committed profile summary."""
+    path = os.path.join(ROOT, "profiles", "stabilize_1080p_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def psnr(a, b):
@@ -45,6 +74,8 @@ def psnr(a, b):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(streams shard across GPUs without communication; see the design notes)", file=sys.stderr)
@@ -125,7 +156,7 @@ def main():
         result["roofline"] = {
             "kernel": "k_consist_blocked (temporally blocked consistency solver)",
             "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile("k_consist_blocked"),
             "launches": n_solver, "avg_launch_us": round(1000.0 * ms_solver / n_solver, 3),
             "algorithmic_bytes_per_launch": round(by_solver / n_solver),
             "share_of_frame": round(ms_solver / float(lat.sum()), 4),
@@ -158,7 +189,7 @@ def main():
             if frames == 0:
                 cpu1 = r
             frames += 1
-            if time.perf_counter() - t0 >= args.cpu_seconds or frames >= 8:
+            if time.perf_counter() - t0 >= args.cpu_seconds or frames >= 200:
                 break
         cpu_s = time.perf_counter() - t0
         result["cpu_baseline"] = {"value": round(frames / cpu_s, 4), "unit": "frames/s", "cores": threads,

@@ -150,6 +150,17 @@ int vsc_stab_consist_out(vsc_image crnt_pr, vsc_image prev_stab_warp, vsc_image 
                          float step_size, float mom_fac, vsc_image consist_out, void* workspace,
                          vsc_hip_stream stream);
 
+/* get_consist_out on planar images: each argument is a W*H*3 float array
+ * holding three W x H planes (channel c at offset c*W*H), the layout the
+ * per-stream driver keeps its solver state in.  Runs the temporally blocked
+ * solver; same results as vsc_stab_consist_out.  workspace: device scratch of
+ * vsc_stab_consist_out_planar_workspace_size() bytes, or NULL to let the call
+ * allocate it stream-ordered. */
+size_t vsc_stab_consist_out_planar_workspace_size(int width, int height);
+int vsc_stab_consist_out_planar(const float* crnt_pr, const float* prev_stab_warp, const float* consist_wt,
+                                int width, int height, int num_iter, float step_size, float mom_fac,
+                                float* consist_out, void* workspace, vsc_hip_stream stream);
+
 /* get_bilinear (flowconsistency.cu:50-75,288-299): resample in -> out. */
 int vsc_stab_bilinear(vsc_image in, vsc_image out, vsc_hip_stream stream);
 
@@ -193,7 +204,7 @@ int vsc_stream_set_last(vsc_stream* s, vsc_image last, vsc_hip_stream stream);
 int vsc_stream_copy_last(vsc_stream* s, vsc_image out, vsc_hip_stream stream);
 
 typedef struct {
-    /* RGBA8 device frames (width*height*4 bytes): the k=1 window. */
+    /* RGBA8 device frames (width*height*4 bytes, 4-byte aligned): the k=1 window. */
     const uint8_t* orig_prev;
     const uint8_t* orig_cur;
     const uint8_t* orig_next;

@@ -204,3 +204,19 @@ def test_frame_errors(vsc, cuda):
     st.controlParameters.pyramidLevels = 2
     st.do_one_step([z, z, z], [z, z, z], f, f)
     st.close()
+
+
+@pytest.mark.parametrize("H,W,iters", [(37, 53, 19), (300, 500, 8), (1080, 1920, 9), (3, 700, 5)])
+def test_consist_out_planar_blocked(vsc, orc, cuda, H, W, iters):
+    """The temporally blocked solver on planar images, checked against the
+    oracle (HWC) over tile and launch boundaries."""
+    import torch
+    rng = np.random.default_rng(H + W)
+    P, A, O = (rng.random((H, W, 3)).astype(np.float32) for _ in range(3))
+    Wt = rng.uniform(0, 2, (H, W, 3)).astype(np.float32)
+    Wt[rng.random(Wt.shape) < 0.2] = 0
+    pl = lambda a: torch.from_numpy(np.ascontiguousarray(a.transpose(2, 0, 1))).to(cuda)
+    out = pl(O)
+    vsc.get_consist_out_planar(pl(P), pl(A), pl(Wt), iters, 0.15, 0.15, out)
+    got = out.cpu().numpy().transpose(1, 2, 0)
+    assert np.array_equal(got, orc.stab_consist_out(P, A, Wt, O, iters))

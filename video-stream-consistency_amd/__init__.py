@@ -42,7 +42,8 @@ EXPORTED = (
     "vsc_stab_consist_out", "vsc_stab_bilinear", "vsc_average_frames", "vsc_rgba_to_float",
     "vsc_float_to_rgba", "vsc_stream_create", "vsc_stream_destroy", "vsc_stream_footprint",
     "vsc_stream_set_last_rgba", "vsc_stream_set_last", "vsc_stream_copy_last", "vsc_stab_frame",
-    "vsc_profile_begin", "vsc_profile_end",
+    "vsc_profile_begin", "vsc_profile_end", "vsc_stab_consist_out_planar_workspace_size",
+    "vsc_stab_consist_out_planar",
 )
 
 
@@ -104,6 +105,9 @@ def lib():
         L.vsc_stab_consist_out_workspace_size.argtypes = [i, i]
         L.vsc_stab_consist_out_workspace_size.restype = ctypes.c_size_t
         L.vsc_stab_consist_out.argtypes = [img, img, img, i, f, f, img, vp, vp]
+        L.vsc_stab_consist_out_planar_workspace_size.argtypes = [i, i]
+        L.vsc_stab_consist_out_planar_workspace_size.restype = ctypes.c_size_t
+        L.vsc_stab_consist_out_planar.argtypes = [vp, vp, vp, i, i, i, f, f, vp, vp, vp]
         L.vsc_stab_bilinear.argtypes = [img, img, vp]
         L.vsc_average_frames.argtypes = [img, img, img, vp]
         L.vsc_rgba_to_float.argtypes = [vp, img, vp]
@@ -294,6 +298,17 @@ def get_consist_out(crnt_pr, prev_stab_warp, cons_wt, num_iter, step_size, mom_f
     ws = None if workspace is None else _ptr(workspace)
     check(lib().vsc_stab_consist_out(crnt_pr.c(), prev_stab_warp.c(), cons_wt.c(), int(num_iter),
                                      float(step_size), float(mom_fac), consis_out.c(), ws, _stream()))
+
+
+def get_consist_out_planar(crnt_pr, prev_stab_warp, cons_wt, num_iter, step_size, mom_fac, consis_out,
+                           workspace=None):
+    """get_consist_out on planar (3, H, W) float32 device tensors (the stream
+    driver's layout; temporally blocked solver)."""
+    _, h, w = consis_out.shape
+    ws = None if workspace is None else _ptr(workspace)
+    check(lib().vsc_stab_consist_out_planar(_ptr(crnt_pr), _ptr(prev_stab_warp), _ptr(cons_wt), w, h,
+                                            int(num_iter), float(step_size), float(mom_fac), _ptr(consis_out),
+                                            ws, _stream()))
 
 
 def consist_out_workspace_bytes(width, height):

@@ -194,39 +194,106 @@ int launch_consist_sweep(View P, View A, View Wt, View Oin, View Oout, View U, f
 // ---------------------------------------------------------------------------
 // consistency solver, temporally blocked (planar state, one channel per block)
 //
-// A 512-thread block owns a TW x TH tile of one channel plane (TW = 128
-// columns, TH = 4*R rows) that overlaps its neighbours by a halo of HALO
+// A block of 128*STRIPS threads owns a TW x TH tile of one channel plane
+// (TW = 128 columns, TH = STRIPS*R rows) that overlaps its neighbours by a halo of HALO
 // pixels.  Thread (tx, strip) keeps the R pixels of column tx in rows
-// strip*R .. strip*R+R-1 in registers: O, U, lapP, A, w.  Each sweep writes O
-// to one of two LDS planes (ping-pong, one barrier per sweep), reads the left
-// and right neighbours from LDS and the up/down neighbours from its own
-// registers (LDS only at the strip ends).  O and U are read with halo from
-// the *in* planes and written to the *out* planes (neighbouring blocks read the
-// pixels a block writes).  After `sweeps` <= HALO sweeps the
+// strip*R .. strip*R+R-1 in registers: O, U, lapP, A, w.  O and U are read
+// with halo from the *in* planes and written to the *out* planes (neighbouring
+// blocks read the pixels a block writes).  After `sweeps` <= HALO sweeps the
 // inner (TW-2*HALO) x (TH-2*HALO) pixels are exact and written back; halo
 // pixels whose out-of-tile neighbours are missing never reach the output.
-// Arithmetic per pixel and sweep is that of k_consist_sweep (lapP is computed
-// once per launch with the same operations), so the result is identical.
+//
+// Neighbour exchange goes through LDS, stored column-major: the R values of
+// (strip, column) are contiguous with a padded stride RS = R+4 (RS/4 odd, so a
+// 16-byte access of 8 consecutive lanes covers all 32 banks once), and each
+// sweep writes its strip with 16-byte stores, then reads the left and right
+// strips with 16-byte loads issued one 4-row chunk ahead of their use.  The
+// up/down neighbours come from the thread's own registers (LDS only at the
+// strip ends).  Two LDS planes alternate (one barrier per sweep).
+//
+// Arithmetic per pixel and sweep is that of k_consist_sweep; lapP (the
+// Laplacian of the processed frame, a constant of the level) is an input,
+// computed once per level with the same operations, so the result is identical.
+// Waves whose 64 x R pixels all have four neighbours take a branch-free path;
+// the others evaluate the reference's neighbour conditions (flowconsistency.cu:215-236).
 // ---------------------------------------------------------------------------
 static constexpr int SB_TW = 128;
-static constexpr int SB_THREADS = 512;
 
-template <int R, int HALO>
-__global__ __launch_bounds__(SB_THREADS) void k_consist_blocked(const float* __restrict__ P, const float* __restrict__ A,
-                                                                const float* __restrict__ Wt, const float* Oin,
-                                                                float* Oout, const float* Uin, float* Uout, int W,
-                                                                int H, int sweeps, int first_sweep, float step,
-                                                                float mom)
+template <int R, bool MOM, bool INTERIOR>
+__device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const float (&a)[R], const float (&w)[R],
+                                            const float (&lp)[R], const float* colL, const float* colR,
+                                            float up_edge, float dn_edge, float step, float mom, bool hr, bool hl,
+                                            int rU, int rD)
+{
+    float4 l4 = *reinterpret_cast<const float4*>(colL);
+    float4 r4 = *reinterpret_cast<const float4*>(colR);
+    float up_old = up_edge;
+#pragma unroll
+    for (int k = 0; k < R / 4; ++k) {
+        float4 ln = l4, rn = r4;
+        if (k + 1 < R / 4) {
+            ln = *reinterpret_cast<const float4*>(colL + 4 * (k + 1));
+            rn = *reinterpret_cast<const float4*>(colR + 4 * (k + 1));
+        }
+        const float lv[4] = {l4.x, l4.y, l4.z, l4.w};
+        const float rv[4] = {r4.x, r4.y, r4.z, r4.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = 4 * k + j;
+            const float cur = o[r];
+            const float dn = (r < R - 1) ? o[r + 1] : dn_edge;
+            float lo;
+            if (INTERIOR) {
+                lo = rv[j] + lv[j];
+                lo += dn;
+                lo += up_old;
+                lo -= 4.0f * cur;
+            } else {
+                int cnt = 0;
+                lo = 0.0f;
+                if (hr) { lo += rv[j]; cnt += 1; }
+                if (hl) { lo += lv[j]; cnt += 1; }
+                if (r <= rD) { lo += dn; cnt += 1; }
+                if (r >= rU) { lo += up_old; cnt += 1; }
+                lo -= (float)cnt * cur;
+            }
+            const float t1 = w[r] * (cur - a[r]);
+            const float t2 = lo - lp[r];
+            const float g = t2 - t1;
+            const float sg = step * g;
+            o[r] = MOM ? (cur + sg) + mom * u[r] : cur + sg;
+            u[r] = sg;
+            up_old = cur;
+        }
+        l4 = ln;
+        r4 = rn;
+    }
+}
+
+template <int R, int HALO, int STRIPS>
+__global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
+    const float* __restrict__ LapP, const float* __restrict__ A, const float* __restrict__ Wt, const float* Oin,
+    float* Oout, const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step, float mom,
+    int tiles_x, int tiles_y, int per_xcd)
 {
     constexpr int TW = SB_TW;
-    constexpr int STRIPS = SB_THREADS / TW;
     constexpr int TH = STRIPS * R;
+    constexpr int RS = ((R / 4) % 2 == 0) ? R + 4 : R + 8;
     constexpr int OW = TW - 2 * HALO, OH = TH - 2 * HALO;
-    __shared__ float sO[2][TH][TW];
+    static_assert(R % 4 == 0 && (RS / 4) % 2 == 1, "strip stride must be an odd number of 16-byte groups");
+    __shared__ __attribute__((aligned(16))) float sO[2][STRIPS][TW][RS];
 
+    // XCD-aware tile order: blocks are dispatched to the 8 XCDs round robin
+    // (block b runs on XCD b % 8), so give XCD x the contiguous run of tiles
+    // [x*per_xcd, (x+1)*per_xcd) in raster order; neighbouring tiles then
+    // share that XCD's L2 for their overlapping halo reads.
+    const int tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (tile >= tiles_x * tiles_y * 3) return;
+    const int c = tile / (tiles_x * tiles_y);
+    const int bxy = tile - c * tiles_x * tiles_y;
+    const int bty = bxy / tiles_x, btx = bxy - bty * tiles_x;
     const long long plane = (long long)W * H;
-    const int c = blockIdx.z;
-    const float* __restrict__ Pc = P + c * plane;
+    const float* __restrict__ Lc = LapP + c * plane;
     const float* __restrict__ Ac = A + c * plane;
     const float* __restrict__ Wc = Wt + c * plane;
     const float* __restrict__ Oi = Oin + c * plane;
@@ -238,107 +305,78 @@ __global__ __launch_bounds__(SB_THREADS) void k_consist_blocked(const float* __r
     // TW is a multiple of the wave size, so a strip is wave-uniform: keep the
     // row bookkeeping in scalar registers.
     const int strip = __builtin_amdgcn_readfirstlane(threadIdx.x / TW);
+    const int wave_tx0 = __builtin_amdgcn_readfirstlane((threadIdx.x % TW) & ~63);
     const int ty0 = strip * R;
-    const int gx0 = blockIdx.x * OW - HALO;
-    const int gy0 = blockIdx.y * OH - HALO;
+    const int gx0 = btx * OW - HALO;
+    const int gy0 = bty * OH - HALO;
     const int gx = gx0 + tx;
     const bool colin = gx >= 0 && gx < W;
     const bool hr = (gx + 1) < (W - 1), hl = (gx - 1) >= 0;
-    const bool interior = gx0 >= 1 && gx0 + TW - 1 <= W - 3 && gy0 >= 1 && gy0 + TH - 1 <= H - 3;
+    const int gys = gy0 + ty0;
+    // every pixel of this wave (64 columns x R rows) has all four neighbours
+    const bool interior = (gx0 + wave_tx0 >= 1) && (gx0 + wave_tx0 + 63 <= W - 3) && (gys >= 1) &&
+                          (gys + R - 1 <= H - 3);
 
     float o[R], u[R], a[R], w[R], lp[R];
-    // 32-bit element offsets (a plane holds < 2^31 pixels); row r of the strip
-    // is at base + r*W, so no per-row address has to stay live.
-    const int base = (gy0 + ty0) * W + gx;
+    // Loads are unconditional, from clamped positions, so that all of them
+    // issue back to back: a tile pixel outside the image holds some finite
+    // in-image value that no in-image pixel ever reads (the neighbour sets
+    // exclude out-of-image positions) and that is never stored.
+    // 32-bit element offsets (a plane holds < 2^31 pixels).
+    const int gxc = min(max(gx, 0), W - 1);
+    const bool load_u = first_sweep > 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int gy = gy0 + ty0 + r;
-        const bool in = colin && gy >= 0 && gy < H;
-        const int i = base + r * W;
-        o[r] = in ? Oi[i] : 0.0f;
-        u[r] = (in && first_sweep > 0) ? Ui[i] : 0.0f;
-        a[r] = in ? Ac[i] : 0.0f;
-        w[r] = in ? Wc[i] : 0.0f;
-        float l = 0.0f;
-        if (in) {
-            int cnt = 0;
-            if (hr) { l += Pc[i + 1]; cnt += 1; }
-            if (hl) { l += Pc[i - 1]; cnt += 1; }
-            if ((gy + 1) < (H - 1)) { l += Pc[i + W]; cnt += 1; }
-            if ((gy - 1) >= 0) { l += Pc[i - W]; cnt += 1; }
-            l -= (float)cnt * Pc[i];
-        }
-        lp[r] = l;
+        const int gyc = min(max(gys + r, 0), H - 1);
+        const int i = gyc * W + gxc;
+        o[r] = Oi[i];
+        a[r] = Ac[i];
+        w[r] = Wc[i];
+        lp[r] = Lc[i];
+    }
+    if (load_u) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[r] = Ui[min(max(gys + r, 0), H - 1) * W + gxc];
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[r] = 0.0f;
     }
 
     const int txl = tx > 0 ? tx - 1 : 0;
     const int txr = tx < TW - 1 ? tx + 1 : TW - 1;
-    const int rowU = ty0 > 0 ? ty0 - 1 : 0;
-    const int rowD = ty0 + R < TH ? ty0 + R : TH - 1;
+    const int stU = strip > 0 ? strip - 1 : 0;
+    const int stD = strip < STRIPS - 1 ? strip + 1 : STRIPS - 1;
+    // row r has an up neighbour iff r >= rU and a down neighbour iff r <= rD
+    const int rU0 = 1 - gys;
+    const int rD0 = (H - 3) - gys;
 
     for (int s = 0; s < sweeps; ++s) {
-        float (*buf)[TW] = sO[s & 1];
+        float (*buf)[TW][RS] = sO[s & 1];
+        float* own = buf[strip][tx];
 #pragma unroll
-        for (int r = 0; r < R; ++r) buf[ty0 + r][tx] = o[r];
+        for (int k = 0; k < R / 4; ++k)
+            *reinterpret_cast<float4*>(own + 4 * k) = make_float4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
         __syncthreads();
-        const bool momk = (first_sweep + s) > 0;
-        float up_old = buf[rowU][tx];
-        const float below = buf[rowD][tx];
+        const float up_edge = buf[stU][tx][R - 1];
+        const float dn_edge = buf[stD][tx][0];
+        const float* colL = buf[strip][txl];
+        const float* colR = buf[strip][txr];
+        int rU = rU0, rD = rD0;
+        // opaque per sweep: keeps the compiler from hoisting 2*R row
+        // predicates out of the sweep loop into scalar registers
+        asm volatile("" : "+s"(rU), "+s"(rD));
+        const bool mom_on = (first_sweep + s) > 0;
         if (interior) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const float cur = o[r];
-                const float dn = (r < R - 1) ? o[r + 1] : below;
-                float lo = buf[ty0 + r][txr];
-                lo += buf[ty0 + r][txl];
-                lo += dn;
-                lo += up_old;
-                lo -= 4.0f * cur;
-                const float t1 = w[r] * (cur - a[r]);
-                const float t2 = lo - lp[r];
-                const float g = t2 - t1;
-                const float sg = step * g;
-                o[r] = momk ? (cur + sg) + mom * u[r] : cur + sg;
-                u[r] = sg;
-                up_old = cur;
-            }
+            if (mom_on) sweep_strip<R, true, true>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+            else sweep_strip<R, false, true>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
         } else {
-            // image-border tiles.  Column conditions are per lane; the row
-            // conditions reduce to two wave-uniform bounds: row r has an up
-            // neighbour iff r >= rU and a down neighbour iff r <= rD.
-            const int gys = gy0 + ty0;
-            int rU = 1 - gys;
-            int rD = (H - 3) - gys;
-            // opaque per sweep: keeps the compiler from hoisting 2*R row
-            // predicates out of the sweep loop into scalar registers
-            asm volatile("" : "+s"(rU), "+s"(rD));
-            const int ccol = (hr ? 1 : 0) + (hl ? 1 : 0);
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const float cur = o[r];
-                const float dn = (r < R - 1) ? o[r + 1] : below;
-                const float vr = buf[ty0 + r][txr];
-                const float vl = buf[ty0 + r][txl];
-                float lo = 0.0f;
-                lo = hr ? lo + vr : lo;
-                lo = hl ? lo + vl : lo;
-                int cnt = ccol;
-                if (r <= rD) { lo += dn; cnt += 1; }
-                if (r >= rU) { lo += up_old; cnt += 1; }
-                lo -= (float)cnt * cur;
-                const float t1 = w[r] * (cur - a[r]);
-                const float t2 = lo - lp[r];
-                const float g = t2 - t1;
-                const float sg = step * g;
-                o[r] = momk ? (cur + sg) + mom * u[r] : cur + sg;
-                u[r] = sg;
-                up_old = cur;
-            }
+            if (mom_on) sweep_strip<R, true, false>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+            else sweep_strip<R, false, false>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
         }
     }
 
     if (tx >= HALO && tx < TW - HALO && colin) {
-        const int wbase = (gy0 + ty0) * W + gx;
+        const int wbase = gys * W + gx;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int ty = ty0 + r;
@@ -351,64 +389,159 @@ __global__ __launch_bounds__(SB_THREADS) void k_consist_blocked(const float* __r
     }
 }
 
-static constexpr int SB_R = 24;
-static constexpr int SB_HALO = 8;
+// lapP of one planar level: the processed-frame half of the Laplacian in
+// kernel_consist_out (flowconsistency.cu:215-238), constant over the sweeps.
+__global__ __launch_bounds__(256) void k_lap(const float* __restrict__ P, float* __restrict__ L, int W, int H)
+{
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    const int c = blockIdx.z;
+    if (x >= W || y >= H) return;
+    const float* Pc = P + (long long)c * W * H;
+    const int i = y * W + x;
+    int cnt = 0;
+    float l = 0.0f;
+    if ((x + 1) < (W - 1)) { l += Pc[i + 1]; cnt += 1; }
+    if ((x - 1) >= 0) { l += Pc[i - 1]; cnt += 1; }
+    if ((y + 1) < (H - 1)) { l += Pc[i + W]; cnt += 1; }
+    if ((y - 1) >= 0) { l += Pc[i - W]; cnt += 1; }
+    l -= (float)cnt * Pc[i];
+    L[(long long)c * W * H + i] = l;
+}
+
+int launch_lap(const float* P, float* L, int W, int H, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lap, dim3((W + BX - 1) / BX, (H + BY - 1) / BY, 3), dim3(BX, BY), 0, s, P, L, W, H);
+    return check_launch("lap");
+}
+
+#ifndef VSC_SB_R
+#define VSC_SB_R 28
+#endif
+#ifndef VSC_SB_HALO
+#define VSC_SB_HALO 12
+#endif
+#ifndef VSC_SB_STRIPS
+#define VSC_SB_STRIPS 4
+#endif
+static constexpr int SB_R = VSC_SB_R;
+static constexpr int SB_HALO = VSC_SB_HALO;
+static constexpr int SB_STRIPS = VSC_SB_STRIPS;
 
 int consist_blocked_halo() { return SB_HALO; }
 
-int launch_consist_blocked(const float* P, const float* A, const float* Wt, const float* Oin, float* Oout,
+int launch_consist_blocked(const float* lapP, const float* A, const float* Wt, const float* Oin, float* Oout,
                            const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step,
                            float mom, hipStream_t s)
 {
     if (sweeps < 1 || sweeps > SB_HALO) return set_error(VSC_E_INVALID, "consist_blocked: sweeps out of range");
-    constexpr int TH = (SB_THREADS / SB_TW) * SB_R;
+    constexpr int TH = SB_STRIPS * SB_R;
     constexpr int OW = SB_TW - 2 * SB_HALO, OH = TH - 2 * SB_HALO;
-    dim3 grid((W + OW - 1) / OW, (H + OH - 1) / OH, 3);
-    // algorithmic bytes: read O, U, P, A, w and write O, U once per pixel and channel
+    const int tiles_x = (W + OW - 1) / OW, tiles_y = (H + OH - 1) / OH;
+    const int per_xcd = (tiles_x * tiles_y * 3 + 7) / 8;
+    // algorithmic bytes: read O, U, lapP, A, w and write O, U once per pixel and channel
     const int slot = prof_start(PROF_CONSIST_BLOCKED, 7.0 * 3 * 4 * (double)W * H, s);
-    hipLaunchKernelGGL((k_consist_blocked<SB_R, SB_HALO>), grid, dim3(SB_THREADS), 0, s, P, A, Wt, Oin, Oout, Uin, Uout,
-                       W, H, sweeps, first_sweep, step, mom);
+    hipLaunchKernelGGL((k_consist_blocked<SB_R, SB_HALO, SB_STRIPS>), dim3(8 * per_xcd), dim3(SB_TW * SB_STRIPS), 0, s,
+                       lapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, tiles_x, tiles_y,
+                       per_xcd);
     prof_stop(slot, s);
     return check_launch("consist_blocked");
 }
 
 // ---------------------------------------------------------------------------
 // fused front pass: 5 warps + adaptive combination + consistency weights for
-// one pixel (all 3 channels), reading the u8 frames, the flows and the previous
-// output once; writes level-0 P, A, w (planar) and, for an exact 2x pyramid
-// (even W and H: kernel_bilinear is then pure decimation, fx = fy = 0),
-// the level-1 P, A, w and the level-1 initial solution O1 = P1.
+// one pixel (all 3 channels), reading the RGBA8 frames, the flows and the
+// previous output once.  Writes level-0 lapP, A, w (planar; P itself when the
+// pyramid needs it) and, for an exact 2x pyramid (even W and H:
+// kernel_bilinear is then pure decimation, fx = fy = 0), the level-1 lapP, A,
+// w and the level-1 initial solution O1 = P1.  RGBA8 pixels are read as one
+// 32-bit word each.
 // ---------------------------------------------------------------------------
+struct Rgba32 {
+    const uint32_t* p;
+    int w;
+    __device__ inline uint32_t px(int x, int y) const { return p[y * w + x]; }
+    __device__ inline float ch(uint32_t v, int c) const { return u8_to_f32((v >> (8 * c)) & 0xFFu); }
+};
+
+// bilinear warp sample of all 3 channels of an RGBA8 frame (kernel_warp order)
+__device__ inline void warp_rgba(const Rgba32& im, const WarpTap& t, float out[3])
+{
+    const uint32_t v00 = im.px(t.ix, t.iy), v10 = im.px(t.ix + 1, t.iy);
+    const uint32_t v01 = im.px(t.ix, t.iy + 1), v11 = im.px(t.ix + 1, t.iy + 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float t1 = im.ch(v00, c) * (1.0f - t.fx) + im.ch(v10, c) * t.fx;
+        const float t2 = im.ch(v01, c) * (1.0f - t.fx) + im.ch(v11, c) * t.fx;
+        out[c] = t1 * (1.0f - t.fy) + t2 * t.fy;
+    }
+}
+
+// lapP of the processed frame at (x, y) of a level that samples the frame
+// every `st` pixels (st = 1: level 0; st = 2: the decimated level 1 of size
+// Wl x Hl), for all 3 channels.
+__device__ inline void lap_rgba(const Rgba32& im, int x, int y, int st, int xl, int yl, int Wl, int Hl, float out[3])
+{
+    const uint32_t v = im.px(x, y);
+    const bool hr = (xl + 1) < (Wl - 1), hl = (xl - 1) >= 0, hd = (yl + 1) < (Hl - 1), hu = (yl - 1) >= 0;
+    const uint32_t vr = hr ? im.px(x + st, y) : 0u, vl = hl ? im.px(x - st, y) : 0u;
+    const uint32_t vd = hd ? im.px(x, y + st) : 0u, vu = hu ? im.px(x, y - st) : 0u;
+    const int cnt = (hr ? 1 : 0) + (hl ? 1 : 0) + (hd ? 1 : 0) + (hu ? 1 : 0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float l = 0.0f;
+        if (hr) l += im.ch(vr, c);
+        if (hl) l += im.ch(vl, c);
+        if (hd) l += im.ch(vd, c);
+        if (hu) l += im.ch(vu, c);
+        l -= (float)cnt * im.ch(v, c);
+        out[c] = l;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_front(FrontArgs a)
 {
     const int x = blockIdx.x * BX + threadIdx.x;
     const int y = blockIdx.y * BY + threadIdx.y;
-    const int W = a.P0.w, H = a.P0.h;
+    const int W = a.A0.w, H = a.A0.h;
     if (x >= W || y >= H) return;
+    const Rgba32 oPrev{reinterpret_cast<const uint32_t*>(a.oPrev.p), W};
+    const Rgba32 oCur{reinterpret_cast<const uint32_t*>(a.oCur.p), W};
+    const Rgba32 oNext{reinterpret_cast<const uint32_t*>(a.oNext.p), W};
+    const Rgba32 pPrev{reinterpret_cast<const uint32_t*>(a.pPrev.p), W};
+    const Rgba32 pCur{reinterpret_cast<const uint32_t*>(a.pCur.p), W};
+    const Rgba32 pNext{reinterpret_cast<const uint32_t*>(a.pNext.p), W};
     const WarpTap tb = warp_tap(x, y, a.bwd.rd(x, y, 0), a.bwd.rd(x, y, 1), W, H);
     const WarpTap tf = warp_tap(x, y, a.fwd.rd(x, y, 0), a.fwd.rd(x, y, 1), W, H);
-    const bool dec = a.decimate && ((x | y) & 1) == 0;
+    float pwi[3], pwp[3], nwi[3], nwp[3], lap0[3];
+    warp_rgba(oPrev, tb, pwi);
+    warp_rgba(pPrev, tb, pwp);
+    warp_rgba(oNext, tf, nwi);
+    warp_rgba(pNext, tf, nwp);
+    lap_rgba(pCur, x, y, 1, x, y, W, H, lap0);
+    const uint32_t vci = oCur.px(x, y), vcp = pCur.px(x, y);
+    const bool dec = a.decimate && ((x | y) & 1) == 0 && (x >> 1) < a.A1.w && (y >> 1) < a.A1.h;
     const int x1 = x >> 1, y1 = y >> 1;
+    float lap1[3];
+    if (dec) lap_rgba(pCur, x, y, 2, x1, y1, a.A1.w, a.A1.h, lap1);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const float pwi = warp_sample(a.oPrev, tb, c);
-        const float pwp = warp_sample(a.pPrev, tb, c);
-        const float nwi = warp_sample(a.oNext, tf, c);
-        const float nwp = warp_sample(a.pNext, tf, c);
         const float lw = warp_sample(a.last, tb, c);
-        const float ci = a.oCur.rd(x, y, c);
-        const float cp = a.pCur.rd(x, y, c);
+        const float ci = oCur.ch(vci, c);
+        const float cp = pCur.ch(vcp, c);
         float adIn, adPr;
-        adap_comb(ci, cp, pwi, pwp, nwi, nwp, lw, a.alpha, adIn, adPr);
+        adap_comb(ci, cp, pwi[c], pwp[c], nwi[c], nwp[c], lw, a.alpha, adIn, adPr);
         const float wt = consist_wt(ci, adIn, a.beta, a.gamma);
-        a.P0.wr(x, y, c, cp);
+        a.L0.wr(x, y, c, lap0[c]);
         a.A0.wr(x, y, c, adPr);
         a.W0.wr(x, y, c, wt);
-        if (dec && x1 < a.P1.w && y1 < a.P1.h) {
-            a.P1.wr(x1, y1, c, cp);
+        if (a.write_p) a.P0.wr(x, y, c, cp);
+        if (dec) {
+            a.L1.wr(x1, y1, c, lap1[c]);
             a.A1.wr(x1, y1, c, adPr);
             a.W1.wr(x1, y1, c, wt);
             a.O1.wr(x1, y1, c, cp);
+            if (a.write_p) a.P1.wr(x1, y1, c, cp);
         }
     }
 }
@@ -416,12 +549,13 @@ __global__ __launch_bounds__(256) void k_front(FrontArgs a)
 int launch_front_pass(const FrontArgs& a, hipStream_t s)
 {
     // algorithmic bytes per pixel: 6 RGBA8 frames (24), previous output (12),
-    // 2 flows (2 * 4 * channels), level-0 P, A, w (36) and, when decimating,
-    // a quarter pixel of level-1 P, A, w, O (48 / 4)
-    const double px = (double)a.P0.w * a.P0.h;
-    const double bytes = px * (24.0 + 12.0 + 4.0 * (a.fwd.ch + a.bwd.ch) + 36.0 + (a.decimate ? 12.0 : 0.0));
+    // 2 flows (2 * 4 * channels), level-0 lapP, A, w (36) and, when decimating,
+    // a quarter pixel of level-1 lapP, A, w, O (48 / 4)
+    const double px = (double)a.A0.w * a.A0.h;
+    const double bytes = px * (24.0 + 12.0 + 4.0 * (a.fwd.ch + a.bwd.ch) + 36.0 + (a.decimate ? 12.0 : 0.0) +
+                               (a.write_p ? (a.decimate ? 15.0 : 12.0) : 0.0));
     const int slot = prof_start(PROF_FRONT_PASS, bytes, s);
-    hipLaunchKernelGGL(k_front, grid2d(a.P0.w, a.P0.h), dim3(BX, BY), 0, s, a);
+    hipLaunchKernelGGL(k_front, grid2d(a.A0.w, a.A0.h), dim3(BX, BY), 0, s, a);
     prof_stop(slot, s);
     return check_launch("front_pass");
 }
@@ -535,6 +669,56 @@ extern "C" int vsc_stab_consist_out(vsc_image crnt_pr, vsc_image prev_stab_warp,
     if (!rc && cur.d != consist_out.data) rc = launch_copy_view(cur, O0, s);
     if (owned) {
         int rc2 = check_hip(hipFreeAsync(ws, s), "vsc_stab_consist_out workspace free");
+        if (!rc) rc = rc2;
+    }
+    return rc;
+}
+
+extern "C" size_t vsc_stab_consist_out_planar_workspace_size(int width, int height)
+{
+    if (width < 1 || height < 1) return 0;
+    return (size_t)4 * width * height * 3 * sizeof(float);
+}
+
+extern "C" int vsc_stab_consist_out_planar(const float* crnt_pr, const float* prev_stab_warp, const float* consist_wt,
+                                           int width, int height, int num_iter, float step_size, float mom_fac,
+                                           float* consist_out, void* workspace, vsc_hip_stream stream)
+{
+    if (!crnt_pr || !prev_stab_warp || !consist_wt || !consist_out)
+        return set_error(VSC_E_INVALID, "vsc_stab_consist_out_planar: null pointer");
+    if (width < 1 || height < 1) return set_error(VSC_E_INVALID, "vsc_stab_consist_out_planar: empty image");
+    if (num_iter < 0) return set_error(VSC_E_INVALID, "vsc_stab_consist_out_planar: num_iter < 0");
+    if (num_iter == 0) return VSC_OK;
+    hipStream_t s = as_stream(stream);
+    const size_t n = (size_t)width * height * 3;
+    float* ws = static_cast<float*>(workspace);
+    bool owned = false;
+    int rc = VSC_OK;
+    if (!ws) {
+        rc = check_hip(hipMallocAsync(reinterpret_cast<void**>(&ws),
+                                      vsc_stab_consist_out_planar_workspace_size(width, height), s),
+                       "vsc_stab_consist_out_planar workspace");
+        if (rc) return rc;
+        owned = true;
+    }
+    float* O[2] = {consist_out, ws};
+    float* U[2] = {ws + n, ws + 2 * n};
+    float* lapP = ws + 3 * n;
+    rc = launch_lap(crnt_pr, lapP, width, height, s);
+    const int halo = consist_blocked_halo();
+    int cur = 0;
+    for (int k = 0; k < num_iter && !rc;) {
+        const int m = std::min(halo, num_iter - k);
+        rc = launch_consist_blocked(lapP, prev_stab_warp, consist_wt, O[cur], O[cur ^ 1], U[cur], U[cur ^ 1], width,
+                                    height, m, k, step_size, mom_fac, s);
+        cur ^= 1;
+        k += m;
+    }
+    if (!rc && cur == 1)
+        rc = check_hip(hipMemcpyAsync(consist_out, ws, n * sizeof(float), hipMemcpyDeviceToDevice, s),
+                       "vsc_stab_consist_out_planar copy");
+    if (owned) {
+        int rc2 = check_hip(hipFreeAsync(ws, s), "vsc_stab_consist_out_planar workspace free");
         if (!rc) rc = rc2;
     }
     return rc;

@@ -16,6 +16,7 @@ struct vsc_stream {
     int nlev = 0;                      // pyramid levels allocated (<= VSC_MAX_LEVELS)
     int lw[VSC_MAX_LEVELS] = {}, lh[VSC_MAX_LEVELS] = {};
     float* P[VSC_MAX_LEVELS] = {};     // planar 3-channel level images
+    float* L[VSC_MAX_LEVELS] = {};     // lapP: Laplacian of P, constant per level
     float* A[VSC_MAX_LEVELS] = {};
     float* Wt[VSC_MAX_LEVELS] = {};
     float* O[VSC_MAX_LEVELS][2] = {};  // solution ping-pong
@@ -50,7 +51,7 @@ size_t footprint(int W, int H, int fw, int fh, int* nlev_out = nullptr)
     int lw[VSC_MAX_LEVELS], lh[VSC_MAX_LEVELS];
     const int n = level_dims(W, H, lw, lh);
     size_t floats = 0;
-    for (int j = 0; j < n; ++j) floats += (size_t)7 * 3 * lw[j] * lh[j];
+    for (int j = 0; j < n; ++j) floats += (size_t)8 * 3 * lw[j] * lh[j];
     if (fw != W || fh != H) floats += (size_t)2 * 3 * W * H;
     if (nlev_out) *nlev_out = n;
     return floats * sizeof(float) + 64 * 16;
@@ -96,6 +97,7 @@ extern "C" int vsc_stream_create(int width, int height, int flow_width, int flow
     for (int j = 0; j < s->nlev; ++j) {
         const size_t n = (size_t)3 * s->lw[j] * s->lh[j];
         s->P[j] = take(n);
+        s->L[j] = take(n);
         s->A[j] = take(n);
         s->Wt[j] = take(n);
         s->O[j][0] = take(n);
@@ -164,6 +166,10 @@ int check_frame_in(const vsc_stream* s, const vsc_frame_in* in)
         if (!p) return set_error(VSC_E_INVALID, "vsc_stab_frame: null frame or flow pointer");
     if (in->flow_channels != 2 && in->flow_channels != 3)
         return set_error(VSC_E_INVALID, "vsc_stab_frame: flow_channels must be 2 or 3");
+    const void* frames[6] = {in->orig_prev, in->orig_cur, in->orig_next, in->proc_prev, in->proc_cur, in->proc_next};
+    for (const void* f : frames)
+        if (reinterpret_cast<uintptr_t>(f) % 4 != 0)
+            return set_error(VSC_E_INVALID, "vsc_stab_frame: RGBA8 frames must be 4-byte aligned");
     if (!s->has_last)
         return set_error(VSC_E_INVALID, "vsc_stab_frame: no previous stabilized frame (call vsc_stream_set_last*)");
     return VSC_OK;
@@ -217,7 +223,7 @@ int solve_pyramid(vsc_stream* s, int L, const vsc_params& p, hipStream_t st, int
         int ucur = 0;
         for (int k = 0; k < iters;) {
             const int n = std::min(halo, iters - k);
-            rc = launch_consist_blocked(s->P[j], s->A[j], s->Wt[j], s->O[j][cur], s->O[j][cur ^ 1], s->U[j][ucur],
+            rc = launch_consist_blocked(s->L[j], s->A[j], s->Wt[j], s->O[j][cur], s->O[j][cur ^ 1], s->U[j][ucur],
                                         s->U[j][ucur ^ 1], w, h, n, k, p.step_size, p.mom_fac, st);
             if (rc) return rc;
             cur ^= 1;
@@ -247,24 +253,31 @@ int frame_fused(vsc_stream* s, const vsc_frame_in* in, const vsc_params& p, uint
     a.last = pl(s->O[0][s->last_idx], W, H);
     a.fwd = fwd;
     a.bwd = bwd;
-    a.P0 = pl(s->P[0], W, H);
+    a.L0 = pl(s->L[0], W, H);
     a.A0 = pl(s->A[0], W, H);
     a.W0 = pl(s->Wt[0], W, H);
+    a.P0 = pl(s->P[0], W, H);
     // kernel_bilinear from W x H to W/2 x H/2 is exact decimation when W, H are even
     a.decimate = (L >= 2 && (W % 2) == 0 && (H % 2) == 0) ? 1 : 0;
+    // P itself is needed to initialise a single level or to build levels
+    // that the front pass does not decimate
+    a.write_p = (L == 1 || L > 2 || !a.decimate) ? 1 : 0;
     if (L >= 2) {
-        a.P1 = pl(s->P[1], s->lw[1], s->lh[1]);
-        a.A1 = pl(s->A[1], s->lw[1], s->lh[1]);
-        a.W1 = pl(s->Wt[1], s->lw[1], s->lh[1]);
-        a.O1 = pl(s->O[1][0], s->lw[1], s->lh[1]);
+        const int w1 = s->lw[1], h1 = s->lh[1];
+        a.L1 = pl(s->L[1], w1, h1);
+        a.A1 = pl(s->A[1], w1, h1);
+        a.W1 = pl(s->Wt[1], w1, h1);
+        a.O1 = pl(s->O[1][0], w1, h1);
+        a.P1 = pl(s->P[1], w1, h1);
     } else {
-        a.P1 = a.A1 = a.W1 = a.O1 = a.P0;
+        a.L1 = a.A1 = a.W1 = a.O1 = a.P1 = a.A0;
     }
     a.alpha = p.alpha;
     a.beta = p.beta;
     a.gamma = p.gamma;
     if ((rc = launch_front_pass(a, st))) return rc;
-    // remaining pyramid levels (videostabilizer.cpp:203-217): level j from j-1
+    // remaining pyramid levels (videostabilizer.cpp:203-217): level j from j-1,
+    // the initial solution O_j = bilinear(O_{j-1}) = bilinear(P_{j-1}) = P_j
     for (int j = 1; j < L; ++j) {
         if (j == 1 && a.decimate) continue;
         const int w0 = s->lw[j - 1], h0 = s->lh[j - 1], w1 = s->lw[j], h1 = s->lh[j];
@@ -272,6 +285,7 @@ int frame_fused(vsc_stream* s, const vsc_frame_in* in, const vsc_params& p, uint
         if ((rc = launch_bilinear(pl(s->A[j - 1], w0, h0), pl(s->A[j], w1, h1), st))) return rc;
         if ((rc = launch_bilinear(pl(s->Wt[j - 1], w0, h0), pl(s->Wt[j], w1, h1), st))) return rc;
         if ((rc = launch_bilinear(pl(s->P[j - 1], w0, h0), pl(s->O[j][0], w1, h1), st))) return rc;
+        if ((rc = launch_lap(s->P[j], s->L[j], w1, h1, st))) return rc;
     }
     if (L == 1) {
         // single level: the solution starts at the processed frame (videostabilizer.cpp:209)

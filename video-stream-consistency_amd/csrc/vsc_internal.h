@@ -51,15 +51,25 @@ inline View planar(float* d, int w, int h, int ch) {
     return View{d, w, h, ch, (long long)w * h, 1};
 }
 
-// RGBA8 source image: channel c of pixel (x, y) at p[(y*w + x)*4 + c], value
-// converted exactly as kernel_to_float_image (gpuimage.cu:49): the float
-// quotient v/255.0f equals (float)((double)v/255.0) for all 256 byte values.
+// Byte to float as kernel_to_float_image (gpuimage.cu:49): (float)((double)v/255.0).
+// q = v * (1/255) is off by one ulp for 126 of the 256 values; one FMA
+// residual step (Markstein) rounds the quotient correctly, which for these
+// inputs equals the double quotient rounded to float (checked for all 256
+// values on the host with fmaf and on the device by the conversion tests).
+__host__ __device__ inline float u8_to_f32(uint32_t v)
+{
+    const float r = 1.0f / 255.0f;
+    const float fv = (float)v;
+    float q = fv * r;
+    const float e = __builtin_fmaf(-q, 255.0f, fv);
+    return __builtin_fmaf(e, r, q);
+}
+
+// RGBA8 source image: channel c of pixel (x, y) at p[(y*w + x)*4 + c].
 struct Rgba {
     const uint8_t* p;
     int w, h;
-    __device__ inline float rd(int x, int y, int c) const {
-        return (float)p[((long long)y * w + x) * 4 + c] / 255.0f;
-    }
+    __device__ inline float rd(int x, int y, int c) const { return u8_to_f32(p[((long long)y * w + x) * 4 + c]); }
 };
 
 // ---------------------------------------------------------------------------
@@ -195,7 +205,8 @@ int launch_stab_warp(View in_f, const uint8_t* in_rgba, View flow, View out, hip
 int launch_bilinear(View in, View out, hipStream_t s);
 int launch_consist_sweep(View P, View A, View Wt, View Oin, View Oout, View U, float step, float mom, int use_mom,
                          hipStream_t s);
-int launch_consist_blocked(const float* P, const float* A, const float* Wt, const float* Oin, float* Oout,
+int launch_lap(const float* P, float* L, int W, int H, hipStream_t s);
+int launch_consist_blocked(const float* lapP, const float* A, const float* Wt, const float* Oin, float* Oout,
                            const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step,
                            float mom, hipStream_t s);
 int consist_blocked_halo();
@@ -204,13 +215,14 @@ int launch_rgba_to_view(const uint8_t* in, View out, hipStream_t s);
 int launch_copy_view(View in, View out, hipStream_t s);
 
 struct FrontArgs {
-    Rgba oPrev, oCur, oNext, pPrev, pCur, pNext;
-    View last;      // previous stabilized frame (planar)
-    View fwd, bwd;  // flows (HWC, 2|3 channels) at frame size
-    View P0, A0, W0;          // level-0 outputs (planar)
-    View P1, A1, W1, O1;      // level-1 outputs (planar), written when decimate != 0
+    Rgba oPrev, oCur, oNext, pPrev, pCur, pNext;  // RGBA8 frames (4-byte aligned rows)
+    View last;                // previous stabilized frame (planar)
+    View fwd, bwd;            // flows (HWC, 2|3 channels) at frame size
+    View L0, A0, W0, P0;      // level-0 outputs (planar): lapP, A, w and (write_p) P
+    View L1, A1, W1, O1, P1;  // level-1 outputs (planar), written when decimate != 0
     float alpha, beta, gamma;
     int decimate;
+    int write_p;              // also write P0 (and P1): the pyramid needs them
 };
 int launch_front_pass(const FrontArgs& a, hipStream_t s);
 
