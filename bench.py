@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 PSNR")
+    ap.add_argument("--workload", choices=["stabilize", "ops"], default="stabilize",
+                    help="stabilize: 1080p frames (headline); ops: Correlation/Warp on the 1920x1088 pyramid")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -66,6 +68,105 @@ def traffic_from_profile(kernel):
     return None
 
 
+# feature pyramid of a 1920x1088 frame pair (PWCNet-light levels 1/4 .. 1/64)
+OPS_LEVELS = [(32, 272, 480), (64, 136, 240), (96, 68, 120), (128, 34, 60), (196, 17, 30)]
+
+
+def bench_ops(args):
+    """Correlation (FP32, BF16, FP16, FP8-E4M3) and Warp on the five pyramid
+    levels, batch 1, md = 4: per-call time (HIP events on the launch stream),
+    algorithmic GB/s and the error against the FP64 oracle."""
+    import importlib
+
+    import numpy as np
+    import torch
+
+    vsc = importlib.import_module("video-stream-consistency_amd")
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(0)
+    res = {"metric": "Correlation/Warp on a 1920x1088 feature pyramid (5 levels, md=4)", "unit": "us per pyramid",
+           "higher_is_better": False, "n_gpus": 1, "data": "synthetic N(0,1) features; flow U[-4,4] + smooth field",
+           "forms": {}}
+    feats = []
+    for C, H, W in OPS_LEVELS:
+        f1 = rng.standard_normal((1, C, H, W)).astype(np.float32)
+        f2 = rng.standard_normal((1, C, H, W)).astype(np.float32)
+        ys, xs = np.mgrid[0:H, 0:W]
+        fl = rng.uniform(-4, 4, (1, 2, H, W)) + np.stack([np.sin(xs / 7.0), np.cos(ys / 5.0)])[None]
+        feats.append((f1, f2, fl.astype(np.float32)))
+
+    def timed(fn, reps):
+        for _ in range(3):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) * 1000.0 / reps
+
+    orc = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        orc.set_threads(min(16, len(os.sched_getaffinity(0))))
+    for form in ("f32", "bf16", "f16", "fp8"):
+        tot_us, tot_bytes, worst = 0.0, 0.0, 0.0
+        levels = []
+        for (f1, f2, _), (C, H, W) in zip(feats, OPS_LEVELS):
+            a = torch.from_numpy(f1).to(dev)
+            b = torch.from_numpy(f2).to(dev)
+            eb = {"f32": 4, "bf16": 2, "f16": 2, "fp8": 1}[form]
+            if form == "f32":
+                fn = lambda: vsc.correlation(a, b, 4, 0)
+            elif form == "fp8":
+                q1, q2, sc = vsc.quantize_fp8_pair(a, b)
+                fn = lambda: vsc.correlation_fp8(q1, q2, sc, 4, 0)
+            else:
+                ca, cb = vsc.convert(a, form), vsc.convert(b, form)
+                fn = lambda: vsc.correlation(ca, cb, 4, 0)
+            us = timed(fn, args.steps)
+            by = H * W * (2 * C * eb + 81 * 4)
+            tot_us += us
+            tot_bytes += by
+            lv = {"C": C, "H": H, "W": W, "us": round(us, 2), "GBs": round(by / us / 1e3, 1)}
+            if orc is not None:
+                out = fn().cpu().numpy().astype(np.float64)
+                ref = orc.correlation(f1.astype(np.float64), f2.astype(np.float64), 4, dtype=np.float64)
+                bnd = orc.correlation(np.abs(f1).astype(np.float64), np.abs(f2).astype(np.float64), 4,
+                                      dtype=np.float64)
+                e = float(np.max(np.abs(out - ref) / (bnd + 1e-30)))
+                lv["max_err_over_sum_abs"] = e
+                lv["max_err_over_max_abs"] = float(np.max(np.abs(out - ref)) / np.max(np.abs(ref)))
+                worst = max(worst, e)
+            levels.append(lv)
+        res["forms"]["correlation_" + form] = {"us": round(tot_us, 2), "algorithmic_GBs": round(tot_bytes / tot_us / 1e3, 1),
+                                               "hbm_frac": round(tot_bytes / tot_us / 1e3 / HBM_PEAK_GBS, 4),
+                                               "max_err_over_sum_abs": worst if orc is not None else None,
+                                               "levels": levels}
+    tot_us, tot_bytes, worst = 0.0, 0.0, 0.0
+    levels = []
+    for (f1, _, fl), (C, H, W) in zip(feats, OPS_LEVELS):
+        a = torch.from_numpy(f1).to(dev)
+        f = torch.from_numpy(fl).to(dev)
+        us = timed(lambda: vsc.warp(a, f), args.steps)
+        by = H * W * (8 * C + 8)
+        tot_us += us
+        tot_bytes += by
+        lv = {"C": C, "H": H, "W": W, "us": round(us, 2), "GBs": round(by / us / 1e3, 1)}
+        if orc is not None:
+            out = vsc.warp(a, f).cpu().numpy().astype(np.float64)
+            ref = orc.warp(f1.astype(np.float64), fl.astype(np.float64), dtype=np.float64)
+            lv["max_abs_err"] = float(np.max(np.abs(out - ref)))
+            worst = max(worst, lv["max_abs_err"])
+        levels.append(lv)
+    res["forms"]["warp_f32"] = {"us": round(tot_us, 2), "algorithmic_GBs": round(tot_bytes / tot_us / 1e3, 1),
+                                "hbm_frac": round(tot_bytes / tot_us / 1e3 / HBM_PEAK_GBS, 4),
+                                "max_abs_err": worst if orc is not None else None, "levels": levels}
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def psnr(a, b):
     import numpy as np
     mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
@@ -80,6 +181,8 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(streams shard across GPUs without communication; see the design notes)", file=sys.stderr)
         return 2
+    if args.workload == "ops":
+        return bench_ops(args)
     import importlib
 
     import numpy as np

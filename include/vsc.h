@@ -90,24 +90,29 @@ int vsc_default_params(vsc_params* out);
  *               zero padding, md <= 4 (correlation_cuda.cu:182-331,359-360).
  * in_dtype F32 accumulates sequentially in c in FP32 (the reference CPU op's
  * order); BF16 / F16 / FP8E4M3 inputs are NCHW arrays of that element type
- * (raw bit patterns, 2/2/1 bytes) multiplied on MFMA with FP32 accumulation.
- * chan_scale (FP8 only, nullable = 1): per-channel dequantisation scale s_c so
- *   that out = sum_c s_c * q1_c * q2_c (SmoothQuant-style equalised operands).
+ * (raw bit patterns, 2/2/1 bytes; FP8 is OCP E4M3, bias 7) multiplied on MFMA
+ * with FP32 accumulation, max_displacement <= 4.
+ * scale (FP8 only, nullable = 1): device pointer to one float s, out = s *
+ *   sum_c q1_c * q2_c.  Per-channel scaling is folded into the operands by
+ *   vsc_quantize_fp8_pair (equalisation), so one factor dequantises.
  * Errors: VSC_E_INVALID for N,C,H,W < 1, md < 0, legacy with md > 4;
  *         VSC_E_DTYPE for an unknown in_dtype (the reference throws
  *         "Unsupported input type", correlation.cc:140-141).                  */
 int vsc_correlation_forward(const void* f1, const void* f2, void* out, int N, int C, int H, int W,
-                            int max_displacement, int legacy, int in_dtype, const float* chan_scale,
+                            int max_displacement, int legacy, int in_dtype, const float* scale,
                             vsc_hip_stream stream);
 
-/* Host-side quantiser for the FP8 e4m3 path: SmoothQuant equalisation
- * r_c = sqrt(max|f2_c| / max|f1_c|), f1' = f1*r_c, f2' = f2/r_c, then one
- * per-tensor e4m3 scale per operand.  Writes q1, q2 (device, 1 byte/elem) and
- * chan_scale[c] = s1*s2 (device, C floats).  Device-side, asynchronous.    */
-int vsc_quantize_fp8_pair(const float* f1, const float* f2, uint8_t* q1, uint8_t* q2, float* chan_scale,
+/* Quantiser for the FP8 path, on the device: per-channel equalisation
+ * r_c = sqrt(max|f2_c| / max|f1_c|) (SmoothQuant), f1' = f1*r_c, f2' = f2/r_c,
+ * which leaves every product f1_c*f2_c unchanged, then one per-tensor scale
+ * per operand (s1 = max|f1'|/448, s2 = max|f2'|/448) and round-to-nearest-even
+ * E4M3.  Writes q1, q2 (1 byte per element) and scale[0] = s1*s2.
+ * Allocates a small stream-ordered workspace.                               */
+int vsc_quantize_fp8_pair(const float* f1, const float* f2, uint8_t* q1, uint8_t* q2, float* scale,
                           int N, int C, int H, int W, vsc_hip_stream stream);
 
-/* Device-side converters for the 16-bit paths (round to nearest even). */
+/* FP32 -> BF16 / F16 / FP8E4M3 element conversion on the device, round to
+ * nearest even (FP8 saturates at +-448).                                   */
 int vsc_convert_f32(const float* in, void* out, size_t count, int out_dtype, vsc_hip_stream stream);
 
 /* ---------------------------------------------------------------------------

@@ -169,7 +169,7 @@ def _dtype_code(t):
     return _DT[t.dtype]
 
 
-def correlation(input1, input2, max_displacement: int = 4, legacy: int = 0, chan_scale=None, out=None):
+def correlation(input1, input2, max_displacement: int = 4, legacy: int = 0, scale=None, out=None):
     """``custom::Correlation`` (correlation.h:60-84; correlation.cc:45-143).
 
     input1, input2: NCHW device tensors of float32 / float64 / bfloat16 /
@@ -186,7 +186,7 @@ def correlation(input1, input2, max_displacement: int = 4, legacy: int = 0, chan
     shape = (N, P * P, H, W) if legacy else (N, P, P, H, W)
     if out is None:
         out = torch.empty(shape, dtype=odt, device=a.device)
-    cs = None if chan_scale is None else _ptr(chan_scale.contiguous())
+    cs = None if scale is None else _ptr(scale.contiguous())
     check(lib().vsc_correlation_forward(_ptr(a), _ptr(b), _ptr(out), N, C, H, W, int(max_displacement),
                                         int(bool(legacy)), _dtype_code(a), cs, _stream()))
     return out
@@ -207,20 +207,20 @@ def warp(input, flow, out=None):
 
 
 def quantize_fp8_pair(f1, f2):
-    """SmoothQuant-equalised e4m3 operands and per-channel scales for the FP8
-    correlation (returns q1, q2 as uint8 tensors of e4m3 bits, scale [C])."""
+    """Equalised E4M3 operands for the FP8 correlation: returns q1, q2 (uint8
+    tensors of E4M3 bits) and scale (1 float): f1.f2 ~= scale * q1.q2."""
     torch = _torch()
     a = f1.contiguous().float()
     b = f2.contiguous().float()
     N, C, H, W = a.shape
     q1 = torch.empty(a.shape, dtype=torch.uint8, device=a.device)
     q2 = torch.empty(a.shape, dtype=torch.uint8, device=a.device)
-    sc = torch.empty(C, dtype=torch.float32, device=a.device)
+    sc = torch.empty(1, dtype=torch.float32, device=a.device)
     check(lib().vsc_quantize_fp8_pair(_ptr(a), _ptr(b), _ptr(q1), _ptr(q2), _ptr(sc), N, C, H, W, _stream()))
     return q1, q2, sc
 
 
-def correlation_fp8(q1, q2, chan_scale, max_displacement=4, legacy=0):
+def correlation_fp8(q1, q2, scale, max_displacement=4, legacy=0):
     """Correlation of e4m3 operands given as uint8 bit patterns."""
     torch = _torch()
     N, C, H, W = q1.shape
@@ -229,7 +229,19 @@ def correlation_fp8(q1, q2, chan_scale, max_displacement=4, legacy=0):
     out = torch.empty(shape, dtype=torch.float32, device=q1.device)
     check(lib().vsc_correlation_forward(_ptr(q1.contiguous()), _ptr(q2.contiguous()), _ptr(out), N, C, H, W,
                                         int(max_displacement), int(bool(legacy)), VSC_FP8E4M3,
-                                        _ptr(chan_scale.contiguous()), _stream()))
+                                        _ptr(scale.contiguous()), _stream()))
+    return out
+
+
+def convert(x, dtype):
+    """FP32 device tensor -> "bf16" / "f16" (torch tensors of that type) or
+    "e4m3" (uint8 bit patterns), round to nearest even."""
+    torch = _torch()
+    x = x.contiguous().float()
+    code = {"bf16": VSC_BF16, "f16": VSC_F16, "e4m3": VSC_FP8E4M3}[dtype]
+    tdt = {"bf16": torch.bfloat16, "f16": torch.float16, "e4m3": torch.uint8}[dtype]
+    out = torch.empty(x.shape, dtype=tdt, device=x.device)
+    check(lib().vsc_convert_f32(_ptr(x), _ptr(out), x.numel(), code, _stream()))
     return out
 
 

@@ -1,25 +1,377 @@
-// Reduced-precision correlation on MFMA (BF16 / F16 / FP8 e4m3, FP32 accumulation).
+// Reduced-precision Correlation on MFMA (gfx950): BF16 / FP16 / FP8-e4m3
+// features, FP32 accumulation.  Same output layout and zero-padding semantics
+// as the reference op (correlation.cc:147-275; legacy: correlation_cuda.cu:182-331).
+//
+// Matrix form.  For output row h, a segment of 16 pixels w0..w0+15 and one
+// vertical displacement dy, the 16 x 24 product
+//     G[i][j] = sum_c f1[c][h][w0+i] * f2[c][h+dy][w0-4+j]
+// holds all horizontal displacements on its diagonals: out[dy][dx][h][w0+i] =
+// G[i][i+dx+4].  G is computed as two 16 x 16 MFMA tiles (columns 0..15 and
+// 8..23) with K = 32 channels per instruction (v_mfma_f32_16x16x32_{bf16,f16,
+// fp8_fp8}); 9 of every 16 columns are used, which is fine because the op is
+// bound by the 81 output planes it writes, not by the products.
+//
+// Block: 8 waves; wave q computes one 16-pixel segment (q & 1) of output row
+// h0 + q/2 for all (2*MD+1)^2 displacements, 72 FP32 accumulators per lane.  Channels
+// stream through LDS in chunks of 32: the f1 tile (4 rows x 32 px) and the f2
+// tile (12 rows x 40 px, MD-pixel halo, zeros outside the image) are stored
+// channels-last with a padded pixel stride, so that each lane's 8-channel MFMA
+// operand is one 16-byte (8-byte for FP8) LDS read and 8 lanes cover all banks.
+// After the channel loop, each wave moves the diagonals of its tiles through a
+// small LDS stage and writes every (dy, dx) output row as 32 consecutive floats.
 #include "vsc_internal.h"
 
+#include <algorithm>
+#include <type_traits>
+
 namespace vsc {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+static constexpr int CM_ROWS = 4;    // output rows per block (one per wave)
+static constexpr int CM_PX = 32;     // output pixels per row per block
+static constexpr int CM_HALO = 4;    // f2 halo (max displacement supported)
+static constexpr int CM_KC = 32;     // channels per chunk (one MFMA K step)
+
+// element type traits: bytes per element, LDS pixel stride (elements)
+template <int EB> struct CmLayout;
+template <> struct CmLayout<2> { static constexpr int STRIDE = 40; };   // 80 B: 20 dwords
+template <> struct CmLayout<1> { static constexpr int STRIDE = 40; };   // 40 B: 10 dwords
+
+enum CmKind { CM_BF16 = 0, CM_F16 = 1, CM_FP8 = 2 };
+
+template <int KIND>
+__device__ inline f32x4 cm_mfma(const void* a, const void* b, f32x4 acc)
+{
+    if constexpr (KIND == CM_BF16) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(a),
+                                                       *reinterpret_cast<const bf16x8*>(b), acc, 0, 0, 0);
+    } else if constexpr (KIND == CM_F16) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(*reinterpret_cast<const f16x8*>(a),
+                                                      *reinterpret_cast<const f16x8*>(b), acc, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(*reinterpret_cast<const long*>(a),
+                                                          *reinterpret_cast<const long*>(b), acc, 0, 0, 0);
+    }
+}
+
+template <int KIND, int MD, bool LEGACY>
+__global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v, const void* __restrict__ f2v,
+                                                   float* __restrict__ out, int C, int H, int W, const float* scale)
+{
+    constexpr int EB = (KIND == CM_FP8) ? 1 : 2;          // bytes per element
+    constexpr int ST = CmLayout<EB>::STRIDE;              // LDS pixel stride in elements
+    constexpr int P = 2 * MD + 1;
+    constexpr int BR = CM_ROWS + 2 * CM_HALO, BC = CM_PX + 2 * CM_HALO;
+    using E = typename std::conditional<EB == 2, uint16_t, uint8_t>::type;
+    // LDS: the operand tiles during the channel loop, then the whole output
+    // tile [row][plane][32 px] (the operand tiles are dead by then)
+    constexpr int A_BYTES = CM_ROWS * CM_PX * ST * EB, B_BYTES = BR * BC * ST * EB;
+    constexpr int OUT_BYTES = CM_ROWS * P * P * CM_PX * 4;
+    constexpr int LDS_BYTES = (A_BYTES + B_BYTES) > OUT_BYTES ? (A_BYTES + B_BYTES) : OUT_BYTES;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[LDS_BYTES];
+    E (*sA)[CM_PX][ST] = reinterpret_cast<E (*)[CM_PX][ST]>(lds);
+    E (*sB)[BC][ST] = reinterpret_cast<E (*)[BC][ST]>(lds + A_BYTES);
+    float (*stage)[P * P][CM_PX] = reinterpret_cast<float (*)[P * P][CM_PX]>(lds);
+
+    const E* f1 = static_cast<const E*>(f1v);
+    const E* f2 = static_cast<const E*>(f2v);
+    const int n = blockIdx.z;
+    const int h0 = blockIdx.y * CM_ROWS, w0 = blockIdx.x * CM_PX;
+    const long long HW = (long long)H * W;
+    const E* a = f1 + (long long)n * C * HW;
+    const E* b = f2 + (long long)n * C * HW;
+    const int tid = threadIdx.x;
+    const int q = tid / 64, lane = tid % 64;
+    const int qr = q >> 1, seg = q & 1;         // output row in the block, 16-pixel segment
+    const int lr = lane % 16, lg = lane / 16;   // operand row/column, K group
+
+    f32x4 acc[P][2];
+#pragma unroll
+    for (int d = 0; d < P; ++d)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) acc[d][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int c0 = 0; c0 < C; c0 += CM_KC) {
+        // stage the f1 tile and the f2 tile (channels-last, zero outside the
+        // image and past C).  Global loads are issued from clamped addresses in
+        // batches of SB before their LDS stores, a few memory round trips per
+        // chunk instead of one per element.
+        constexpr int N1 = CM_KC * CM_ROWS * CM_PX, N2 = CM_KC * BR * BC;
+        constexpr int L1 = N1 / 512, L2 = N2 / 512, SB = 15;
+        static_assert(N1 % 512 == 0 && N2 % 512 == 0 && L2 % SB == 0, "staging assumes whole rounds");
+        __syncthreads();
+        {
+            E r1[L1];
+#pragma unroll
+            for (int k = 0; k < L1; ++k) {
+                const int e = tid + 512 * k;
+                const int x = e % CM_PX, r = (e / CM_PX) % CM_ROWS, c = e / (CM_PX * CM_ROWS);
+                const int gh = min(h0 + r, H - 1), gw = min(w0 + x, W - 1), gc = min(c0 + c, C - 1);
+                const E v = a[gc * HW + (long long)gh * W + gw];
+                r1[k] = (c0 + c < C && h0 + r < H && w0 + x < W) ? v : (E)0;
+            }
+#pragma unroll
+            for (int k = 0; k < L1; ++k) {
+                const int e = tid + 512 * k;
+                sA[(e / CM_PX) % CM_ROWS][e % CM_PX][e / (CM_PX * CM_ROWS)] = r1[k];
+            }
+        }
+#pragma unroll
+        for (int k0 = 0; k0 < L2; k0 += SB) {
+            E r2[SB];
+#pragma unroll
+            for (int k = 0; k < SB; ++k) {
+                const int e = tid + 512 * (k0 + k);
+                const int x = e % BC, r = (e / BC) % BR, c = e / (BC * BR);
+                const int ghr = h0 - CM_HALO + r, gwr = w0 - CM_HALO + x;
+                const int gh = min(max(ghr, 0), H - 1), gw = min(max(gwr, 0), W - 1), gc = min(c0 + c, C - 1);
+                const E v = b[gc * HW + (long long)gh * W + gw];
+                r2[k] = (c0 + c < C && ghr >= 0 && ghr < H && gwr >= 0 && gwr < W) ? v : (E)0;
+            }
+#pragma unroll
+            for (int k = 0; k < SB; ++k) {
+                const int e = tid + 512 * (k0 + k);
+                sB[(e / BC) % BR][e % BC][e / (BC * BR)] = r2[k];
+            }
+        }
+        __syncthreads();
+        // A operand: pixel lr of this wave's segment, channels 8*lg .. 8*lg+7
+        const E* pa = &sA[qr][16 * seg + lr][8 * lg];
+#pragma unroll
+        for (int d = 0; d < P; ++d) {
+            const int br = qr + CM_HALO + d - MD;   // f2 row of displacement dy = d - MD
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const E* pb = &sB[br][16 * seg + 8 * t + lr][8 * lg];
+                acc[d][t] = cm_mfma<KIND>(pa, pb, acc[d][t]);
+            }
+        }
+    }
+
+    // FP8: one dequantisation factor s = s1*s2 (per-channel factors are folded
+    // into the operands by the equalisation, see vsc_quantize_fp8_pair)
+    const float sc = scale ? *scale : 1.0f;
+    // diagonals -> stage[row][plane][pixel]: lane holds G[4*lg + r][lr + 8*t]
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < P; ++d)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 4 * lg + r, j = lr + 8 * t;
+                const int dx = j - i - CM_HALO;
+                if (dx >= -MD && dx <= MD && !(t == 1 && j < 16)) {
+                    float v = acc[d][t][r];
+                    if (KIND == CM_FP8) v = v * sc;
+                    if (LEGACY) v = v / (float)C;
+                    stage[qr][d * P + dx + MD][16 * seg + i] = v;
+                }
+            }
+    __syncthreads();
+    // each group of 32 lanes writes one 128-byte row segment of one plane;
+    // the cost volume is not re-read by this kernel: non-temporal stores
+    for (int e = tid; e < CM_ROWS * P * P * CM_PX; e += 512) {
+        const int x = e % CM_PX, pl = (e / CM_PX) % (P * P), r = e / (CM_PX * P * P);
+        const int h = h0 + r, w = w0 + x;
+        if (h < H && w < W)
+            __builtin_nontemporal_store(stage[r][pl][x], out + ((long long)n * P * P + pl) * HW + (long long)h * W + w);
+    }
+}
+
+template <int KIND, int MD>
+static void launch_mfma(const void* f1, const void* f2, float* out, int N, int C, int H, int W, bool legacy,
+                        const float* scale, hipStream_t s)
+{
+    dim3 grid((W + CM_PX - 1) / CM_PX, (H + CM_ROWS - 1) / CM_ROWS, N);
+    if (legacy)
+        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, true>), grid, dim3(512), 0, s, f1, f2, out, C, H, W, scale);
+    else
+        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, false>), grid, dim3(512), 0, s, f1, f2, out, C, H, W, scale);
+}
+
+template <int KIND>
+static int dispatch_md(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, bool legacy,
+                       const float* scale, hipStream_t s)
+{
+    switch (md) {
+        case 0: launch_mfma<KIND, 0>(f1, f2, out, N, C, H, W, legacy, scale, s); break;
+        case 1: launch_mfma<KIND, 1>(f1, f2, out, N, C, H, W, legacy, scale, s); break;
+        case 2: launch_mfma<KIND, 2>(f1, f2, out, N, C, H, W, legacy, scale, s); break;
+        case 3: launch_mfma<KIND, 3>(f1, f2, out, N, C, H, W, legacy, scale, s); break;
+        case 4: launch_mfma<KIND, 4>(f1, f2, out, N, C, H, W, legacy, scale, s); break;
+        default:
+            return set_error(VSC_E_UNSUPPORTED,
+                             "vsc_correlation_forward: reduced-precision correlation supports max_displacement <= 4");
+    }
+    return check_launch("correlation mfma");
+}
 
 int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
                        int in_dtype, const float* chan_scale, hipStream_t s)
 {
-    (void)f1; (void)f2; (void)out; (void)N; (void)C; (void)H; (void)W; (void)md; (void)legacy; (void)in_dtype;
-    (void)chan_scale; (void)s;
-    return set_error(VSC_E_UNSUPPORTED, "reduced-precision correlation is not built yet");
+    if (in_dtype == VSC_BF16) return dispatch_md<CM_BF16>(f1, f2, out, N, C, H, W, md, legacy, nullptr, s);
+    if (in_dtype == VSC_F16) return dispatch_md<CM_F16>(f1, f2, out, N, C, H, W, md, legacy, nullptr, s);
+    if (in_dtype == VSC_FP8E4M3) return dispatch_md<CM_FP8>(f1, f2, out, N, C, H, W, md, legacy, chan_scale, s);
+    return set_error(VSC_E_DTYPE, "vsc_correlation_forward: unsupported input type");
+}
+
+// ---------------------------------------------------------------------------
+// conversions
+// ---------------------------------------------------------------------------
+// binary32 -> bfloat16, round to nearest even (NaN stays NaN)
+__device__ inline uint16_t f32_to_bf16(float x)
+{
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// binary32 -> OCP FP8 E4M3 (bias 7, max 448, no infinities), round to nearest
+// even, saturating to +-448; NaN -> 0x7F.
+__host__ __device__ inline uint8_t f32_to_e4m3(float x)
+{
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    const uint32_t sign = (u >> 24) & 0x80u;
+    const uint32_t au = u & 0x7FFFFFFFu;
+    if (au > 0x7F800000u) return 0x7F;
+    const float a = __builtin_bit_cast(float, au);
+    uint32_t code;
+    if (a >= 448.0f) {
+        code = 0x7E;
+    } else if (a < 0.015625f) {                       // below 2^-6: subnormal grid of 2^-9
+        code = (uint32_t)__builtin_rintf(a * 512.0f);  // exact scaling, RNE; 8 -> min normal 0x08
+    } else {
+        int e = (int)(au >> 23) - 127;                 // -6 .. 8
+        uint32_t mant = au & 0x7FFFFFu;
+        uint32_t keep = mant >> 20, rest = mant & 0xFFFFFu;
+        if (rest > 0x80000u || (rest == 0x80000u && (keep & 1u))) ++keep;
+        if (keep == 8u) { keep = 0; ++e; }
+        if (e > 8 || (e == 8 && keep == 7u)) code = 0x7E;
+        else code = ((uint32_t)(e + 7) << 3) | keep;
+    }
+    return (uint8_t)(sign | code);
+}
+
+__global__ void k_to_bf16(const float* in, uint16_t* out, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f32_to_bf16(in[i]);
+}
+
+__global__ void k_to_f16(const float* in, _Float16* out, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (_Float16)in[i];
+}
+
+__global__ void k_to_e4m3(const float* in, uint8_t* out, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f32_to_e4m3(in[i]);
+}
+
+// ---------------------------------------------------------------------------
+// FP8 pair quantisation (SmoothQuant-style equalisation)
+// ---------------------------------------------------------------------------
+// per-channel max |f| over N, H, W, as ordered float bits (non-negative floats
+// order like their bit patterns), accumulated with atomicMax
+__global__ void k_chan_absmax(const float* f, uint32_t* mx, int N, int C, long long HW)
+{
+    const int c = blockIdx.y, n = blockIdx.z;
+    const float* p = f + ((long long)n * C + c) * HW;
+    float m = 0.0f;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < HW; i += (long long)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(p[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(&mx[c], __builtin_bit_cast(uint32_t, m));
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// r_c = sqrt(max2_c / max1_c) (1 when either is 0); s1 = max_c(max1_c r_c)/448,
+// s2 = max_c(max2_c / r_c)/448; writes r[c], inv scales and scale = s1*s2.
+__global__ void k_fp8_scales(const uint32_t* mx1, const uint32_t* mx2, float* r, float* sc, int C)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float m1 = 0.0f, m2 = 0.0f;
+    for (int c = 0; c < C; ++c) {
+        const float a = __builtin_bit_cast(float, mx1[c]), b = __builtin_bit_cast(float, mx2[c]);
+        const float rc = (a > 0.0f && b > 0.0f) ? sqrtf(b / a) : 1.0f;
+        r[c] = rc;
+        m1 = fmaxf(m1, a * rc);
+        m2 = fmaxf(m2, b / rc);
+    }
+    const float s1 = m1 > 0.0f ? m1 / 448.0f : 1.0f;
+    const float s2 = m2 > 0.0f ? m2 / 448.0f : 1.0f;
+    sc[0] = s1 * s2;
+    sc[1] = s1;
+    sc[2] = s2;
+}
+
+__global__ void k_fp8_quant(const float* f, uint8_t* q, const float* r, const float* sc, int which, int C,
+                            long long HW, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)((i / HW) % C);
+    const float v = which == 0 ? (f[i] * r[c]) / sc[1] : (f[i] / r[c]) / sc[2];
+    q[i] = f32_to_e4m3(v);
 }
 
 }  // namespace vsc
 
-extern "C" int vsc_quantize_fp8_pair(const float*, const float*, uint8_t*, uint8_t*, float*, int, int, int, int,
-                                     vsc_hip_stream)
+using namespace vsc;
+
+extern "C" int vsc_quantize_fp8_pair(const float* f1, const float* f2, uint8_t* q1, uint8_t* q2, float* scale, int N,
+                                     int C, int H, int W, vsc_hip_stream stream)
 {
-    return vsc::set_error(VSC_E_UNSUPPORTED, "vsc_quantize_fp8_pair is not built yet");
+    if (!f1 || !f2 || !q1 || !q2 || !scale) return set_error(VSC_E_INVALID, "vsc_quantize_fp8_pair: null pointer");
+    if (N < 1 || C < 1 || H < 1 || W < 1) return set_error(VSC_E_INVALID, "vsc_quantize_fp8_pair: empty tensor");
+    hipStream_t s = as_stream(stream);
+    const long long HW = (long long)H * W, n = (long long)N * C * HW;
+    // workspace: 2*C maxima + C equalisation factors + 3 scales
+    void* ws = nullptr;
+    int rc = check_hip(hipMallocAsync(&ws, (size_t)(3 * C + 4) * 4, s), "vsc_quantize_fp8_pair workspace");
+    if (rc) return rc;
+    uint32_t* mx = static_cast<uint32_t*>(ws);
+    float* r = reinterpret_cast<float*>(mx + 2 * C);
+    float* sc = r + C;
+    hipLaunchKernelGGL(k_zero_u32, dim3((2 * C + 255) / 256), dim3(256), 0, s, mx, 2 * C);
+    const unsigned gx = (unsigned)std::min<long long>((HW + 255) / 256, 64);
+    hipLaunchKernelGGL(k_chan_absmax, dim3(gx, C, N), dim3(256), 0, s, f1, mx, N, C, HW);
+    hipLaunchKernelGGL(k_chan_absmax, dim3(gx, C, N), dim3(256), 0, s, f2, mx + C, N, C, HW);
+    hipLaunchKernelGGL(k_fp8_scales, dim3(1), dim3(64), 0, s, mx, mx + C, r, sc, C);
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_fp8_quant, dim3(g), dim3(256), 0, s, f1, q1, r, sc, 0, C, HW, n);
+    hipLaunchKernelGGL(k_fp8_quant, dim3(g), dim3(256), 0, s, f2, q2, r, sc, 1, C, HW, n);
+    rc = check_launch("vsc_quantize_fp8_pair");
+    if (!rc) rc = check_hip(hipMemcpyAsync(scale, sc, sizeof(float), hipMemcpyDeviceToDevice, s), "vsc_quantize_fp8_pair");
+    const int rc2 = check_hip(hipFreeAsync(ws, s), "vsc_quantize_fp8_pair workspace free");
+    return rc ? rc : rc2;
 }
 
-extern "C" int vsc_convert_f32(const float*, void*, size_t, int, vsc_hip_stream)
+extern "C" int vsc_convert_f32(const float* in, void* out, size_t count, int out_dtype, vsc_hip_stream stream)
 {
-    return vsc::set_error(VSC_E_UNSUPPORTED, "vsc_convert_f32 is not built yet");
+    if (!in || !out) return set_error(VSC_E_INVALID, "vsc_convert_f32: null pointer");
+    if (count == 0) return VSC_OK;
+    hipStream_t s = as_stream(stream);
+    const long long n = (long long)count;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (out_dtype == VSC_BF16)
+        hipLaunchKernelGGL(k_to_bf16, dim3(g), dim3(256), 0, s, in, static_cast<uint16_t*>(out), n);
+    else if (out_dtype == VSC_F16)
+        hipLaunchKernelGGL(k_to_f16, dim3(g), dim3(256), 0, s, in, static_cast<_Float16*>(out), n);
+    else if (out_dtype == VSC_FP8E4M3)
+        hipLaunchKernelGGL(k_to_e4m3, dim3(g), dim3(256), 0, s, in, static_cast<uint8_t*>(out), n);
+    else
+        return set_error(VSC_E_DTYPE, "vsc_convert_f32: out_dtype must be BF16, F16 or FP8E4M3");
+    return check_launch("vsc_convert_f32");
 }

@@ -9,65 +9,114 @@ namespace vsc {
 // Correlation, FP32, sequential-in-c accumulation (the order of the reference
 // CPU op, correlation.cc:163-177: *dst = 0; *dst += v1*v2 for c = 0..C-1).
 //
-// A 256-thread block owns a 64 x 4 tile of output pixels of one image and all
-// (2*MD+1)^2 displacements; thread (tx, ty) keeps the P*P accumulators of its
-// pixel in registers.  Channels are streamed in chunks of CC: the f2 tile with
-// an MD-pixel halo, (4+2MD) x (64+2MD) x CC, is staged in LDS with zeros
-// outside the image (an out-of-range term of the reference contributes nothing,
-// and acc + (+-0) == acc for the accumulator values that occur, so zero terms
-// leave every bit unchanged).  The transpose to the LDS layout replaces the
-// reference's channels-last rearrange kernels and their per-call cudaMalloc
-// (correlation_cuda.cu:32-95,365-370).  Output rows are written coalesced
-// along W for each displacement plane.
+// A 576-thread block owns a tile of 8 rows x 8*QX pixels of one image.  Wave
+// d (9 waves) handles vertical displacement dy = d - MD; its lane (row, quad)
+// keeps QX pixels x (2*MD+1) horizontal displacements in registers (QX = 4
+// for large images: 12 f2 loads from LDS feed 36 multiply-adds; QX = 1 for
+// small pyramid levels, where parallelism matters more than reuse).  Channels
+// stream through LDS in chunks (8 channels for QX = 4, 32 for QX = 1): the f1 tile and the f2 tile with a
+// 4-pixel halo, zeros outside the image (an out-of-range term of the
+// reference contributes nothing, and acc + (+-0) == acc for the accumulator
+// values that occur, so zero terms leave every bit unchanged).  Every thread
+// issues all its global loads of the next chunk before computing the current
+// one, so the chunk loop overlaps memory and arithmetic.  This staging replaces the reference's
+// channels-last rearrange kernels and their per-call cudaMalloc
+// (correlation_cuda.cu:32-95,365-370).  Each (dy, dx) output row segment of
+// 8*QX pixels is written contiguously.
 // ---------------------------------------------------------------------------
-static constexpr int CT_W = 64, CT_H = 4, CT_CC = 8;
+static constexpr int CF_TH = 8, CF_HALO = 4;
 
-template <int MD, bool LEGACY>
-__global__ __launch_bounds__(256) void k_corr_f32(const float* __restrict__ f1, const float* __restrict__ f2,
+template <int MD, bool LEGACY, int QX, int CC>
+__global__ __launch_bounds__(576) void k_corr_f32(const float* __restrict__ f1, const float* __restrict__ f2,
                                                   float* __restrict__ out, int C, int H, int W)
 {
     constexpr int P = 2 * MD + 1;
-    constexpr int SW = CT_W + 2 * MD, SH = CT_H + 2 * MD;
-    __shared__ float s2[CT_CC][SH][SW];
+    constexpr int TW = 8 * QX;
+    constexpr int SW = TW + 2 * CF_HALO, SH = CF_TH + 2 * CF_HALO;
+    constexpr int N1 = CC * CF_TH * TW, N2 = CC * SH * SW;
+    constexpr int L1 = (N1 + 575) / 576, L2 = (N2 + 575) / 576;
+    __shared__ __attribute__((aligned(16))) float s1[CC][CF_TH][TW];
+    __shared__ __attribute__((aligned(16))) float s2[CC][SH][SW];
 
     const int n = blockIdx.z;
-    const int tx = threadIdx.x % CT_W, ty = threadIdx.x / CT_W;
-    const int w0 = blockIdx.x * CT_W, h0 = blockIdx.y * CT_H;
-    const int w = w0 + tx, h = h0 + ty;
+    const int tid = threadIdx.x;
+    const int d = tid / 64, rem = tid % 64;
+    const int row = rem / 8, quad = rem % 8;
+    const int w0 = blockIdx.x * TW, h0 = blockIdx.y * CF_TH;
     const long long HW = (long long)H * W;
     const float* a = f1 + (long long)n * C * HW;
     const float* b = f2 + (long long)n * C * HW;
-    const bool valid = (w < W) && (h < H);
 
-    float acc[P * P];
+    float acc[QX][P];
 #pragma unroll
-    for (int d = 0; d < P * P; ++d) acc[d] = 0.0f;
+    for (int x = 0; x < QX; ++x)
+#pragma unroll
+        for (int k = 0; k < P; ++k) acc[x][k] = 0.0f;
 
-    for (int c0 = 0; c0 < C; c0 += CT_CC) {
+    // chunk staging: global loads from clamped addresses into registers (all
+    // in flight together), LDS stores after the previous chunk's compute
+    float r1[L1], r2[L2];
+    auto load_chunk = [&](int c0) {
+#pragma unroll
+        for (int k = 0; k < L1; ++k) {
+            const int e = tid + 576 * k;
+            const int x = e % TW, r = (e / TW) % CF_TH, c = e / (TW * CF_TH);
+            const int gh = min(h0 + r, H - 1), gw = min(w0 + x, W - 1), gc = min(c0 + c, C - 1);
+            const bool ok = e < N1 && c0 + c < C && h0 + r < H && w0 + x < W;
+            const float v = a[gc * HW + (long long)gh * W + gw];
+            r1[k] = ok ? v : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < L2; ++k) {
+            const int e = tid + 576 * k;
+            const int x = e % SW, r = (e / SW) % SH, c = e / (SW * SH);
+            const int ghr = h0 - CF_HALO + r, gwr = w0 - CF_HALO + x;
+            const int gh = min(max(ghr, 0), H - 1), gw = min(max(gwr, 0), W - 1), gc = min(c0 + c, C - 1);
+            const bool ok = e < N2 && c0 + c < C && ghr >= 0 && ghr < H && gwr >= 0 && gwr < W;
+            const float v = b[gc * HW + (long long)gh * W + gw];
+            r2[k] = ok ? v : 0.0f;
+        }
+    };
+    load_chunk(0);
+    for (int c0 = 0; c0 < C; c0 += CC) {
         __syncthreads();
-        for (int e = threadIdx.x; e < CT_CC * SH * SW; e += 256) {
-            const int cc = e / (SH * SW);
-            const int rem = e - cc * SH * SW;
-            const int r = rem / SW, q = rem - r * SW;
-            const int gh = h0 - MD + r, gw = w0 - MD + q, gc = c0 + cc;
-            float v = 0.0f;
-            if (gc < C && gh >= 0 && gh < H && gw >= 0 && gw < W) v = b[gc * HW + (long long)gh * W + gw];
-            s2[cc][r][q] = v;
+#pragma unroll
+        for (int k = 0; k < L1; ++k) {
+            const int e = tid + 576 * k;
+            if (e < N1) (&s1[0][0][0])[e] = r1[k];
+        }
+#pragma unroll
+        for (int k = 0; k < L2; ++k) {
+            const int e = tid + 576 * k;
+            if (e < N2) (&s2[0][0][0])[e] = r2[k];
         }
         __syncthreads();
-        const int cn = min(CT_CC, C - c0);
-        for (int cc = 0; cc < cn; ++cc) {
-            const float v1 = valid ? a[(c0 + cc) * HW + (long long)h * W + w] : 0.0f;
+        if (c0 + CC < C) load_chunk(c0 + CC);   // next chunk in flight during the compute
+        if (d < P) {
+            const int cn = min(CC, C - c0);
+            const int br = row + d - MD + CF_HALO;
+            for (int cc = 0; cc < cn; ++cc) {
+                float va[QX], vb[QX + 2 * MD];
 #pragma unroll
-            for (int dy = 0; dy < P; ++dy)
+                for (int x = 0; x < QX; ++x) va[x] = s1[cc][row][quad * QX + x];
 #pragma unroll
-                for (int dx = 0; dx < P; ++dx) acc[dy * P + dx] += v1 * s2[cc][ty + dy][tx + dx];
+                for (int x = 0; x < QX + 2 * MD; ++x) vb[x] = s2[cc][br][quad * QX + CF_HALO - MD + x];
+#pragma unroll
+                for (int x = 0; x < QX; ++x)
+#pragma unroll
+                    for (int k = 0; k < P; ++k) acc[x][k] += va[x] * vb[x + k];
+            }
         }
     }
-    if (!valid) return;
-    float* o = out + (long long)n * P * P * HW + (long long)h * W + w;
+    const int h = h0 + row;
+    if (d >= P || h >= H) return;
+    float* o = out + ((long long)n * P * P + (long long)d * P) * HW + (long long)h * W + w0 + quad * QX;
 #pragma unroll
-    for (int d = 0; d < P * P; ++d) o[d * HW] = LEGACY ? acc[d] / (float)C : acc[d];
+    for (int k = 0; k < P; ++k)
+#pragma unroll
+        for (int x = 0; x < QX; ++x)
+            if (w0 + quad * QX + x < W)
+                __builtin_nontemporal_store(LEGACY ? acc[x][k] / (float)C : acc[x][k], o + k * HW + x);
 }
 
 // Any displacement, FP32 or FP64: one thread per output element, sequential
@@ -97,33 +146,49 @@ __global__ __launch_bounds__(256) void k_corr_generic(const T* __restrict__ f1, 
     out[i] = LEGACY ? acc / (T)C : acc;
 }
 
+template <int MD, int QX, int CC>
+static void launch_corr_f32_q(const float* f1, const float* f2, float* out, int N, int C, int H, int W, bool legacy,
+                              hipStream_t s)
+{
+    dim3 grid((W + 8 * QX - 1) / (8 * QX), (H + CF_TH - 1) / CF_TH, N);
+    if (legacy)
+        hipLaunchKernelGGL((k_corr_f32<MD, true, QX, CC>), grid, dim3(576), 0, s, f1, f2, out, C, H, W);
+    else
+        hipLaunchKernelGGL((k_corr_f32<MD, false, QX, CC>), grid, dim3(576), 0, s, f1, f2, out, C, H, W);
+}
+
+// 4 pixels per thread when that still gives a full machine of blocks
 template <int MD>
 static void launch_corr_f32(const float* f1, const float* f2, float* out, int N, int C, int H, int W, bool legacy,
                             hipStream_t s)
 {
-    dim3 grid((W + CT_W - 1) / CT_W, (H + CT_H - 1) / CT_H, N);
-    if (legacy)
-        hipLaunchKernelGGL((k_corr_f32<MD, true>), grid, dim3(256), 0, s, f1, f2, out, C, H, W);
-    else
-        hipLaunchKernelGGL((k_corr_f32<MD, false>), grid, dim3(256), 0, s, f1, f2, out, C, H, W);
+    const long long blocks4 = (long long)((W + 31) / 32) * ((H + CF_TH - 1) / CF_TH) * N;
+    if (blocks4 >= 256) launch_corr_f32_q<MD, 4, 8>(f1, f2, out, N, C, H, W, legacy, s);
+    else launch_corr_f32_q<MD, 1, 32>(f1, f2, out, N, C, H, W, legacy, s);
 }
 
 // ---------------------------------------------------------------------------
-// Warp (warp.cc:71-134 / warp_cuda.cu:28-84).  One thread per output pixel of
-// one image: the flow is read and the bilinear weights and in-bounds mask are
-// computed once, then the C planes are gathered (the reference re-reads the
-// flow once per channel, grid.y = C).  FP32 weights; the reference evaluates
+// Warp (warp.cc:71-134 / warp_cuda.cu:28-84).  One thread per output pixel and
+// group of 8 channels: the flow is read and the bilinear weights and in-bounds
+// mask are computed once per group, then the 8 planes are gathered with
+// independent loads (the reference re-reads the flow once per channel,
+// grid.y = C, and serialises nothing else).  FP32 weights; the reference evaluates
 // weights in double for float tensors, so FP32 agrees to rounding, and the
 // keep/zero decision (double)mask > 0.999 becomes mask >= 0.999f, the same
 // threshold on the float grid (0.999f is the first float above 0.999).
 // ---------------------------------------------------------------------------
+static constexpr int WG_CH = 8;
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_warp(const T* __restrict__ in, const T* __restrict__ flow, T* __restrict__ out,
                                               int C, int H, int W)
 {
     const int x = blockIdx.x * 64 + threadIdx.x % 64;
     const int y = blockIdx.y * 4 + threadIdx.x / 64;
-    const int n = blockIdx.z;
+    const int groups = (C + WG_CH - 1) / WG_CH;
+    const int n = blockIdx.z / groups;
+    const int c0 = (blockIdx.z % groups) * WG_CH;
+    const int c1 = min(C, c0 + WG_CH);
     if (x >= W || y >= H) return;
     const long long HW = (long long)H * W;
     const long long p = (long long)y * W + x;
@@ -151,14 +216,15 @@ __global__ __launch_bounds__(256) void k_warp(const T* __restrict__ in, const T*
     const long long base = (long long)n * C * HW;
     T* o = out + base + p;
     if (!keep) {
-        for (int c = 0; c < C; ++c) o[c * HW] = (T)0;
+        for (int c = c0; c < c1; ++c) o[c * HW] = (T)0;
         return;
     }
     const int iL = (int)xL, iR = (int)xR, iT = (int)yT, iB = (int)yB;
     const long long oTL = (long long)iT * W + iL, oTR = (long long)iT * W + iR;
     const long long oBL = (long long)iB * W + iL, oBR = (long long)iB * W + iR;
     const T* src = in + base;
-    for (int c = 0; c < C; ++c) {
+#pragma unroll
+    for (int c = c0; c < c1; ++c) {
         const T* pl = src + c * HW;
         T v = (T)0;
         if (mT && mL) v = v + wTL * pl[oTL];
@@ -225,7 +291,7 @@ extern "C" int vsc_warp_forward(const void* in, const void* flow, void* out, int
 {
     if (!in || !flow || !out) return set_error(VSC_E_INVALID, "vsc_warp_forward: null pointer");
     if (N < 1 || C < 1 || H < 1 || W < 1) return set_error(VSC_E_INVALID, "vsc_warp_forward: empty tensor");
-    const dim3 grid((W + 63) / 64, (H + 3) / 4, N);
+    const dim3 grid((W + 63) / 64, (H + 3) / 4, N * ((C + WG_CH - 1) / WG_CH));
     hipStream_t s = as_stream(stream);
     if (dtype == VSC_F32) {
         hipLaunchKernelGGL(k_warp<float>, grid, dim3(256), 0, s, static_cast<const float*>(in),
