@@ -236,6 +236,70 @@ int vsc_stab_frame(vsc_stream* s, const vsc_frame_in* in, const vsc_params* para
                    float* out_float, int mode, vsc_hip_stream stream);
 
 /* ---------------------------------------------------------------------------
+ * Flow network building blocks on channels-last (NHWC) device tensors.  In
+ * the reference these layers are ONNX Conv nodes that ORT runs through cuDNN
+ * (pwcnet.py:186-241 extractor, 511-535 DecoderLight, 625-644 Refiner); here
+ * they are part of the library so the whole flow network runs on its kernels.
+ * A tensor slice is (pointer, pixel stride cs, channel offset c0): channel c
+ * of pixel (n, y, x) lives at p[((n*H + y)*W + x)*cs + c0 + c].             */
+
+/* 3x3 convolution, padding `pad`, stride 1|2, dilation d (stride 2 needs d = 1):
+ * out = act(sum_{ky,kx,ci} wt[co][ky][kx][ci] * in[y*s - pad + ky*d][x*s - pad + kx*d][ci]
+ *           + bias[co]) (+ add), act 0 none, 1 LeakyReLU(0.1), 2 ReLU.
+ * dtype VSC_F32 / VSC_BF16 / VSC_F16 for `in` and `wt` (FP32 accumulation on
+ * MFMA); out_dtype = dtype or VSC_F32; `add` (nullable) has out_dtype.  Cin,
+ * in_cs and in_c0 are multiples of 16 bytes' worth of elements (4 FP32 / 8
+ * 16-bit); out_c0 and out_cs are multiples of 4 when Cout >= 4.  bias nullable. */
+int vsc_conv3x3_nhwc(const void* in, int in_cs, int in_c0, int N, int H, int W, int Cin, const void* wt,
+                     const float* bias, void* out, int out_cs, int out_c0, int Cout, int stride, int dilation,
+                     int pad, int act, const void* add, int add_cs, int add_c0, int dtype, int out_dtype,
+                     vsc_hip_stream stream);
+
+/* ---------------------------------------------------------------------------
+ * Flow network: PWCNet-light ("pwcnet-my-light4dec", model_spec.py:32; the
+ * network of pwcnet.py:787-1014 with Extractor, DecoderLight at levels 6..3
+ * and Refiner) wrapped by the ONNX pre/post-processing of convert_onnx.py:
+ * RGBA8 frames in, HWC flow (u, v[, 0]) out at the frame resolution.  Replaces
+ * FlowModel::run's ORT session (flowmodel.cpp:121-168) for one frame pair.
+ * The trained weights are not distributed with the reference; the network is
+ * initialised from `seed` with the reference's initialisation (pwcnet.py:
+ * 887-891) or loaded with vsc_flownet_set_params.
+ *
+ * Features are cached per frame in `slots`: vsc_flownet_extract runs the
+ * extractor once per frame, vsc_flownet_flow runs the decoders for one or
+ * two (first, second) slot pairs in one batch (a frame's forward and backward
+ * flows).  downscale (FLOWDOWNSCALE, videostabilizer.cpp:74-94) resizes the
+ * frames by nearest sampling to round(W/f) x round(H/f) before the network;
+ * the flow is then produced at that size (flowmodel.cpp:126-133).
+ * dtype: VSC_F32 (plain form) or VSC_BF16 (BF16 storage and MFMA operands,
+ * FP32 accumulation).                                                       */
+typedef struct vsc_flownet vsc_flownet;
+
+int vsc_flownet_create(int frame_width, int frame_height, int downscale, int dtype, unsigned long long seed, int slots,
+                       vsc_flownet** out);
+int vsc_flownet_destroy(vsc_flownet* net);
+size_t vsc_flownet_footprint(int frame_width, int frame_height, int downscale, int dtype, int slots);
+/* flow resolution (round(W/f), round(H/f)) */
+int vsc_flownet_flow_size(const vsc_flownet* net, int* width, int* height);
+/* parameters as float32 in layer order (extractor 18 convs, upflow
+ * ConvTranspose2d of levels 5, 4, 3, decoder convs of levels 6, 5, 4, 3 (6 each),
+ * refiner 7 convs), each weight in torch layout then its bias. */
+size_t vsc_flownet_param_count(const vsc_flownet* net);
+int vsc_flownet_get_params(const vsc_flownet* net, float* host, size_t count);
+int vsc_flownet_set_params(vsc_flownet* net, const float* host, size_t count);
+/* rgba: device RGBA8 frame_width x frame_height (4-byte aligned) */
+int vsc_flownet_extract(vsc_flownet* net, const uint8_t* rgba, int slot, vsc_hip_stream stream);
+/* npairs 1|2; out[i]: device HWC float flow_width x flow_height x out_channels (2|3) */
+int vsc_flownet_flow(vsc_flownet* net, int npairs, const int* first_slots, const int* second_slots, float* const* out,
+                     int out_channels, vsc_hip_stream stream);
+/* Copy an FP32 flow of the last vsc_flownet_flow call (item < npairs) into
+ * dst ((H/2^level) x (W/2^level) x 2 device floats of the 64-padded frame):
+ * which 0 = upsampled flow entering level 5..3, 1 = decoder flow of level
+ * 6..3, 2 = refined flow (level 3).  For tests and diagnostics. */
+int vsc_flownet_intermediate(const vsc_flownet* net, int which, int level, int item, float* dst,
+                             vsc_hip_stream stream);
+
+/* ---------------------------------------------------------------------------
  * Per-kernel timing (replaces the per-stage QElapsedTimer accounting of
  * VideoStabilizer, videostabilizer.h:72-76 / videostabilizer.cpp:175,240-262,
  * with device-side timing).  Between begin and end, every launch of the

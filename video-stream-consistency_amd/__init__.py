@@ -43,7 +43,9 @@ EXPORTED = (
     "vsc_float_to_rgba", "vsc_stream_create", "vsc_stream_destroy", "vsc_stream_footprint",
     "vsc_stream_set_last_rgba", "vsc_stream_set_last", "vsc_stream_copy_last", "vsc_stab_frame",
     "vsc_profile_begin", "vsc_profile_end", "vsc_stab_consist_out_planar_workspace_size",
-    "vsc_stab_consist_out_planar",
+    "vsc_stab_consist_out_planar", "vsc_conv3x3_nhwc", "vsc_flownet_create", "vsc_flownet_destroy",
+    "vsc_flownet_footprint", "vsc_flownet_flow_size", "vsc_flownet_param_count", "vsc_flownet_get_params",
+    "vsc_flownet_set_params", "vsc_flownet_extract", "vsc_flownet_flow", "vsc_flownet_intermediate",
 )
 
 
@@ -120,6 +122,19 @@ def lib():
         L.vsc_stream_set_last.argtypes = [vp, img, vp]
         L.vsc_stream_copy_last.argtypes = [vp, img, vp]
         L.vsc_stab_frame.argtypes = [vp, ctypes.POINTER(VscFrameIn), ctypes.POINTER(VscParams), vp, vp, i, vp]
+        L.vsc_conv3x3_nhwc.argtypes = [vp, i, i, i, i, i, i, vp, vp, vp, i, i, i, i, i, i, i, vp, i, i, i, i, vp]
+        L.vsc_flownet_create.argtypes = [i, i, i, i, ctypes.c_ulonglong, i, ctypes.POINTER(vp)]
+        L.vsc_flownet_destroy.argtypes = [vp]
+        L.vsc_flownet_footprint.argtypes = [i, i, i, i, i]
+        L.vsc_flownet_footprint.restype = ctypes.c_size_t
+        L.vsc_flownet_flow_size.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.vsc_flownet_param_count.argtypes = [vp]
+        L.vsc_flownet_param_count.restype = ctypes.c_size_t
+        L.vsc_flownet_get_params.argtypes = [vp, vp, ctypes.c_size_t]
+        L.vsc_flownet_set_params.argtypes = [vp, vp, ctypes.c_size_t]
+        L.vsc_flownet_extract.argtypes = [vp, vp, i, vp]
+        L.vsc_flownet_flow.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(vp), i, vp]
+        L.vsc_flownet_intermediate.argtypes = [vp, i, i, i, vp, vp]
         L.vsc_profile_begin.argtypes = [i]
         L.vsc_profile_end.argtypes = [ctypes.c_char_p, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -148,6 +163,108 @@ def _stream():
 
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
+
+
+# ---------------------------------------------------------------------------
+# flow network building blocks (channels-last tensors)
+# ---------------------------------------------------------------------------
+def conv3x3_nhwc(x, wt, bias=None, stride=1, dilation=1, pad=None, act=0, out=None, out_dtype=None,
+                 add=None, in_slice=None, out_slice=None, add_slice=None):
+    """3x3 convolution of an NHWC device tensor on MFMA (vsc_conv3x3_nhwc).
+
+    x: [N, H, W, Cs] float32 / bfloat16 / float16; in_slice = (c0, Cin) selects
+    channels (default all).  wt: [Cout, 3, 3, Cin] of x's dtype; bias: float32
+    [Cout].  act: 0 none, 1 LeakyReLU(0.1), 2 ReLU.  out: [N, Ho, Wo, Cs_out]
+    (allocated when None) with out_slice = (c0, Cout); add: residual of the
+    output's dtype, add_slice = (c0, Cout)."""
+    torch = _torch()
+    N, H, W, cs = x.shape
+    c0, cin = in_slice if in_slice is not None else (0, cs)
+    cout = wt.shape[0]
+    if pad is None:
+        pad = dilation
+    Ho = (H + 2 * pad - 2 * dilation - 1) // stride + 1
+    Wo = (W + 2 * pad - 2 * dilation - 1) // stride + 1
+    odt = out_dtype if out_dtype is not None else x.dtype
+    if out is None:
+        out = torch.zeros((N, Ho, Wo, cout), dtype=odt, device=x.device)
+    oc0, ocout = out_slice if out_slice is not None else (0, cout)
+    if ocout != cout:
+        raise VscError(-1, "out_slice width must equal Cout")
+    ac0 = add_slice[0] if add_slice is not None else 0
+    check(lib().vsc_conv3x3_nhwc(_ptr(x), cs, c0, N, H, W, cin, _ptr(wt), _ptr(bias) if bias is not None else None,
+                                 _ptr(out), out.shape[3], oc0, cout, stride, dilation, pad, act,
+                                 _ptr(add) if add is not None else None, add.shape[3] if add is not None else 0,
+                                 ac0, _dtype_code(x), _dtype_code(out), _stream()))
+    return out
+
+
+class FlowNet:
+    """PWCNet-light flow network on the library's kernels (vsc_flownet_*).
+
+    Frames are device RGBA8 tensors [H, W, 4]; extract() caches a frame's
+    features in a slot, flow() runs the decoders for one or two (first,
+    second) slot pairs and returns HWC float flows at flow_size."""
+
+    def __init__(self, width, height, downscale=1, dtype="f32", seed=0, slots=4):
+        self.dtype = {"f32": VSC_F32, "bf16": VSC_BF16}[dtype]
+        h = ctypes.c_void_p()
+        check(lib().vsc_flownet_create(width, height, downscale, self.dtype, seed, slots, ctypes.byref(h)))
+        self.h = h
+        w_, h_ = ctypes.c_int(), ctypes.c_int()
+        check(lib().vsc_flownet_flow_size(self.h, ctypes.byref(w_), ctypes.byref(h_)))
+        self.flow_size = (w_.value, h_.value)
+        self.width, self.height, self.downscale = width, height, downscale
+        self.padded = ((self.flow_size[0] + 63) // 64 * 64, (self.flow_size[1] + 63) // 64 * 64)
+
+    @staticmethod
+    def footprint(width, height, downscale=1, dtype="f32", slots=4):
+        return lib().vsc_flownet_footprint(width, height, downscale, {"f32": VSC_F32, "bf16": VSC_BF16}[dtype], slots)
+
+    def params(self):
+        import numpy as np
+        n = lib().vsc_flownet_param_count(self.h)
+        a = np.empty(n, dtype=np.float32)
+        check(lib().vsc_flownet_get_params(self.h, a.ctypes.data, n))
+        return a
+
+    def set_params(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        check(lib().vsc_flownet_set_params(self.h, a.ctypes.data, a.size))
+
+    def extract(self, rgba, slot):
+        check(lib().vsc_flownet_extract(self.h, _ptr(rgba), slot, _stream()))
+
+    def flow(self, pairs, out_channels=2, out=None):
+        torch = _torch()
+        n = len(pairs)
+        if out is None:
+            out = [torch.empty((self.flow_size[1], self.flow_size[0], out_channels), dtype=torch.float32,
+                               device="cuda") for _ in range(n)]
+        a = (ctypes.c_int * n)(*[p[0] for p in pairs])
+        b = (ctypes.c_int * n)(*[p[1] for p in pairs])
+        o = (ctypes.c_void_p * n)(*[t.data_ptr() for t in out])
+        check(lib().vsc_flownet_flow(self.h, n, a, b, o, out_channels, _stream()))
+        return out
+
+    def intermediate(self, which, level, item=0):
+        torch = _torch()
+        h, w = self.padded[1] >> level, self.padded[0] >> level
+        t = torch.empty((h, w, 2), dtype=torch.float32, device="cuda")
+        check(lib().vsc_flownet_intermediate(self.h, which, level, item, _ptr(t), _stream()))
+        return t
+
+    def close(self):
+        if self.h:
+            lib().vsc_flownet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -222,14 +222,6 @@ int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C,
 // ---------------------------------------------------------------------------
 // conversions
 // ---------------------------------------------------------------------------
-// binary32 -> bfloat16, round to nearest even (NaN stays NaN)
-__device__ inline uint16_t f32_to_bf16(float x)
-{
-    const uint32_t u = __builtin_bit_cast(uint32_t, x);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
-    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-
 // binary32 -> OCP FP8 E4M3 (bias 7, max 448, no infinities), round to nearest
 // even, saturating to +-448; NaN -> 0x7F.
 __host__ __device__ inline uint8_t f32_to_e4m3(float x)

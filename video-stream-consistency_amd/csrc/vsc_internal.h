@@ -226,6 +226,65 @@ struct FrontArgs {
 };
 int launch_front_pass(const FrontArgs& a, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// 16-bit storage formats of the flow network (FP32 arithmetic in registers)
+// ---------------------------------------------------------------------------
+// binary32 -> bfloat16, round to nearest even (NaN stays NaN)
+__host__ __device__ inline uint16_t f32_to_bf16(float x)
+{
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+__host__ __device__ inline uint16_t f32_to_f16(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+__host__ __device__ inline float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+// element kinds of the channels-last (NHWC) network tensors
+enum ElemKind { EK_F32 = 0, EK_BF16 = 1, EK_F16 = 2 };
+template <int EK> struct Elem;
+template <> struct Elem<EK_F32> {
+    using T = float;
+    static constexpr int VEC = 4;    // elements per 16-byte load
+    __device__ static inline float ld(T v) { return v; }
+    __device__ static inline T st(float v) { return v; }
+};
+template <> struct Elem<EK_BF16> {
+    using T = uint16_t;
+    static constexpr int VEC = 8;
+    __device__ static inline float ld(T v) { return bf16_to_f32(v); }
+    __device__ static inline T st(float v) { return f32_to_bf16(v); }
+};
+template <> struct Elem<EK_F16> {
+    using T = uint16_t;
+    static constexpr int VEC = 8;
+    __device__ static inline float ld(T v) { return f16_to_f32(v); }
+    __device__ static inline T st(float v) { return f32_to_f16(v); }
+};
+inline int elem_kind(int vsc_dtype)
+{
+    return vsc_dtype == VSC_F32 ? EK_F32 : vsc_dtype == VSC_BF16 ? EK_BF16 : vsc_dtype == VSC_F16 ? EK_F16 : -1;
+}
+inline int elem_bytes(int ek) { return ek == EK_F32 ? 4 : 2; }
+
+// NHWC tensor slice: channel c of pixel (n, y, x) at p[((n*h + y)*w + x)*cs + c0 + c]
+struct Nhwc {
+    void* p;
+    int n, h, w, c;      // c: channels of the slice
+    int cs, c0;          // pixel stride and channel offset (elements)
+};
+
+// 3x3 convolution on MFMA (conv.hip).  act: 0 none, 1 LeakyReLU(0.1), 2 ReLU;
+// add (optional, same element kind as out) is added after the activation.
+struct ConvDesc {
+    Nhwc in, out, add;   // add.p == nullptr: no residual
+    const void* wt;      // [Cout][3][3][in.c], element kind of in
+    const float* bias;   // [Cout] or nullptr
+    int stride, dil, pad, act;
+    int ek_in, ek_out;
+};
+int launch_conv3x3(const ConvDesc& d, hipStream_t s);
+
 // reduced-precision (BF16 / F16 / FP8 e4m3) correlation on MFMA, corr_mfma.hip
 int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
                        int in_dtype, const float* chan_scale, hipStream_t s);
