@@ -300,6 +300,56 @@ int vsc_flownet_intermediate(const vsc_flownet* net, int which, int level, int i
                              vsc_hip_stream stream);
 
 /* ---------------------------------------------------------------------------
+ * Learned flow upsampler for FLOWDOWNSCALE (replaces the get_bilinear flow
+ * upsampling of flowmodel.cpp:158-161 when enabled): input = bilinear x scale
+ * of the low-resolution flow (the reference's kernel_bilinear mapping) plus
+ * the luma of the full-resolution frame; conv3x3 3->16, ReLU, 16->16, ReLU,
+ * 16->2, added to the bilinear flow.  Inference: FP16 storage, FP32
+ * accumulation.  Training (FP32 master weights, forward, backward and Adam on
+ * the device): Charbonnier end-point loss sqrt(|out - gt|^2 + 1e-6), mean
+ * over pixels.  Parameters in torch layout: w1 [16][3][3][3], b1 [16],
+ * w2 [16][16][3][3], b2 [16], w3 [2][16][3][3], b3 [2].                     */
+typedef struct vsc_upsampler vsc_upsampler;
+
+int vsc_upsampler_create(unsigned long long seed, vsc_upsampler** out);
+int vsc_upsampler_destroy(vsc_upsampler* ups);
+size_t vsc_upsampler_param_count(void);
+int vsc_upsampler_get_params(const vsc_upsampler* ups, float* host);
+int vsc_upsampler_set_params(vsc_upsampler* ups, const float* host);
+/* device bytes of the inference workspace for a width x height output */
+size_t vsc_upsampler_workspace_size(int width, int height);
+/* flow_lo: device HWC lo_w x lo_h x lo_ch (2|3); rgba: device RGBA8 width x
+ * height; flow_out: device HWC width x height x out_ch (2|3, third channel 0);
+ * scale: factor applied to the bilinear flow vectors (FLOWDOWNSCALE f with
+ * vector scaling, 1 for the reference's unscaled vectors); gate: NULL or a
+ * device float multiplying the learned residual (see vsc_flow_fb_gate);
+ * workspace: 256-byte aligned, vsc_upsampler_workspace_size bytes.         */
+int vsc_upsampler_run(vsc_upsampler* ups, const float* flow_lo, int lo_w, int lo_h, int lo_ch, const uint8_t* rgba,
+                      int width, int height, float scale, float* flow_out, int out_ch, const float* gate,
+                      void* workspace, vsc_hip_stream stream);
+/* Training on a batch: flow_lo [batch][lo_h][lo_w][2], rgba [batch][height][width][4],
+ * flow_gt [batch][height][width][2] (device).  loss (nullable host pointer)
+ * receives the batch loss (synchronises).  train_step applies one Adam step
+ * (beta 0.9 / 0.999, eps 1e-8); loss_grad only computes the gradient, copied
+ * to grad_host (nullable) in the parameter layout above. */
+int vsc_upsampler_train_step(vsc_upsampler* ups, int batch, const float* flow_lo, int lo_w, int lo_h,
+                             const uint8_t* rgba, const float* flow_gt, int width, int height, float scale, float lr,
+                             float* loss, vsc_hip_stream stream);
+int vsc_upsampler_loss_grad(vsc_upsampler* ups, int batch, const float* flow_lo, int lo_w, int lo_h,
+                            const uint8_t* rgba, const float* flow_gt, int width, int height, float scale, float* loss,
+                            float* grad_host, vsc_hip_stream stream);
+
+/* Forward-backward flow consistency (Sundaram, Brox & Keutzer 2010) as the
+ * per-frame trigger of the bilinear fallback: fwd maps frame A to B, bwd maps
+ * B to A (device HWC, channels 2|3).  A pixel is inconsistent when x + fwd(x)
+ * leaves the frame or |fwd(x) + bwd(x + fwd(x))|^2 >= 0.01 (|fwd|^2 + |bwd|^2)
+ * + 0.5 (bwd sampled bilinearly).  Writes the count to *count and to *gate 1.0
+ * when count <= max_inconsistent * width * height, else 0.0 (both device
+ * scalars; no host synchronisation). */
+int vsc_flow_fb_gate(const float* fwd, const float* bwd, int width, int height, int channels, float max_inconsistent,
+                     uint32_t* count, float* gate, vsc_hip_stream stream);
+
+/* ---------------------------------------------------------------------------
  * Per-kernel timing (replaces the per-stage QElapsedTimer accounting of
  * VideoStabilizer, videostabilizer.h:72-76 / videostabilizer.cpp:175,240-262,
  * with device-side timing).  Between begin and end, every launch of the

@@ -46,6 +46,9 @@ EXPORTED = (
     "vsc_stab_consist_out_planar", "vsc_conv3x3_nhwc", "vsc_flownet_create", "vsc_flownet_destroy",
     "vsc_flownet_footprint", "vsc_flownet_flow_size", "vsc_flownet_param_count", "vsc_flownet_get_params",
     "vsc_flownet_set_params", "vsc_flownet_extract", "vsc_flownet_flow", "vsc_flownet_intermediate",
+    "vsc_upsampler_create", "vsc_upsampler_destroy", "vsc_upsampler_param_count", "vsc_upsampler_get_params",
+    "vsc_upsampler_set_params", "vsc_upsampler_workspace_size", "vsc_upsampler_run", "vsc_upsampler_train_step",
+    "vsc_upsampler_loss_grad", "vsc_flow_fb_gate",
 )
 
 
@@ -91,6 +94,10 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                                "(make -C video-stream-consistency_amd)")
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7);
+        # loading torch first makes the library bind to that one, so torch's
+        # streams and allocations and this library's launches share one runtime
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
         img = VscImage
@@ -135,6 +142,18 @@ def lib():
         L.vsc_flownet_extract.argtypes = [vp, vp, i, vp]
         L.vsc_flownet_flow.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(vp), i, vp]
         L.vsc_flownet_intermediate.argtypes = [vp, i, i, i, vp, vp]
+        L.vsc_upsampler_create.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(vp)]
+        L.vsc_upsampler_destroy.argtypes = [vp]
+        L.vsc_upsampler_param_count.argtypes = []
+        L.vsc_upsampler_param_count.restype = ctypes.c_size_t
+        L.vsc_upsampler_get_params.argtypes = [vp, vp]
+        L.vsc_upsampler_set_params.argtypes = [vp, vp]
+        L.vsc_upsampler_workspace_size.argtypes = [i, i]
+        L.vsc_upsampler_workspace_size.restype = ctypes.c_size_t
+        L.vsc_upsampler_run.argtypes = [vp, vp, i, i, i, vp, i, i, f, vp, i, vp, vp, vp]
+        L.vsc_upsampler_train_step.argtypes = [vp, i, vp, i, i, vp, vp, i, i, f, f, vp, vp]
+        L.vsc_upsampler_loss_grad.argtypes = [vp, i, vp, i, i, vp, vp, i, i, f, vp, vp, vp]
+        L.vsc_flow_fb_gate.argtypes = [vp, vp, i, i, i, f, vp, vp, vp]
         L.vsc_profile_begin.argtypes = [i]
         L.vsc_profile_end.argtypes = [ctypes.c_char_p, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -265,6 +284,95 @@ class FlowNet:
             self.close()
         except Exception:
             pass
+
+
+class Upsampler:
+    """Learned flow upsampler (vsc_upsampler_*): FP16 inference, FP32 training
+    with the library's forward/backward/Adam kernels."""
+
+    def __init__(self, seed=0, params=None):
+        h = ctypes.c_void_p()
+        check(lib().vsc_upsampler_create(seed, ctypes.byref(h)))
+        self.h = h
+        self._ws = None
+        if params is not None:
+            self.set_params(params)
+
+    @staticmethod
+    def param_count():
+        return lib().vsc_upsampler_param_count()
+
+    def params(self):
+        import numpy as np
+        a = np.empty(self.param_count(), dtype=np.float32)
+        check(lib().vsc_upsampler_get_params(self.h, a.ctypes.data))
+        return a
+
+    def set_params(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.size != self.param_count():
+            raise VscError(-1, "wrong parameter count")
+        check(lib().vsc_upsampler_set_params(self.h, a.ctypes.data))
+
+    def workspace(self, width, height):
+        torch = _torch()
+        n = lib().vsc_upsampler_workspace_size(width, height)
+        if self._ws is None or self._ws.numel() < n:
+            self._ws = torch.empty(n, dtype=torch.uint8, device="cuda")
+        return self._ws
+
+    def run(self, flow_lo, rgba, scale=2.0, out_ch=2, gate=None, out=None):
+        """flow_lo [lh, lw, 2|3] float32, rgba [H, W, 4] uint8 (device) -> flow [H, W, out_ch]."""
+        torch = _torch()
+        H, W = rgba.shape[:2]
+        lh, lw, lch = flow_lo.shape
+        if out is None:
+            out = torch.empty((H, W, out_ch), dtype=torch.float32, device=rgba.device)
+        ws = self.workspace(W, H)
+        check(lib().vsc_upsampler_run(self.h, _ptr(flow_lo), lw, lh, lch, _ptr(rgba), W, H, scale, _ptr(out), out_ch,
+                                      _ptr(gate) if gate is not None else None, _ptr(ws), _stream()))
+        return out
+
+    def train_step(self, flow_lo, rgba, flow_gt, lr=1e-3, scale=2.0, want_loss=False):
+        """One Adam step on a batch: flow_lo [B, lh, lw, 2], rgba [B, H, W, 4], flow_gt [B, H, W, 2]."""
+        B, H, W = rgba.shape[:3]
+        loss = ctypes.c_float()
+        check(lib().vsc_upsampler_train_step(self.h, B, _ptr(flow_lo), flow_lo.shape[2], flow_lo.shape[1], _ptr(rgba),
+                                             _ptr(flow_gt), W, H, scale, lr,
+                                             ctypes.byref(loss) if want_loss else None, _stream()))
+        return loss.value if want_loss else None
+
+    def loss_grad(self, flow_lo, rgba, flow_gt, scale=2.0):
+        import numpy as np
+        B, H, W = rgba.shape[:3]
+        loss = ctypes.c_float()
+        g = np.empty(self.param_count(), dtype=np.float32)
+        check(lib().vsc_upsampler_loss_grad(self.h, B, _ptr(flow_lo), flow_lo.shape[2], flow_lo.shape[1], _ptr(rgba),
+                                            _ptr(flow_gt), W, H, scale, ctypes.byref(loss), g.ctypes.data, _stream()))
+        return loss.value, g
+
+    def close(self):
+        if self.h:
+            lib().vsc_upsampler_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def flow_fb_gate(fwd, bwd, max_inconsistent=0.25):
+    """Forward-backward consistency count and fallback gate (device scalars)."""
+    torch = _torch()
+    h, w, ch = fwd.shape
+    count = torch.empty(1, dtype=torch.int32, device=fwd.device)
+    gate = torch.empty(1, dtype=torch.float32, device=fwd.device)
+    check(lib().vsc_flow_fb_gate(_ptr(fwd), _ptr(bwd), w, h, ch, max_inconsistent, _ptr(count), _ptr(gate),
+                                 _stream()))
+    return count, gate
 
 
 # ---------------------------------------------------------------------------

@@ -38,6 +38,7 @@ struct ConvArgs {
     const void* wt; const float* bias;
     void* out; int out_cs, out_c0;
     const void* add; int add_cs, add_c0;
+    const float* gate;
     int Ho, Wo, Cout;
     int pad, pstep, act;
     int tiles_x, tiles_y;
@@ -179,6 +180,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
     }
 
     // epilogue: lane holds output channels nt0 + 16i + 4lg + r of pixel l16 of row q*MR+m
+    const float g = a.gate ? *a.gate : 1.0f;
 #pragma unroll
     for (int m = 0; m < MR; ++m) {
         const int oy = py + ps * (ty * TH + q * MR + m), ox = px + ps * (tx * TW + l16);
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
                 if (a.bias && r < nv) x += a.bias[co + r];
                 if (a.act == 1) x = x > 0.0f ? x : x * 0.1f;
                 else if (a.act == 2) x = x > 0.0f ? x : 0.0f;
-                v[r] = x;
+                v[r] = a.gate ? x * g : x;
             }
             if (a.add) conv_load4<EKO>(a.add, pix * a.add_cs + a.add_c0 + co, v, nv);
             conv_store4<EKO>(a.out, pix * a.out_cs + a.out_c0 + co, v, nv);
@@ -275,6 +277,7 @@ int launch_conv3x3(const ConvDesc& d, hipStream_t s)
     a.wt = d.wt; a.bias = d.bias;
     a.out = d.out.p; a.out_cs = d.out.cs; a.out_c0 = d.out.c0;
     a.add = d.add.p; a.add_cs = d.add.cs; a.add_c0 = d.add.c0;
+    a.gate = d.gate;
     a.Ho = Ho; a.Wo = Wo; a.Cout = Cout;
     a.pad = d.pad; a.pstep = d.dil; a.act = d.act;
     if (ek == EK_F32) conv_stride<EK_F32, EK_F32>(a, d.stride, s);

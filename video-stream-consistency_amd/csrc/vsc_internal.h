@@ -280,6 +280,7 @@ struct ConvDesc {
     Nhwc in, out, add;   // add.p == nullptr: no residual
     const void* wt;      // [Cout][3][3][in.c], element kind of in
     const float* bias;   // [Cout] or nullptr
+    const float* gate;   // nullptr, or a device scalar multiplying the activated result before `add`
     int stride, dil, pad, act;
     int ek_in, ek_out;
 };
