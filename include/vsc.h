@@ -350,6 +350,53 @@ int vsc_flow_fb_gate(const float* fwd, const float* bwd, int width, int height, 
                      uint32_t* count, float* gate, vsc_hip_stream stream);
 
 /* ---------------------------------------------------------------------------
+ * Per-stream frame pipeline: VideoStabilizer::doOneStep (videostabilizer.cpp:
+ * 167-265) including retrieveOpticalFlow (:267-279) through the flow network,
+ * FLOWDOWNSCALE (videostabilizer.cpp:74-94, flowmodel.cpp:126-133,158-161)
+ * and the learned upsampler.  Owns a vsc_flownet (3 feature slots), the flow
+ * buffers, an upsampler and a vsc_stream; a frame allocates nothing.        */
+typedef struct vsc_pipeline vsc_pipeline;
+
+typedef struct {
+    int flow_dtype;             /* VSC_F32 (plain form) | VSC_BF16 */
+    int downscale;              /* FLOWDOWNSCALE f: 1 | 2 | 4 */
+    int upsample;               /* f > 1: 0 bilinear (get_bilinear), 1 learned upsampler */
+    int scale_vectors;          /* bilinear path: multiply vectors by f (the reference does not: 0);
+                                   the learned path always scales (trained on full-resolution vectors) */
+    float fb_max_inconsistent;  /* learned path: bilinear fallback when the inconsistent fraction exceeds this (< 0: never) */
+    int ref_bwd_index;          /* 1: backward flow = net(next -> cur) as videostabilizer.cpp:271-272; 0: cur -> prev */
+    unsigned long long seed;    /* flow network (and untrained upsampler) initialisation */
+} vsc_pipeline_cfg;
+
+/* The k = 1 frame window (device RGBA8 frames, 4-byte aligned). */
+typedef struct {
+    const uint8_t* orig_prev;
+    const uint8_t* orig_cur;
+    const uint8_t* orig_next;
+    const uint8_t* proc_prev;
+    const uint8_t* proc_cur;
+    const uint8_t* proc_next;
+} vsc_window;
+
+void vsc_pipeline_default_cfg(vsc_pipeline_cfg* cfg);
+int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg* cfg, vsc_pipeline** out);
+int vsc_pipeline_destroy(vsc_pipeline* p);
+size_t vsc_pipeline_footprint(int width, int height, const vsc_pipeline_cfg* cfg);
+int vsc_pipeline_set_upsampler_params(vsc_pipeline* p, const float* host);
+/* borrowed handles of the parts (for `last` preloading and diagnostics) */
+int vsc_pipeline_flownet(vsc_pipeline* p, vsc_flownet** net);
+int vsc_pipeline_stream(vsc_pipeline* p, vsc_stream** s);
+/* Flows of window `frame` (= index of orig_cur; frames extracted once, keyed
+ * by index) at the flow resolution, HWC 2 channels (device). */
+int vsc_pipeline_flows(vsc_pipeline* p, const vsc_window* w, long long frame, float* fwd, float* bwd,
+                       vsc_hip_stream stream);
+/* One stabilized frame of window `frame` (flows, upsampling, vsc_stab_frame). */
+int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long long frame, const vsc_params* params,
+                       uint8_t* out_rgba, float* out_float, vsc_hip_stream stream);
+/* The last frame's fallback gate (1 learned, 0 bilinear) and inconsistent count. */
+int vsc_pipeline_gate(const vsc_pipeline* p, float* gate, uint32_t* count, vsc_hip_stream stream);
+
+/* ---------------------------------------------------------------------------
  * Per-kernel timing (replaces the per-stage QElapsedTimer accounting of
  * VideoStabilizer, videostabilizer.h:72-76 / videostabilizer.cpp:175,240-262,
  * with device-side timing).  Between begin and end, every launch of the

@@ -48,7 +48,9 @@ EXPORTED = (
     "vsc_flownet_set_params", "vsc_flownet_extract", "vsc_flownet_flow", "vsc_flownet_intermediate",
     "vsc_upsampler_create", "vsc_upsampler_destroy", "vsc_upsampler_param_count", "vsc_upsampler_get_params",
     "vsc_upsampler_set_params", "vsc_upsampler_workspace_size", "vsc_upsampler_run", "vsc_upsampler_train_step",
-    "vsc_upsampler_loss_grad", "vsc_flow_fb_gate",
+    "vsc_upsampler_loss_grad", "vsc_flow_fb_gate", "vsc_pipeline_default_cfg", "vsc_pipeline_create",
+    "vsc_pipeline_destroy", "vsc_pipeline_footprint", "vsc_pipeline_set_upsampler_params", "vsc_pipeline_flownet",
+    "vsc_pipeline_stream", "vsc_pipeline_flows", "vsc_pipeline_frame", "vsc_pipeline_gate",
 )
 
 
@@ -67,6 +69,17 @@ class VscFrameIn(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in ("orig_prev", "orig_cur", "orig_next", "proc_prev",
                                                 "proc_cur", "proc_next", "flow_fwd", "flow_bwd")] + \
                [("flow_channels", ctypes.c_int)]
+
+
+class PipelineCfg(ctypes.Structure):
+    _fields_ = [("flow_dtype", ctypes.c_int), ("downscale", ctypes.c_int), ("upsample", ctypes.c_int),
+                ("scale_vectors", ctypes.c_int), ("fb_max_inconsistent", ctypes.c_float),
+                ("ref_bwd_index", ctypes.c_int), ("seed", ctypes.c_ulonglong)]
+
+
+class Window(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("orig_prev", "orig_cur", "orig_next", "proc_prev", "proc_cur",
+                                                "proc_next")]
 
 
 class SynthCfg(ctypes.Structure):
@@ -154,6 +167,19 @@ def lib():
         L.vsc_upsampler_train_step.argtypes = [vp, i, vp, i, i, vp, vp, i, i, f, f, vp, vp]
         L.vsc_upsampler_loss_grad.argtypes = [vp, i, vp, i, i, vp, vp, i, i, f, vp, vp, vp]
         L.vsc_flow_fb_gate.argtypes = [vp, vp, i, i, i, f, vp, vp, vp]
+        L.vsc_pipeline_default_cfg.argtypes = [ctypes.POINTER(PipelineCfg)]
+        L.vsc_pipeline_default_cfg.restype = None
+        L.vsc_pipeline_create.argtypes = [i, i, ctypes.POINTER(PipelineCfg), ctypes.POINTER(vp)]
+        L.vsc_pipeline_destroy.argtypes = [vp]
+        L.vsc_pipeline_footprint.argtypes = [i, i, ctypes.POINTER(PipelineCfg)]
+        L.vsc_pipeline_footprint.restype = ctypes.c_size_t
+        L.vsc_pipeline_set_upsampler_params.argtypes = [vp, vp]
+        L.vsc_pipeline_flownet.argtypes = [vp, ctypes.POINTER(vp)]
+        L.vsc_pipeline_stream.argtypes = [vp, ctypes.POINTER(vp)]
+        L.vsc_pipeline_flows.argtypes = [vp, ctypes.POINTER(Window), ctypes.c_longlong, vp, vp, vp]
+        L.vsc_pipeline_frame.argtypes = [vp, ctypes.POINTER(Window), ctypes.c_longlong, ctypes.POINTER(VscParams),
+                                         vp, vp, vp]
+        L.vsc_pipeline_gate.argtypes = [vp, vp, vp, vp]
         L.vsc_profile_begin.argtypes = [i]
         L.vsc_profile_end.argtypes = [ctypes.c_char_p, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -639,6 +665,112 @@ class VideoStabilizer:
         if getattr(self, "_h", None):
             lib().vsc_stream_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_upsampler_weights():
+    """The upsampler weights trained by upsampler_train.py (float32, torch layout)."""
+    path = os.path.join(_HERE, "upsampler_weights.npy")
+    return np.load(path, allow_pickle=False)
+
+
+class Pipeline:
+    """Per-stream frame pipeline (vsc_pipeline_*): flow network, FLOWDOWNSCALE,
+    bilinear or learned upsampling with the consistency fallback, stabilization.
+
+    form "plain": FP32 network at full resolution; form "chosen": BF16 network
+    at FLOWDOWNSCALE 2 with the learned upsampler.  Frames are RGBA8 device
+    tensors; a window is (orig_prev, orig_cur, orig_next, proc_prev, proc_cur,
+    proc_next) and `frame` is the index of orig_cur (features are cached by
+    index)."""
+
+    FORMS = {
+        "plain": dict(flow_dtype="f32", downscale=1, upsample="bilinear"),
+        "chosen": dict(flow_dtype="bf16", downscale=2, upsample="learned"),
+    }
+
+    def __init__(self, width, height, form=None, flow_dtype="f32", downscale=1, upsample="bilinear",
+                 scale_vectors=False, fb_max_inconsistent=0.25, ref_bwd_index=False, seed=0, upsampler_params=None):
+        if form is not None:
+            kw = self.FORMS[form]
+            flow_dtype, downscale, upsample = kw["flow_dtype"], kw["downscale"], kw["upsample"]
+        c = PipelineCfg()
+        lib().vsc_pipeline_default_cfg(ctypes.byref(c))
+        c.flow_dtype = {"f32": VSC_F32, "bf16": VSC_BF16}[flow_dtype]
+        c.downscale = downscale
+        c.upsample = {"bilinear": 0, "learned": 1}[upsample]
+        c.scale_vectors = int(bool(scale_vectors))
+        c.fb_max_inconsistent = fb_max_inconsistent
+        c.ref_bwd_index = int(bool(ref_bwd_index))
+        c.seed = seed
+        self.cfg = c
+        self.width, self.height = width, height
+        h = ctypes.c_void_p()
+        check(lib().vsc_pipeline_create(width, height, ctypes.byref(c), ctypes.byref(h)))
+        self.h = h
+        self.flow_size = (int(round(width / downscale)), int(round(height / downscale)))
+        self.controlParameters = HyperParams()
+        if c.upsample == 1:
+            p = upsampler_params if upsampler_params is not None else load_upsampler_weights()
+            p = np.ascontiguousarray(p, dtype=np.float32)
+            check(lib().vsc_pipeline_set_upsampler_params(self.h, p.ctypes.data))
+
+    @staticmethod
+    def footprint(width, height, form="plain"):
+        """Device bytes of a pipeline of this form (admission by HBM budget)."""
+        fk = Pipeline.FORMS[form]
+        c = PipelineCfg()
+        lib().vsc_pipeline_default_cfg(ctypes.byref(c))
+        c.flow_dtype = {"f32": VSC_F32, "bf16": VSC_BF16}[fk["flow_dtype"]]
+        c.downscale = fk["downscale"]
+        c.upsample = {"bilinear": 0, "learned": 1}[fk["upsample"]]
+        return int(lib().vsc_pipeline_footprint(width, height, ctypes.byref(c)))
+
+    def _stream_handle(self):
+        s = ctypes.c_void_p()
+        check(lib().vsc_pipeline_stream(self.h, ctypes.byref(s)))
+        return s
+
+    def set_last_rgba(self, processed_rgba):
+        check(lib().vsc_stream_set_last_rgba(self._stream_handle(), _ptr(processed_rgba), _stream()))
+
+    @staticmethod
+    def _window(w):
+        return Window(*(t.data_ptr() for t in w))
+
+    def flows(self, window, frame):
+        torch = _torch()
+        fw, fh = self.flow_size
+        f = torch.empty((fh, fw, 2), dtype=torch.float32, device="cuda")
+        b = torch.empty((fh, fw, 2), dtype=torch.float32, device="cuda")
+        check(lib().vsc_pipeline_flows(self.h, ctypes.byref(self._window(window)), frame, _ptr(f), _ptr(b), _stream()))
+        return f, b
+
+    def frame(self, window, frame, out_rgba=None, out_float=None, params=None):
+        torch = _torch()
+        if out_rgba is None:
+            out_rgba = torch.empty((self.height, self.width, 4), dtype=torch.uint8, device="cuda")
+        prm = (params or self.controlParameters).c()
+        check(lib().vsc_pipeline_frame(self.h, ctypes.byref(self._window(window)), frame, ctypes.byref(prm),
+                                       _ptr(out_rgba), None if out_float is None else _ptr(out_float), _stream()))
+        return out_rgba
+
+    def gate(self):
+        torch = _torch()
+        g = torch.empty(1, dtype=torch.float32, device="cuda")
+        c = torch.empty(1, dtype=torch.int32, device="cuda")
+        check(lib().vsc_pipeline_gate(self.h, _ptr(g), _ptr(c), _stream()))
+        return float(g.cpu()[0]), int(c.cpu()[0])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vsc_pipeline_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

@@ -1,0 +1,263 @@
+// Per-stream frame pipeline: VideoStabilizer::doOneStep with its flow
+// acquisition (retrieveOpticalFlow, videostabilizer.cpp:267-279, through
+// FlowModel::run, flowmodel.cpp:121-168) on one device stream:
+//   1. flow network features of the new frame (cached per frame: each frame
+//      is extracted once and serves three windows),
+//   2. forward (cur -> next) and backward flows in one batched decode, at the
+//      FLOWDOWNSCALE resolution,
+//   3. upsampling to the frame: the reference's get_bilinear (vectors
+//      unscaled, as flowmodel.cpp:158-161, or scaled by the factor), or the
+//      learned upsampler gated per frame by the forward-backward check,
+//   4. the fused stabilization of vsc_stab_frame.
+// All buffers are allocated at creation; a frame allocates nothing.
+#include "vsc_internal.h"
+
+#include <cmath>
+#include <new>
+
+struct vsc_pipeline {
+    int W, H, Wd, Hd;
+    vsc_pipeline_cfg cfg;
+    vsc_flownet* net = nullptr;
+    vsc_upsampler* ups = nullptr;
+    vsc_stream* stab = nullptr;
+    float* lo[3] = {nullptr, nullptr, nullptr};   // low-res flows: fwd, bwd, previous fwd (HWC 2)
+    float* hi[2] = {nullptr, nullptr};            // full-res fwd, bwd (HWC 2) when upsampling here
+    void* ws = nullptr;                            // upsampler workspace
+    uint32_t* count = nullptr;
+    float* gate = nullptr;
+    long long slot_frame[3] = {-1, -1, -1};        // frame index held by each feature slot
+    long long prev_fwd_frame = -2;                 // frame whose fwd flow lo[2] holds
+};
+
+namespace vsc {
+namespace {
+
+__global__ void k_flow_upscale(const float* __restrict__ lo, int lw, int lh, float* hi, int W, int H, float scale)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const BiTap t = bilinear_tap(x, y, lw, lh, W, H);
+    View fl{const_cast<float*>(lo), lw, lh, 2, 1, 2};
+    const long long p = (long long)y * W + x;
+    hi[p * 2] = bilinear_sample(fl, t, 0) * scale;
+    hi[p * 2 + 1] = bilinear_sample(fl, t, 1) * scale;
+}
+
+__global__ void k_set_gate(float* g, float v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) g[0] = v;
+}
+
+int dalloc(void** p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) return set_error(VSC_E_NOMEM, "vsc_pipeline_create: device allocation failed");
+    return VSC_OK;
+}
+
+bool learned(const vsc_pipeline_cfg& c) { return c.downscale > 1 && c.upsample == 1; }
+bool upsample_here(const vsc_pipeline_cfg& c) { return c.downscale > 1 && (c.upsample == 1 || c.scale_vectors); }
+
+}  // namespace
+}  // namespace vsc
+
+using namespace vsc;
+
+extern "C" void vsc_pipeline_default_cfg(vsc_pipeline_cfg* c)
+{
+    if (!c) return;
+    c->flow_dtype = VSC_F32;
+    c->downscale = 1;
+    c->upsample = 0;
+    c->scale_vectors = 0;
+    c->fb_max_inconsistent = 0.25f;
+    c->ref_bwd_index = 0;
+    c->seed = 0;
+}
+
+static int cfg_ok(int W, int H, const vsc_pipeline_cfg* c)
+{
+    if (!c || W <= 0 || H <= 0) return set_error(VSC_E_INVALID, "vsc_pipeline: bad size or NULL config");
+    if (c->flow_dtype != VSC_F32 && c->flow_dtype != VSC_BF16)
+        return set_error(VSC_E_DTYPE, "vsc_pipeline: flow_dtype must be VSC_F32 or VSC_BF16");
+    if (c->downscale != 1 && c->downscale != 2 && c->downscale != 4)
+        return set_error(VSC_E_INVALID, "vsc_pipeline: downscale must be 1, 2 or 4");
+    if (c->upsample != 0 && c->upsample != 1) return set_error(VSC_E_INVALID, "vsc_pipeline: upsample must be 0 or 1");
+    return VSC_OK;
+}
+
+extern "C" size_t vsc_pipeline_footprint(int width, int height, const vsc_pipeline_cfg* cfg)
+{
+    if (cfg_ok(width, height, cfg)) return 0;
+    const int Wd = (int)std::lround((double)width / cfg->downscale), Hd = (int)std::lround((double)height / cfg->downscale);
+    size_t t = vsc_flownet_footprint(width, height, cfg->downscale, cfg->flow_dtype, 3);
+    const bool here = upsample_here(*cfg);
+    t += vsc_stream_footprint(width, height, here ? width : Wd, here ? height : Hd);
+    t += 3 * (size_t)Wd * Hd * 2 * 4;
+    if (here) t += 2 * (size_t)width * height * 2 * 4;
+    if (learned(*cfg)) t += vsc_upsampler_workspace_size(width, height) + 4 * 3202 * 4 + 512;
+    return t + 64;
+}
+
+extern "C" int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg* cfg, vsc_pipeline** out)
+{
+    if (!out) return set_error(VSC_E_INVALID, "vsc_pipeline_create: out is NULL");
+    *out = nullptr;
+    int rc = cfg_ok(width, height, cfg);
+    if (rc) return rc;
+    vsc_pipeline* p = new (std::nothrow) vsc_pipeline();
+    if (!p) return set_error(VSC_E_NOMEM, "vsc_pipeline_create: host memory");
+    p->W = width; p->H = height; p->cfg = *cfg;
+    if ((rc = vsc_flownet_create(width, height, cfg->downscale, cfg->flow_dtype, cfg->seed, 3, &p->net)) ||
+        (rc = vsc_flownet_flow_size(p->net, &p->Wd, &p->Hd))) {
+        vsc_pipeline_destroy(p);
+        return rc;
+    }
+    const bool here = upsample_here(*cfg);
+    if ((rc = vsc_stream_create(width, height, here ? width : p->Wd, here ? height : p->Hd, &p->stab))) {
+        vsc_pipeline_destroy(p);
+        return rc;
+    }
+    const size_t lo_b = (size_t)p->Wd * p->Hd * 2 * 4, hi_b = (size_t)width * height * 2 * 4;
+    for (int i = 0; i < 3 && !rc; ++i) rc = dalloc(reinterpret_cast<void**>(&p->lo[i]), lo_b);
+    if (here)
+        for (int i = 0; i < 2 && !rc; ++i) rc = dalloc(reinterpret_cast<void**>(&p->hi[i]), hi_b);
+    if (!rc) rc = dalloc(reinterpret_cast<void**>(&p->count), 256);
+    if (!rc) rc = dalloc(reinterpret_cast<void**>(&p->gate), 256);
+    if (!rc && learned(*cfg)) {
+        rc = vsc_upsampler_create(cfg->seed, &p->ups);
+        if (!rc) rc = dalloc(&p->ws, vsc_upsampler_workspace_size(width, height));
+    }
+    if (rc) {
+        vsc_pipeline_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return VSC_OK;
+}
+
+extern "C" int vsc_pipeline_destroy(vsc_pipeline* p)
+{
+    if (!p) return VSC_OK;
+    if (p->net) vsc_flownet_destroy(p->net);
+    if (p->ups) vsc_upsampler_destroy(p->ups);
+    if (p->stab) vsc_stream_destroy(p->stab);
+    void* bufs[] = {p->lo[0], p->lo[1], p->lo[2], p->hi[0], p->hi[1], p->ws, p->count, p->gate};
+    for (void* b : bufs) (void)hipFree(b);
+    delete p;
+    return VSC_OK;
+}
+
+extern "C" int vsc_pipeline_set_upsampler_params(vsc_pipeline* p, const float* host)
+{
+    if (!p || !host) return set_error(VSC_E_INVALID, "vsc_pipeline_set_upsampler_params: NULL argument");
+    if (!p->ups) return set_error(VSC_E_INVALID, "vsc_pipeline_set_upsampler_params: pipeline has no learned upsampler");
+    return vsc_upsampler_set_params(p->ups, host);
+}
+
+extern "C" int vsc_pipeline_flownet(vsc_pipeline* p, vsc_flownet** net)
+{
+    if (!p || !net) return set_error(VSC_E_INVALID, "vsc_pipeline_flownet: NULL argument");
+    *net = p->net;
+    return VSC_OK;
+}
+
+extern "C" int vsc_pipeline_stream(vsc_pipeline* p, vsc_stream** s)
+{
+    if (!p || !s) return set_error(VSC_E_INVALID, "vsc_pipeline_stream: NULL argument");
+    *s = p->stab;
+    return VSC_OK;
+}
+
+static int ensure_features(vsc_pipeline* p, const uint8_t* rgba, long long frame, hipStream_t s)
+{
+    const int slot = (int)(((frame % 3) + 3) % 3);
+    if (p->slot_frame[slot] == frame) return VSC_OK;
+    int rc = vsc_flownet_extract(p->net, rgba, slot, reinterpret_cast<vsc_hip_stream>(s));
+    if (rc) return rc;
+    p->slot_frame[slot] = frame;
+    return VSC_OK;
+}
+
+extern "C" int vsc_pipeline_flows(vsc_pipeline* p, const vsc_window* w, long long frame, float* fwd_out,
+                                  float* bwd_out, vsc_hip_stream stream)
+{
+    if (!p || !w || !w->orig_prev || !w->orig_cur || !w->orig_next)
+        return set_error(VSC_E_INVALID, "vsc_pipeline_flows: NULL argument");
+    hipStream_t s = as_stream(stream);
+    int rc;
+    if ((rc = ensure_features(p, w->orig_prev, frame - 1, s)) || (rc = ensure_features(p, w->orig_cur, frame, s)) ||
+        (rc = ensure_features(p, w->orig_next, frame + 1, s)))
+        return rc;
+    const int sp = (int)(((frame - 1) % 3 + 3) % 3), sc = (int)((frame % 3 + 3) % 3), sn = (int)(((frame + 1) % 3 + 3) % 3);
+    // forward cur -> next; backward cur -> prev (or next -> cur as videostabilizer.cpp:271-272)
+    const int first[2] = {sc, p->cfg.ref_bwd_index ? sn : sc};
+    const int second[2] = {sn, p->cfg.ref_bwd_index ? sc : sp};
+    float* outs[2] = {fwd_out ? fwd_out : p->lo[0], bwd_out ? bwd_out : p->lo[1]};
+    return vsc_flownet_flow(p->net, 2, first, second, outs, 2, stream);
+}
+
+extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long long frame, const vsc_params* params,
+                                  uint8_t* out_rgba, float* out_float, vsc_hip_stream stream)
+{
+    if (!p || !w) return set_error(VSC_E_INVALID, "vsc_pipeline_frame: NULL argument");
+    hipStream_t s = as_stream(stream);
+    const int f = p->cfg.downscale;
+    // previous forward flow for the consistency check of this frame's backward flow
+    const bool have_prev = p->prev_fwd_frame == frame - 1;
+    if (have_prev) {
+        float* t = p->lo[2];
+        p->lo[2] = p->lo[0];
+        p->lo[0] = t;
+    }
+    int rc = vsc_pipeline_flows(p, w, frame, p->lo[0], p->lo[1], stream);
+    if (rc) return rc;
+    const float* ff = p->lo[0];
+    const float* fb = p->lo[1];
+    if (learned(p->cfg)) {
+        if (p->cfg.fb_max_inconsistent < 0.0f || (!have_prev && !p->cfg.ref_bwd_index)) {
+            hipLaunchKernelGGL(k_set_gate, dim3(1), dim3(64), 0, s, p->gate, 1.0f);
+            if ((rc = check_launch("pipeline gate"))) return rc;
+        } else if (p->cfg.ref_bwd_index) {
+            // fwd: cur -> next, bwd: next -> cur are each other's inverse
+            if ((rc = vsc_flow_fb_gate(p->lo[0], p->lo[1], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent, p->count,
+                                       p->gate, stream)))
+                return rc;
+        } else {
+            // bwd: cur -> prev against the previous frame's fwd: prev -> cur
+            if ((rc = vsc_flow_fb_gate(p->lo[1], p->lo[2], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent, p->count,
+                                       p->gate, stream)))
+                return rc;
+        }
+        // the learned upsampler is trained on full-resolution vectors: always scaled
+        const float sc = (float)f;
+        if ((rc = vsc_upsampler_run(p->ups, p->lo[0], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[0], 2,
+                                    p->gate, p->ws, stream)) ||
+            (rc = vsc_upsampler_run(p->ups, p->lo[1], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[1], 2,
+                                    p->gate, p->ws, stream)))
+            return rc;
+        ff = p->hi[0];
+        fb = p->hi[1];
+    } else if (upsample_here(p->cfg)) {
+        dim3 grid((p->W + 255) / 256, p->H);
+        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, s, p->lo[0], p->Wd, p->Hd, p->hi[0], p->W, p->H, (float)f);
+        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, s, p->lo[1], p->Wd, p->Hd, p->hi[1], p->W, p->H, (float)f);
+        if ((rc = check_launch("pipeline flow upscale"))) return rc;
+        ff = p->hi[0];
+        fb = p->hi[1];
+    }
+    p->prev_fwd_frame = frame;
+    vsc_frame_in in{w->orig_prev, w->orig_cur, w->orig_next, w->proc_prev, w->proc_cur, w->proc_next, ff, fb, 2};
+    return vsc_stab_frame(p->stab, &in, params, out_rgba, out_float, VSC_FRAME_FUSED, stream);
+}
+
+extern "C" int vsc_pipeline_gate(const vsc_pipeline* p, float* gate_dev, uint32_t* count_dev, vsc_hip_stream stream)
+{
+    if (!p) return set_error(VSC_E_INVALID, "vsc_pipeline_gate: NULL argument");
+    hipStream_t s = as_stream(stream);
+    if (gate_dev && check_hip(hipMemcpyAsync(gate_dev, p->gate, 4, hipMemcpyDeviceToDevice, s), "vsc_pipeline_gate"))
+        return VSC_E_HIP;
+    if (count_dev && check_hip(hipMemcpyAsync(count_dev, p->count, 4, hipMemcpyDeviceToDevice, s), "vsc_pipeline_gate"))
+        return VSC_E_HIP;
+    return VSC_OK;
+}
