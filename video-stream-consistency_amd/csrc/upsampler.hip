@@ -10,6 +10,7 @@
 // by the scale factor.
 #include "vsc_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <new>
 #include <vector>
@@ -24,8 +25,11 @@ constexpr int U_W2 = U_B1 + 16, U_B2 = U_W2 + 16 * 9 * 16;
 constexpr int U_W3 = U_B2 + 16, U_B3 = U_W3 + 2 * 9 * 16;
 constexpr int U_NP = U_B3 + 2;                            // 3202 floats
 constexpr int U_NREF = 16 * 3 * 9 + 16 + 16 * 16 * 9 + 16 + 2 * 16 * 9 + 2;  // torch layout: 3058
-// FP16 inference weights (layer 1 input padded to 8)
-constexpr int H_W1 = 0, H_W2 = 16 * 9 * 8, H_W3 = H_W2 + 16 * 9 * 16, H_N = H_W3 + 2 * 9 * 16;
+// FP16 inference weights of the fused kernel, MFMA A-operand rows [co][K]:
+// layer 1 K = tap*4 + c (36 used of 64), layers 2 and 3 K = tap*16 + ci (144
+// used of 160); layer 3 rows 2..15 are zero
+constexpr int H_K1 = 64, H_K2 = 160;
+constexpr int H_W1 = 0, H_W2 = 16 * H_K1, H_W3 = H_W2 + 16 * H_K2, H_N = H_W3 + 16 * H_K2;
 // flipped / transposed FP32 weights for the data gradients
 constexpr int T_W2 = 0, T_W3 = 16 * 9 * 16, T_N = T_W3 + 16 * 9 * 4;
 
@@ -173,12 +177,14 @@ __global__ void k_ups_derive(const float* __restrict__ p, uint16_t* w16, float* 
     if (i < H_N) {
         float v = 0.0f;
         if (i < H_W2) {
-            const int ci = i % 8, r = i / 8;   // r = co*9 + tap
-            v = ci < U_C1 ? p[U_W1 + r * U_C1 + ci] : 0.0f;
+            const int co = i / H_K1, k = i % H_K1, tap = k / 4, c = k % 4;
+            v = (tap < 9 && c < 3) ? p[U_W1 + (co * 9 + tap) * U_C1 + c] : 0.0f;
         } else if (i < H_W3) {
-            v = p[U_W2 + (i - H_W2)];
+            const int co = (i - H_W2) / H_K2, k = (i - H_W2) % H_K2;
+            v = k < 144 ? p[U_W2 + co * 144 + k] : 0.0f;
         } else {
-            v = p[U_W3 + (i - H_W3)];
+            const int co = (i - H_W3) / H_K2, k = (i - H_W3) % H_K2;
+            v = (co < 2 && k < 144) ? p[U_W3 + co * 144 + k] : 0.0f;
         }
         w16[i] = f32_to_f16(v);
     }
@@ -192,6 +198,153 @@ __global__ void k_ups_derive(const float* __restrict__ p, uint16_t* w16, float* 
             v = co < 2 ? p[U_W3 + (co * 9 + 8 - t) * 16 + ci] : 0.0f;
         }
         wT[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused FP16 inference: the three convolutions of one 16 x 32 output tile in
+// one block, activations in LDS.  The input tile carries a 3-pixel halo, the
+// first activation a 2-pixel and the second a 1-pixel halo; activations of
+// pixels outside the frame are zero (each layer's zero padding).  Each layer
+// is an MFMA GEMM D[co][px] = W[co][K] * X[K][px] over 16-pixel groups, K =
+// (tap, channel) pairs gathered from LDS: layer 1 in two 16x16x32 steps (taps
+// 0..7, tap 8), layers 2 and 3 in five (tap pairs).  Weights stay in
+// registers for the whole block.
+// ---------------------------------------------------------------------------
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 up_f16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float up_f32x4;
+typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t up_u32x4;
+
+constexpr int UF_TH = 16, UF_TW = 32;
+constexpr int UF_XH = UF_TH + 6, UF_XW = UF_TW + 6;      // input tile
+constexpr int UF_1H = UF_TH + 4, UF_1W = UF_TW + 4;      // layer-1 output tile
+constexpr int UF_2H = UF_TH + 2, UF_2W = UF_TW + 2;      // layer-2 output tile
+constexpr int UF_AS = 24;                                 // LDS pixel stride of activations (halfs): 48 B
+
+__device__ inline up_f32x4 up_mma(const up_u32x4& a, const up_u32x4& b, up_f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(up_f16x8, a), __builtin_bit_cast(up_f16x8, b), c,
+                                                  0, 0, 0);
+}
+
+__device__ inline uint32_t pack_h2(float a, float b)
+{
+    return (uint32_t)f32_to_f16(a) | ((uint32_t)f32_to_f16(b) << 16);
+}
+
+__global__ __launch_bounds__(256) void k_ups_fused(const float* __restrict__ lo, int lw, int lh, int lch,
+                                                   const uint8_t* __restrict__ rgba, int W, int H, float scale,
+                                                   const uint16_t* __restrict__ wf, const float* __restrict__ prm,
+                                                   const float* gate, float* __restrict__ out, int out_ch,
+                                                   int tiles_x)
+{
+    __shared__ __attribute__((aligned(16))) uint2 sx[UF_XH * UF_XW];
+    __shared__ __attribute__((aligned(16))) uint16_t s1[UF_1H * UF_1W * UF_AS];
+    __shared__ __attribute__((aligned(16))) uint16_t s2[UF_2H * UF_2W * UF_AS];
+    const int tid = threadIdx.x, q = tid / 64, lane = tid % 64, l16 = lane % 16, lg = lane / 16;
+    const int y0 = (blockIdx.x / tiles_x) * UF_TH, x0 = (blockIdx.x % tiles_x) * UF_TW;
+    View fl{const_cast<float*>(lo), lw, lh, lch, 1, (long long)lch};
+
+    // weights (A operand rows co = l16) and biases for this lane's 4 output rows
+    up_u32x4 w1[2], w2[5], w3[5];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) w1[m] = *reinterpret_cast<const up_u32x4*>(wf + H_W1 + l16 * H_K1 + 32 * m + 8 * lg);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        w2[m] = *reinterpret_cast<const up_u32x4*>(wf + H_W2 + l16 * H_K2 + 32 * m + 8 * lg);
+        w3[m] = *reinterpret_cast<const up_u32x4*>(wf + H_W3 + l16 * H_K2 + 32 * m + 8 * lg);
+    }
+    float b1[4], b2[4], b3[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        b1[r] = prm[U_B1 + 4 * lg + r];
+        b2[r] = prm[U_B2 + 4 * lg + r];
+        b3[r] = 4 * lg + r < 2 ? prm[U_B3 + 4 * lg + r] : 0.0f;
+    }
+
+    // input tile: (u, v) = bilinear x scale, luma; zero outside the frame
+    for (int e = tid; e < UF_XH * UF_XW; e += 256) {
+        const int iy = y0 - 3 + e / UF_XW, ix = x0 - 3 + e % UF_XW;
+        uint2 v = make_uint2(0u, 0u);
+        if (iy >= 0 && iy < H && ix >= 0 && ix < W) {
+            const BiTap t = bilinear_tap(ix, iy, lw, lh, W, H);
+            const float u = bilinear_sample(fl, t, 0) * scale, w = bilinear_sample(fl, t, 1) * scale;
+            const uint8_t* px = rgba + ((long long)iy * W + ix) * 4;
+            const float luma = (0.299f * (float)px[0] + 0.587f * (float)px[1] + 0.114f * (float)px[2]) * (1.0f / 255.0f);
+            v = make_uint2(pack_h2(u, w), pack_h2(luma, 0.0f));
+        }
+        sx[e] = v;
+    }
+    __syncthreads();
+
+    // layer 1 on the 20 x 36 tile
+    constexpr int N1 = UF_1H * UF_1W, G1 = (N1 + 15) / 16;
+    for (int g = q; g < G1; g += 4) {
+        const int p = min(16 * g + l16, N1 - 1), r = p / UF_1W, c = p % UF_1W;
+        up_f32x4 acc = up_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        {
+            const int t0 = 2 * lg, t1 = 2 * lg + 1;
+            const uint2 a = sx[(r + t0 / 3) * UF_XW + c + t0 % 3], b = sx[(r + t1 / 3) * UF_XW + c + t1 % 3];
+            acc = up_mma(w1[0], up_u32x4{a.x, a.y, b.x, b.y}, acc);
+        }
+        {
+            const uint2 a = lg == 0 ? sx[(r + 2) * UF_XW + c + 2] : make_uint2(0u, 0u);
+            acc = up_mma(w1[1], up_u32x4{a.x, a.y, 0u, 0u}, acc);
+        }
+        const int iy = y0 - 2 + r, ix = x0 - 2 + c;
+        const bool in = iy >= 0 && iy < H && ix >= 0 && ix < W;
+        float o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = in ? fmaxf(acc[k] + b1[k], 0.0f) : 0.0f;
+        if (16 * g + l16 < N1)
+            *reinterpret_cast<uint2*>(&s1[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
+    }
+    __syncthreads();
+
+    // layer 2 on the 18 x 34 tile
+    constexpr int N2 = UF_2H * UF_2W, G2 = (N2 + 15) / 16;
+    for (int g = q; g < G2; g += 4) {
+        const int p = min(16 * g + l16, N2 - 1), r = p / UF_2W, c = p % UF_2W;
+        up_f32x4 acc = up_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const int tap = 2 * m + (lg >> 1);
+            up_u32x4 xb = up_u32x4{0u, 0u, 0u, 0u};
+            if (tap < 9) xb = *reinterpret_cast<const up_u32x4*>(&s1[((r + tap / 3) * UF_1W + c + tap % 3) * UF_AS + 8 * (lg & 1)]);
+            acc = up_mma(w2[m], xb, acc);
+        }
+        const int iy = y0 - 1 + r, ix = x0 - 1 + c;
+        const bool in = iy >= 0 && iy < H && ix >= 0 && ix < W;
+        float o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = in ? fmaxf(acc[k] + b2[k], 0.0f) : 0.0f;
+        if (16 * g + l16 < N2)
+            *reinterpret_cast<uint2*>(&s2[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
+    }
+    __syncthreads();
+
+    // layer 3 on the 16 x 32 output tile, added to the FP32 bilinear flow
+    const float gv = gate ? *gate : 1.0f;
+    constexpr int G3 = UF_TH * UF_TW / 16;
+    for (int g = q; g < G3; g += 4) {
+        const int p = 16 * g + l16, r = p / UF_TW, c = p % UF_TW;
+        up_f32x4 acc = up_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const int tap = 2 * m + (lg >> 1);
+            up_u32x4 xb = up_u32x4{0u, 0u, 0u, 0u};
+            if (tap < 9) xb = *reinterpret_cast<const up_u32x4*>(&s2[((r + tap / 3) * UF_2W + c + tap % 3) * UF_AS + 8 * (lg & 1)]);
+            acc = up_mma(w3[m], xb, acc);
+        }
+        const int iy = y0 + r, ix = x0 + c;
+        if (lg == 0 && iy < H && ix < W) {
+            const BiTap t = bilinear_tap(ix, iy, lw, lh, W, H);
+            const float u = bilinear_sample(fl, t, 0) * scale, w = bilinear_sample(fl, t, 1) * scale;
+            float* o = out + ((long long)iy * W + ix) * out_ch;
+            o[0] = u + gv * (acc[0] + b3[0]);
+            o[1] = w + gv * (acc[1] + b3[1]);
+            if (out_ch == 3) o[2] = 0.0f;
+        }
     }
 }
 
@@ -211,9 +364,10 @@ __global__ void k_fb_zero(uint32_t* c)
 __global__ void k_fb_count(const float* __restrict__ f, const float* __restrict__ b, int w, int h, int ch,
                            uint32_t* count)
 {
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    bool bad = false;
-    if (i < (long long)w * h) {
+    uint32_t n = 0;
+    const long long npx = (long long)w * h;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < npx; i += (long long)gridDim.x * 256) {
+        bool bad;
         const int x = (int)(i % w), y = (int)(i / w);
         const float fx = f[i * ch], fy = f[i * ch + 1];
         const float tx = (float)x + fx, ty = (float)y + fy;
@@ -234,9 +388,17 @@ __global__ void k_fb_count(const float* __restrict__ f, const float* __restrict_
             const float rhs = 0.01f * (fx * fx + fy * fy + bv[0] * bv[0] + bv[1] * bv[1]) + 0.5f;
             bad = !(lhs < rhs);
         }
+        n += bad ? 1u : 0u;
     }
-    const unsigned long long m = __ballot(bad);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (uint32_t)__popcll(m));
+    // block sum, one atomic per block (few hundred per frame)
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+    __shared__ uint32_t wsum[4];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x / 64] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (t) atomicAdd(count, t);
+    }
 }
 
 __global__ void k_fb_gate(const uint32_t* count, long long npx, float max_frac, float* gate)
@@ -462,53 +624,27 @@ extern "C" int vsc_upsampler_set_params(vsc_upsampler* u, const float* host)
     return check_hip(hipDeviceSynchronize(), "vsc_upsampler_set_params");
 }
 
-// inference workspace: x16 [px][8] FP16 | base [px][2] FP32 | a1, a2 [px][16] FP16, each 256-byte aligned
-static void ws_layout(size_t px, size_t off[4], size_t* total)
-{
-    const size_t sz[4] = {px * 16, px * 8, px * 32, px * 32};
-    size_t o = 0;
-    for (int i = 0; i < 4; ++i) {
-        off[i] = o;
-        o += (sz[i] + 255) / 256 * 256;
-    }
-    *total = o;
-}
-
+// the fused inference kernel keeps its activations in LDS: no workspace
 extern "C" size_t vsc_upsampler_workspace_size(int width, int height)
 {
-    if (width <= 0 || height <= 0) return 0;
-    size_t off[4], total;
-    ws_layout((size_t)width * height, off, &total);
-    return total;
+    (void)width;
+    (void)height;
+    return 0;
 }
 
 extern "C" int vsc_upsampler_run(vsc_upsampler* u, const float* flow_lo, int lo_w, int lo_h, int lo_ch,
                                  const uint8_t* rgba, int width, int height, float scale, float* flow_out, int out_ch,
                                  const float* gate, void* workspace, vsc_hip_stream stream)
 {
-    if (!u || !flow_lo || !rgba || !flow_out || !workspace)
-        return set_error(VSC_E_INVALID, "vsc_upsampler_run: NULL argument");
+    (void)workspace;
+    if (!u || !flow_lo || !rgba || !flow_out) return set_error(VSC_E_INVALID, "vsc_upsampler_run: NULL argument");
     if (lo_w <= 0 || lo_h <= 0 || width <= 0 || height <= 0 || (lo_ch != 2 && lo_ch != 3) ||
         (out_ch != 2 && out_ch != 3))
         return set_error(VSC_E_INVALID, "vsc_upsampler_run: bad shape");
-    hipStream_t s = as_stream(stream);
-    if (reinterpret_cast<uintptr_t>(workspace) % 256)
-        return set_error(VSC_E_INVALID, "vsc_upsampler_run: workspace must be 256-byte aligned");
-    size_t off[4], total;
-    ws_layout((size_t)width * height, off, &total);
-    char* ws = static_cast<char*>(workspace);
-    void* x16 = ws + off[0];
-    float* base = reinterpret_cast<float*>(ws + off[1]);
-    void* a1 = ws + off[2];
-    void* a2 = ws + off[3];
-    hipLaunchKernelGGL(k_ups_prep<EK_F16>, dim3((width + 255) / 256, height, 1), dim3(256), 0, s, flow_lo, lo_w, lo_h,
-                       lo_ch, 0LL, rgba, width, height, scale, x16, 8, base, out_ch == 3 ? flow_out : nullptr);
-    int rc = check_launch("upsampler prep");
-    if (rc) return rc;
-    if ((rc = conv(x16, 8, u->w16 + H_W1, u->p + U_B1, a1, 16, 16, 2, nullptr, 0, nullptr, EK_F16, EK_F16, 1, height, width, s))) return rc;
-    if ((rc = conv(a1, 16, u->w16 + H_W2, u->p + U_B2, a2, 16, 16, 2, nullptr, 0, nullptr, EK_F16, EK_F16, 1, height, width, s))) return rc;
-    return conv(a2, 16, u->w16 + H_W3, u->p + U_B3, flow_out, out_ch, 2, 0, base, 2, gate, EK_F16, EK_F32, 1, height,
-                width, s);
+    const int tx = (width + UF_TW - 1) / UF_TW, ty = (height + UF_TH - 1) / UF_TH;
+    hipLaunchKernelGGL(k_ups_fused, dim3(tx * ty), dim3(256), 0, as_stream(stream), flow_lo, lo_w, lo_h, lo_ch, rgba,
+                       width, height, scale, u->w16, u->p, gate, flow_out, out_ch, tx);
+    return check_launch("upsampler fused");
 }
 
 extern "C" int vsc_upsampler_loss_grad(vsc_upsampler* u, int batch, const float* flow_lo, int lo_w, int lo_h,
@@ -556,7 +692,7 @@ extern "C" int vsc_flow_fb_gate(const float* fwd, const float* bwd, int width, i
     hipStream_t s = as_stream(stream);
     const long long npx = (long long)width * height;
     hipLaunchKernelGGL(k_fb_zero, dim3(1), dim3(64), 0, s, count);
-    hipLaunchKernelGGL(k_fb_count, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, fwd, bwd, width, height,
+    hipLaunchKernelGGL(k_fb_count, dim3((unsigned)std::min<long long>(512, (npx + 255) / 256)), dim3(256), 0, s, fwd, bwd, width, height,
                        channels, count);
     hipLaunchKernelGGL(k_fb_gate, dim3(1), dim3(64), 0, s, count, npx, max_inconsistent, gate);
     return check_launch("vsc_flow_fb_gate");
