@@ -40,6 +40,7 @@ typedef struct ihipStream_t* vsc_hip_stream;
 #define VSC_E_HIP (-3)       /* a HIP runtime call or kernel launch failed        */
 #define VSC_E_NOMEM (-4)     /* device allocation failed                          */
 #define VSC_E_UNSUPPORTED (-5) /* valid request this build does not implement     */
+#define VSC_E_AGAIN (-6)     /* not now: a ring slot is still in use; poll, tick and retry */
 
 typedef enum {
     VSC_F32 = 0,
@@ -398,6 +399,63 @@ int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long long frame, co
                        uint8_t* out_rgba, float* out_float, vsc_hip_stream stream);
 /* The last frame's fallback gate (1 learned, 0 bilinear) and inconsistent count. */
 int vsc_pipeline_gate(const vsc_pipeline* p, float* gate, uint32_t* count, vsc_hip_stream stream);
+
+/* ---------------------------------------------------------------------------
+ * Multi-stream frame server: many concurrent video streams of different
+ * resolutions on one GPU (the reference serves one stream per process:
+ * VideoPlayer / stabilizefiles, videoplayer.cpp:185-190).  Streams are admitted
+ * by device-memory footprint against hbm_budget; each owns a vsc_pipeline
+ * (form: the pipeline configuration, by default the chosen form: BF16 flow
+ * network at FLOWDOWNSCALE 2 with the learned upsampler), a 3-slot ring of
+ * input frames and a 3-slot output ring.  vsc_server_submit copies frame t of
+ * a stream into its ring (host or device memory) on the stream's HIP stream;
+ * from frame 2 on, each submission makes the window centred at t-1 ready.
+ * vsc_server_tick launches every ready window of every stream, streams spread
+ * over hip_streams HIP streams (default priority) so their kernels overlap;
+ * streams opened or frames submitted while earlier work runs join the next
+ * tick.  vsc_server_poll reports finished frames in order per stream; a
+ * reported frame's output stays in the stream's 4-slot output ring until the
+ * next tick (read it with vsc_server_read before ticking again).  A window is
+ * launched only when its output slot has been reported, and frame t is taken
+ * only when window t-2 has been launched; otherwise submit returns
+ * VSC_E_AGAIN.  Parameter changes (vsc_server_set_params) apply from the next
+ * tick, as the GUI's sliders do (videostabilizer.cpp:192).                   */
+typedef struct vsc_server vsc_server;
+
+typedef struct {
+    int max_streams;
+    size_t hbm_budget;        /* bytes of device memory the server may commit */
+    int hip_streams;          /* HIP streams in the pool (1..16) */
+    vsc_pipeline_cfg form;
+} vsc_server_cfg;
+
+typedef struct {
+    int stream;
+    long long tag;            /* the tag given with the submission that completed the window */
+    long long frame;          /* stabilized frame index (window centre) */
+    const uint8_t* out_rgba;  /* device RGBA8 W x H */
+} vsc_server_done;
+
+void vsc_server_default_cfg(vsc_server_cfg* cfg);
+/* upsampler_params: the learned upsampler's weights (vsc_upsampler layout),
+ * required when the form uses it. */
+int vsc_server_create(const vsc_server_cfg* cfg, const float* upsampler_params, vsc_server** out);
+int vsc_server_destroy(vsc_server* sv);
+size_t vsc_server_stream_footprint(const vsc_server* sv, int width, int height);
+size_t vsc_server_used_bytes(const vsc_server* sv);
+/* VSC_E_NOMEM when the budget (or the stream table) cannot admit the stream now */
+int vsc_server_open(vsc_server* sv, int width, int height, int* stream_id);
+int vsc_server_close(vsc_server* sv, int stream_id);
+int vsc_server_set_params(vsc_server* sv, int stream_id, const vsc_params* params);
+int vsc_server_submit(vsc_server* sv, int stream_id, const uint8_t* orig_rgba, const uint8_t* proc_rgba, long long tag);
+int vsc_server_tick(vsc_server* sv, int* launched);
+int vsc_server_poll(vsc_server* sv, vsc_server_done* done, int max, int* count);
+/* wait for all launched work */
+int vsc_server_drain(vsc_server* sv);
+/* Copy stabilized frame `frame` (still in the output ring) of a stream into
+ * dst (device RGBA8), ordered after the stream's work and before later work on
+ * `stream`. */
+int vsc_server_read(vsc_server* sv, int stream_id, long long frame, uint8_t* dst, vsc_hip_stream stream);
 
 /* ---------------------------------------------------------------------------
  * Per-kernel timing (replaces the per-stage QElapsedTimer accounting of

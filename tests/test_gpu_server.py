@@ -1,0 +1,119 @@
+"""Multi-stream server: each stream's frames are the same bits alone and in a
+mix of concurrent streams of other resolutions (continuous batching over a
+pool of HIP streams), equal to the single-stream pipeline; admission by the
+device-memory budget; parameter changes apply from the next tick."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(192, 128), (256, 144), (128, 96)]
+
+
+def _video(vsc, W, H, n, seed):
+    orig, proc, _, _ = vsc.synth_video(width=W, height=H, nframes=n, nblobs=50, motion=3.0, npatches=1,
+                                       patch_size=max(8, W // 8), seed=seed)
+    return orig, proc
+
+
+def _run(vsc, cuda, streams, order, form="chosen", hip_streams=4):
+    """streams: {name: (W, H, orig, proc)}; order: list of names, one submission
+    each (in that order), ticks after every submission.  Returns {name: {frame: out}}."""
+    import torch
+    sv = vsc.Server(form=form, hip_streams=hip_streams)
+    ids = {k: sv.open(v[0], v[1]) for k, v in streams.items()}
+    frames = {k: torch.from_numpy(np.concatenate([v[2][None], v[3][None]])).to(cuda) for k, v in streams.items()}
+    nxt = {k: 0 for k in streams}
+    outs = {k: {} for k in streams}
+    def collect():
+        for sid, tag, frame, _ in sv.poll():
+            k = [n for n, i in ids.items() if i == sid][0]
+            assert tag == frame + 1
+            outs[k][frame] = sv.read(sid, frame).cpu()
+
+    for name in order:
+        t = nxt[name]
+        while True:
+            try:
+                sv.submit(ids[name], frames[name][0][t], frames[name][1][t], tag=t)
+                break
+            except vsc.VscError as e:           # ring slot still in use: let work finish
+                assert e.code == vsc.VSC_E_AGAIN
+                sv.drain()
+                collect()
+                sv.tick()
+        nxt[name] += 1
+        sv.tick()
+        collect()
+    for _ in range(3):
+        sv.drain()
+        collect()
+        sv.tick()
+    sv.close()
+    return outs
+
+
+def test_server_alone_equals_mix_equals_pipeline(vsc, cuda):
+    import torch
+    vids = {f"s{i}": (W, H) + _video(vsc, W, H, 7, seed=30 + i) for i, (W, H) in enumerate(SIZES)}
+    alone = _run(vsc, cuda, {"s0": vids["s0"]}, ["s0"] * 7)
+    rng = np.random.default_rng(3)
+    order = ["s0"] * 7 + ["s1"] * 7 + ["s2"] * 7
+    rng.shuffle(order)
+    mix = _run(vsc, cuda, vids, order)
+    assert sorted(alone["s0"]) == [1, 2, 3, 4, 5]
+    for f in alone["s0"]:
+        assert torch.equal(alone["s0"][f], mix["s0"][f]), f"frame {f}"
+    # the server adds scheduling only: same bits as the single-stream pipeline
+    W, H, orig, proc = vids["s1"]
+    o = torch.from_numpy(orig).to(cuda)
+    p = torch.from_numpy(proc).to(cuda)
+    pipe = vsc.Pipeline(W, H, form="chosen")
+    pipe.set_last_rgba(p[2])
+    for t in range(1, 6):
+        ref = pipe.frame((o[t - 1], o[t], o[t + 1], p[t - 1], p[t], p[t + 1]), t).cpu()
+        assert torch.equal(ref, mix["s1"][t]), f"pipeline frame {t}"
+
+
+def test_server_admission_by_budget(vsc, cuda):
+    probe = vsc.Server(hbm_budget=1 << 40)
+    need = probe.footprint(256, 144)
+    probe.close()
+    sv = vsc.Server(hbm_budget=int(2.5 * need))
+    a = sv.open(256, 144)
+    b = sv.open(256, 144)
+    with pytest.raises(vsc.VscError):
+        sv.open(256, 144)                       # a third does not fit
+    assert sv.used_bytes() == 2 * need
+    sv.close_stream(a)
+    c = sv.open(256, 144)                       # admitted once memory is free
+    assert c >= 0 and sv.used_bytes() == 2 * need
+    sv.close()
+
+
+def test_server_params_apply_from_next_tick(vsc, cuda):
+    import torch
+    W, H = SIZES[0]
+    orig, proc = _video(vsc, W, H, 6, seed=9)
+    o = torch.from_numpy(orig).to(cuda)
+    p = torch.from_numpy(proc).to(cuda)
+    sv = vsc.Server(form="plain")
+    sid = sv.open(W, H)
+    hp = vsc.HyperParams()
+    hp.gamma = 0.5
+    got = {}
+    for t in range(6):
+        if t == 4:
+            sv.set_params(sid, hp)             # frames stabilized from now on use gamma 0.5
+        sv.submit(sid, o[t], p[t], tag=t)
+        sv.tick()
+        sv.drain()
+        for _, _, frame, _ in sv.poll():
+            got[frame] = sv.read(sid, frame).cpu()
+    sv.close()
+    pipe = vsc.Pipeline(W, H, form="plain")
+    pipe.set_last_rgba(p[2])
+    for t in range(1, 5):
+        prm = hp if t >= 3 else None
+        ref = pipe.frame((o[t - 1], o[t], o[t + 1], p[t - 1], p[t], p[t + 1]), t, params=prm).cpu()
+        assert torch.equal(ref, got[t]), f"frame {t}"

@@ -30,6 +30,7 @@ LIB_PATH = os.path.join(_HERE, "libvsc.so")
 SYNTH_PATH = os.path.join(_HERE, "libvsc_synth.so")
 
 VSC_OK = 0
+VSC_E_AGAIN = -6
 VSC_F32, VSC_F64, VSC_BF16, VSC_F16, VSC_FP8E4M3 = 0, 1, 2, 3, 4
 VSC_FRAME_FUSED, VSC_FRAME_STAGED = 0, 1
 VSC_MAX_LEVELS = 4
@@ -51,6 +52,9 @@ EXPORTED = (
     "vsc_upsampler_loss_grad", "vsc_flow_fb_gate", "vsc_pipeline_default_cfg", "vsc_pipeline_create",
     "vsc_pipeline_destroy", "vsc_pipeline_footprint", "vsc_pipeline_set_upsampler_params", "vsc_pipeline_flownet",
     "vsc_pipeline_stream", "vsc_pipeline_flows", "vsc_pipeline_frame", "vsc_pipeline_gate",
+    "vsc_server_default_cfg", "vsc_server_create", "vsc_server_destroy", "vsc_server_stream_footprint",
+    "vsc_server_used_bytes", "vsc_server_open", "vsc_server_close", "vsc_server_set_params", "vsc_server_submit",
+    "vsc_server_tick", "vsc_server_poll", "vsc_server_drain", "vsc_server_read",
 )
 
 
@@ -80,6 +84,16 @@ class PipelineCfg(ctypes.Structure):
 class Window(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in ("orig_prev", "orig_cur", "orig_next", "proc_prev", "proc_cur",
                                                 "proc_next")]
+
+
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("max_streams", ctypes.c_int), ("hbm_budget", ctypes.c_size_t), ("hip_streams", ctypes.c_int),
+                ("form", PipelineCfg)]
+
+
+class ServerDone(ctypes.Structure):
+    _fields_ = [("stream", ctypes.c_int), ("tag", ctypes.c_longlong), ("frame", ctypes.c_longlong),
+                ("out_rgba", ctypes.c_void_p)]
 
 
 class SynthCfg(ctypes.Structure):
@@ -180,6 +194,22 @@ def lib():
         L.vsc_pipeline_frame.argtypes = [vp, ctypes.POINTER(Window), ctypes.c_longlong, ctypes.POINTER(VscParams),
                                          vp, vp, vp]
         L.vsc_pipeline_gate.argtypes = [vp, vp, vp, vp]
+        L.vsc_server_default_cfg.argtypes = [ctypes.POINTER(ServerCfg)]
+        L.vsc_server_default_cfg.restype = None
+        L.vsc_server_create.argtypes = [ctypes.POINTER(ServerCfg), vp, ctypes.POINTER(vp)]
+        L.vsc_server_destroy.argtypes = [vp]
+        L.vsc_server_stream_footprint.argtypes = [vp, i, i]
+        L.vsc_server_stream_footprint.restype = ctypes.c_size_t
+        L.vsc_server_used_bytes.argtypes = [vp]
+        L.vsc_server_used_bytes.restype = ctypes.c_size_t
+        L.vsc_server_open.argtypes = [vp, i, i, ctypes.POINTER(i)]
+        L.vsc_server_close.argtypes = [vp, i]
+        L.vsc_server_set_params.argtypes = [vp, i, ctypes.POINTER(VscParams)]
+        L.vsc_server_submit.argtypes = [vp, i, vp, vp, ctypes.c_longlong]
+        L.vsc_server_tick.argtypes = [vp, ctypes.POINTER(i)]
+        L.vsc_server_poll.argtypes = [vp, ctypes.POINTER(ServerDone), i, ctypes.POINTER(i)]
+        L.vsc_server_drain.argtypes = [vp]
+        L.vsc_server_read.argtypes = [vp, i, ctypes.c_longlong, vp, vp]
         L.vsc_profile_begin.argtypes = [i]
         L.vsc_profile_end.argtypes = [ctypes.c_char_p, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -770,6 +800,86 @@ class Pipeline:
     def close(self):
         if getattr(self, "h", None):
             lib().vsc_pipeline_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Multi-stream frame server (vsc_server_*): admission by device-memory
+    budget, continuous batching of ready windows over a pool of HIP streams."""
+
+    def __init__(self, hbm_budget=64 << 30, max_streams=64, hip_streams=4, form="chosen", upsampler_params=None):
+        c = ServerCfg()
+        lib().vsc_server_default_cfg(ctypes.byref(c))
+        c.max_streams, c.hbm_budget, c.hip_streams = max_streams, hbm_budget, hip_streams
+        fk = Pipeline.FORMS[form]
+        c.form.flow_dtype = {"f32": VSC_F32, "bf16": VSC_BF16}[fk["flow_dtype"]]
+        c.form.downscale = fk["downscale"]
+        c.form.upsample = {"bilinear": 0, "learned": 1}[fk["upsample"]]
+        self._params = None
+        if c.form.upsample == 1:
+            self._params = np.ascontiguousarray(
+                upsampler_params if upsampler_params is not None else load_upsampler_weights(), dtype=np.float32)
+        h = ctypes.c_void_p()
+        check(lib().vsc_server_create(ctypes.byref(c), None if self._params is None else self._params.ctypes.data,
+                                      ctypes.byref(h)))
+        self.h = h
+        self.max_streams = max_streams
+        self.sizes = {}
+
+    def footprint(self, width, height):
+        return int(lib().vsc_server_stream_footprint(self.h, width, height))
+
+    def used_bytes(self):
+        return int(lib().vsc_server_used_bytes(self.h))
+
+    def open(self, width, height):
+        i = ctypes.c_int()
+        check(lib().vsc_server_open(self.h, width, height, ctypes.byref(i)))
+        self.sizes[i.value] = (width, height)
+        return i.value
+
+    def close_stream(self, sid):
+        check(lib().vsc_server_close(self.h, sid))
+        self.sizes.pop(sid, None)
+
+    def set_params(self, sid, params: "HyperParams"):
+        prm = params.c()
+        check(lib().vsc_server_set_params(self.h, sid, ctypes.byref(prm)))
+
+    def submit(self, sid, orig, proc, tag=0):
+        check(lib().vsc_server_submit(self.h, sid, _ptr(orig), _ptr(proc), int(tag)))
+
+    def tick(self):
+        n = ctypes.c_int()
+        check(lib().vsc_server_tick(self.h, ctypes.byref(n)))
+        return n.value
+
+    def poll(self, max_items=1024):
+        arr = (ServerDone * max_items)()
+        n = ctypes.c_int()
+        check(lib().vsc_server_poll(self.h, arr, max_items, ctypes.byref(n)))
+        return [(arr[k].stream, arr[k].tag, arr[k].frame, arr[k].out_rgba) for k in range(n.value)]
+
+    def read(self, sid, frame):
+        """Stabilized frame `frame` of stream sid as a new device tensor (stream-ordered)."""
+        torch = _torch()
+        w, h = self.sizes[sid]
+        t = torch.empty((h, w, 4), dtype=torch.uint8, device="cuda")
+        check(lib().vsc_server_read(self.h, sid, int(frame), _ptr(t), _stream()))
+        return t
+
+    def drain(self):
+        check(lib().vsc_server_drain(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vsc_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -1,0 +1,321 @@
+// Multi-stream frame server: many concurrent video streams of different
+// resolutions on one GPU.  Each admitted stream owns a vsc_pipeline, a 3-slot
+// device ring of original and processed frames, a 3-slot output ring and its
+// hyper-parameters.  Frames are submitted as they arrive; every tick launches
+// the stabilization of each stream whose next window is complete, streams
+// spread over a pool of HIP streams so that their kernels overlap on the
+// device (continuous batching: a stream that arrives or is admitted between
+// ticks joins at the next tick, while earlier work is still running).
+// Admission is by device-memory footprint against a budget; a refused stream
+// can be opened again once others have closed.
+//
+// Frame protocol per stream (k = 1, B = 1; videostabilizer.cpp:136-164): frame
+// t's original and processed images are copied into ring slot t % 3 on the
+// stream's HIP stream.  Submitting frame 2 sets `last` to processed frame 2
+// (preloadProcessedFrames); from then on, submitting frame t makes window
+// (t-2, t-1, t) ready, and the next tick stabilizes frame t-1 into output
+// slot (t-1) % 4 once that slot's previous frame has been reported by poll.  Frames 0 and the final frame are the processed frames
+// unchanged, as in the reference (the caller owns them).
+#include "vsc_internal.h"
+
+#include <deque>
+#include <new>
+#include <vector>
+
+namespace vsc {
+namespace {
+
+struct Done {
+    long long tag;
+    long long frame;
+    int slot;
+    hipEvent_t ev;
+};
+
+struct Stream {
+    bool open = false;
+    int W = 0, H = 0;
+    size_t bytes = 0;
+    vsc_pipeline* pipe = nullptr;
+    uint8_t* orig[3] = {nullptr, nullptr, nullptr};
+    uint8_t* proc[3] = {nullptr, nullptr, nullptr};
+    uint8_t* out[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t hs = nullptr;       // borrowed from the pool
+    long long next_frame = 0;       // index of the next submitted frame
+    long long ready = -1;           // highest window centre ready to run
+    long long done_to = 0;          // highest window centre launched
+    long long polled_to = 0;        // highest window centre reported by poll
+    std::deque<long long> tags;     // tag of each ready window, in order
+    vsc_params params{};
+    std::vector<Done> inflight;
+};
+
+}  // namespace
+}  // namespace vsc
+
+struct vsc_server {
+    vsc_server_cfg cfg;
+    std::vector<float> ups_params;
+    std::vector<vsc::Stream> s;
+    std::vector<hipStream_t> pool;
+    size_t used = 0;
+    int rr = 0;
+};
+
+namespace vsc {
+namespace {
+
+constexpr int R_OUT = 4;
+size_t ring_bytes(int W, int H) { return (size_t)W * H * 4 * (6 + R_OUT); }
+
+void release(Stream& st)
+{
+    for (Done& d : st.inflight) (void)hipEventDestroy(d.ev);
+    st.inflight.clear();
+    if (st.pipe) vsc_pipeline_destroy(st.pipe);
+    st.pipe = nullptr;
+    if (st.orig[0]) (void)hipFree(st.orig[0]);
+    for (int i = 0; i < 3; ++i) st.orig[i] = st.proc[i] = nullptr;
+    for (int i = 0; i < R_OUT; ++i) st.out[i] = nullptr;
+    st.tags.clear();
+    st.open = false;
+}
+
+}  // namespace
+}  // namespace vsc
+
+using namespace vsc;
+
+extern "C" void vsc_server_default_cfg(vsc_server_cfg* c)
+{
+    if (!c) return;
+    c->max_streams = 64;
+    c->hbm_budget = (size_t)64 << 30;
+    c->hip_streams = 4;
+    vsc_pipeline_default_cfg(&c->form);
+    c->form.flow_dtype = VSC_BF16;
+    c->form.downscale = 2;
+    c->form.upsample = 1;
+}
+
+extern "C" int vsc_server_create(const vsc_server_cfg* cfg, const float* upsampler_params, vsc_server** out)
+{
+    if (!cfg || !out) return set_error(VSC_E_INVALID, "vsc_server_create: NULL argument");
+    *out = nullptr;
+    if (cfg->max_streams < 1 || cfg->max_streams > 4096 || cfg->hip_streams < 1 || cfg->hip_streams > 16)
+        return set_error(VSC_E_INVALID, "vsc_server_create: max_streams in [1, 4096], hip_streams in [1, 16]");
+    if (cfg->form.downscale > 1 && cfg->form.upsample == 1 && !upsampler_params)
+        return set_error(VSC_E_INVALID, "vsc_server_create: the learned upsampler needs its parameters");
+    vsc_server* sv = new (std::nothrow) vsc_server();
+    if (!sv) return set_error(VSC_E_NOMEM, "vsc_server_create: host memory");
+    sv->cfg = *cfg;
+    if (upsampler_params) sv->ups_params.assign(upsampler_params, upsampler_params + vsc_upsampler_param_count());
+    sv->s.resize(cfg->max_streams);
+    for (int i = 0; i < cfg->hip_streams; ++i) {
+        hipStream_t h = nullptr;
+        if (hipStreamCreateWithFlags(&h, hipStreamNonBlocking) != hipSuccess) {
+            vsc_server_destroy(sv);
+            return set_error(VSC_E_HIP, "vsc_server_create: stream creation failed");
+        }
+        sv->pool.push_back(h);
+    }
+    *out = sv;
+    return VSC_OK;
+}
+
+extern "C" int vsc_server_destroy(vsc_server* sv)
+{
+    if (!sv) return VSC_OK;
+    for (hipStream_t h : sv->pool) (void)hipStreamSynchronize(h);
+    for (Stream& st : sv->s)
+        if (st.open) release(st);
+    for (hipStream_t h : sv->pool) (void)hipStreamDestroy(h);
+    delete sv;
+    return VSC_OK;
+}
+
+extern "C" size_t vsc_server_stream_footprint(const vsc_server* sv, int width, int height)
+{
+    if (!sv || width <= 0 || height <= 0) return 0;
+    return vsc_pipeline_footprint(width, height, &sv->cfg.form) + ring_bytes(width, height);
+}
+
+extern "C" size_t vsc_server_used_bytes(const vsc_server* sv) { return sv ? sv->used : 0; }
+
+extern "C" int vsc_server_open(vsc_server* sv, int width, int height, int* id)
+{
+    if (!sv || !id) return set_error(VSC_E_INVALID, "vsc_server_open: NULL argument");
+    *id = -1;
+    if (width <= 0 || height <= 0) return set_error(VSC_E_INVALID, "vsc_server_open: bad frame size");
+    int slot = -1;
+    for (int i = 0; i < (int)sv->s.size(); ++i)
+        if (!sv->s[i].open) { slot = i; break; }
+    if (slot < 0) return set_error(VSC_E_NOMEM, "vsc_server_open: all stream slots are in use");
+    const size_t need = vsc_server_stream_footprint(sv, width, height);
+    if (need == 0) return set_error(VSC_E_INVALID, "vsc_server_open: invalid configuration");
+    if (sv->used + need > sv->cfg.hbm_budget)
+        return set_error(VSC_E_NOMEM, "vsc_server_open: device memory budget exhausted (retry after a stream closes)");
+    Stream& st = sv->s[slot];
+    st = Stream();
+    st.W = width; st.H = height;
+    int rc = vsc_pipeline_create(width, height, &sv->cfg.form, &st.pipe);
+    if (!rc && !sv->ups_params.empty() && sv->cfg.form.upsample == 1 && sv->cfg.form.downscale > 1)
+        rc = vsc_pipeline_set_upsampler_params(st.pipe, sv->ups_params.data());
+    if (!rc) {
+        uint8_t* ring = nullptr;
+        const size_t fb = (size_t)width * height * 4;
+        if (hipMalloc(&ring, fb * (6 + R_OUT)) != hipSuccess) {
+            rc = set_error(VSC_E_NOMEM, "vsc_server_open: ring allocation failed");
+        } else {
+            for (int i = 0; i < 3; ++i) {
+                st.orig[i] = ring + fb * i;
+                st.proc[i] = ring + fb * (3 + i);
+            }
+            for (int i = 0; i < R_OUT; ++i) st.out[i] = ring + fb * (6 + i);
+        }
+    }
+    if (rc) {
+        release(st);
+        return rc;
+    }
+    vsc_default_params(&st.params);
+    st.hs = sv->pool[sv->rr++ % sv->pool.size()];
+    st.bytes = need;
+    st.open = true;
+    sv->used += need;
+    *id = slot;
+    return VSC_OK;
+}
+
+static Stream* get(vsc_server* sv, int id)
+{
+    if (!sv || id < 0 || id >= (int)sv->s.size() || !sv->s[id].open) return nullptr;
+    return &sv->s[id];
+}
+
+extern "C" int vsc_server_close(vsc_server* sv, int id)
+{
+    Stream* st = get(sv, id);
+    if (!st) return set_error(VSC_E_INVALID, "vsc_server_close: no such stream");
+    (void)hipStreamSynchronize(st->hs);
+    sv->used -= st->bytes;
+    release(*st);
+    return VSC_OK;
+}
+
+extern "C" int vsc_server_set_params(vsc_server* sv, int id, const vsc_params* p)
+{
+    Stream* st = get(sv, id);
+    if (!st || !p) return set_error(VSC_E_INVALID, "vsc_server_set_params: no such stream or NULL params");
+    st->params = *p;   // read by the next tick (hyperParams are copied per frame, videostabilizer.cpp:192)
+    return VSC_OK;
+}
+
+extern "C" int vsc_server_submit(vsc_server* sv, int id, const uint8_t* orig, const uint8_t* proc, long long tag)
+{
+    Stream* st = get(sv, id);
+    if (!st || !orig || !proc) return set_error(VSC_E_INVALID, "vsc_server_submit: no such stream or NULL frame");
+    const long long t = st->next_frame;
+    if (t >= 3 && st->done_to < t - 2)
+        return set_error(VSC_E_AGAIN, "vsc_server_submit: window t-2 not launched yet (poll, tick, retry)");
+    const size_t fb = (size_t)st->W * st->H * 4;
+    // the slot's previous frame (t - 3) was last read by window t - 2, enqueued
+    // earlier on this same HIP stream: stream order makes the overwrite safe
+    if (check_hip(hipMemcpyAsync(st->orig[t % 3], orig, fb, hipMemcpyDefault, st->hs), "vsc_server_submit") ||
+        check_hip(hipMemcpyAsync(st->proc[t % 3], proc, fb, hipMemcpyDefault, st->hs), "vsc_server_submit"))
+        return VSC_E_HIP;
+    if (t == 2) {
+        vsc_stream* vs = nullptr;
+        int rc = vsc_pipeline_stream(st->pipe, &vs);
+        if (!rc) rc = vsc_stream_set_last_rgba(vs, st->proc[2], reinterpret_cast<vsc_hip_stream>(st->hs));
+        if (rc) return rc;
+    }
+    if (t >= 2) {
+        st->ready = t - 1;
+        st->tags.push_back(tag);
+    }
+    st->next_frame = t + 1;
+    return VSC_OK;
+}
+
+extern "C" int vsc_server_tick(vsc_server* sv, int* launched)
+{
+    if (!sv) return set_error(VSC_E_INVALID, "vsc_server_tick: NULL server");
+    int n = 0;
+    for (Stream& st : sv->s) {
+        if (!st.open) continue;
+        while (st.done_to < st.ready && st.done_to + 1 - R_OUT <= st.polled_to) {
+            const long long c = st.done_to + 1;
+            vsc_window w{st.orig[(c - 1) % 3], st.orig[c % 3], st.orig[(c + 1) % 3],
+                         st.proc[(c - 1) % 3], st.proc[c % 3], st.proc[(c + 1) % 3]};
+            int rc = vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
+                                        reinterpret_cast<vsc_hip_stream>(st.hs));
+            if (rc) return rc;
+            Done d{st.tags.front(), c, (int)(c % R_OUT), nullptr};
+            st.tags.pop_front();
+            if (check_hip(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming), "vsc_server_tick") ||
+                check_hip(hipEventRecord(d.ev, st.hs), "vsc_server_tick"))
+                return VSC_E_HIP;
+            st.inflight.push_back(d);
+            st.done_to = c;
+            ++n;
+        }
+    }
+    if (launched) *launched = n;
+    return VSC_OK;
+}
+
+extern "C" int vsc_server_poll(vsc_server* sv, vsc_server_done* done, int max, int* count)
+{
+    if (!sv || !count || (max > 0 && !done)) return set_error(VSC_E_INVALID, "vsc_server_poll: NULL argument");
+    int n = 0;
+    for (int id = 0; id < (int)sv->s.size(); ++id) {
+        Stream& st = sv->s[id];
+        if (!st.open) continue;
+        size_t k = 0;
+        while (k < st.inflight.size() && n < max) {
+            Done& d = st.inflight[k];
+            const hipError_t q = hipEventQuery(d.ev);
+            if (q == hipErrorNotReady) break;   // later frames of this stream are not done either
+            if (q != hipSuccess) return check_hip(q, "vsc_server_poll");
+            done[n].stream = id;
+            done[n].tag = d.tag;
+            done[n].frame = d.frame;
+            done[n].out_rgba = st.out[d.slot];
+            st.polled_to = d.frame;
+            ++n;
+            (void)hipEventDestroy(d.ev);
+            ++k;
+        }
+        st.inflight.erase(st.inflight.begin(), st.inflight.begin() + k);
+    }
+    *count = n;
+    return VSC_OK;
+}
+
+extern "C" int vsc_server_drain(vsc_server* sv)
+{
+    if (!sv) return set_error(VSC_E_INVALID, "vsc_server_drain: NULL server");
+    for (hipStream_t h : sv->pool)
+        if (check_hip(hipStreamSynchronize(h), "vsc_server_drain")) return VSC_E_HIP;
+    return VSC_OK;
+}
+
+extern "C" int vsc_server_read(vsc_server* sv, int id, long long frame, uint8_t* dst, vsc_hip_stream stream)
+{
+    Stream* st = get(sv, id);
+    if (!st || !dst) return set_error(VSC_E_INVALID, "vsc_server_read: no such stream or NULL destination");
+    if (frame < 1 || frame > st->polled_to || frame + R_OUT <= st->done_to)
+        return set_error(VSC_E_INVALID, "vsc_server_read: frame not reported or no longer in the output ring");
+    const size_t fb = (size_t)st->W * st->H * 4;
+    hipEvent_t ev = nullptr;
+    if (check_hip(hipMemcpyAsync(dst, st->out[frame % R_OUT], fb, hipMemcpyDeviceToDevice, st->hs), "vsc_server_read") ||
+        check_hip(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "vsc_server_read") ||
+        check_hip(hipEventRecord(ev, st->hs), "vsc_server_read") ||
+        check_hip(hipStreamWaitEvent(as_stream(stream), ev, 0), "vsc_server_read")) {
+        if (ev) (void)hipEventDestroy(ev);
+        return VSC_E_HIP;
+    }
+    (void)hipEventDestroy(ev);
+    return VSC_OK;
+}
