@@ -43,9 +43,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 PSNR")
-    ap.add_argument("--workload", choices=["pipeline", "stabilize", "ops"], default="pipeline",
+    ap.add_argument("--workload", choices=["pipeline", "stabilize", "ops", "serve"], default="pipeline",
                     help="pipeline: 1080p frames with computed flows, chosen vs plain form (headline); "
-                         "stabilize: 1080p frames with flows given; ops: Correlation/Warp on the 1920x1088 pyramid")
+                         "stabilize: 1080p frames with flows given; ops: Correlation/Warp on the 1920x1088 pyramid; "
+                         "serve: 24 concurrent streams of mixed resolutions at 30 fps (multi-stream server)")
+    ap.add_argument("--serve-frames", type=int, default=60, help="frames per stream (serve workload)")
+    ap.add_argument("--serve-fps", type=float, default=30.0, help="arrival rate per stream (serve workload)")
     ap.add_argument("--plain-steps", type=int, default=0, help="timed frames of the plain form (default steps/3)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
@@ -359,6 +362,94 @@ def bench_pipeline(args):
     return 0
 
 
+SERVE_STREAMS = [(640, 360)] * 8 + [(1280, 720)] * 8 + [(1920, 1080)] * 6 + [(3840, 2160)] * 2
+
+
+def bench_serve(args):
+    """24 concurrent streams (8 x 640x360, 8 x 1280x720, 6 x 1920x1080,
+    2 x 3840x2160), each its own seeded generator, frames arriving at
+    --serve-fps with zero-mean exponential jitter; the chosen form on the
+    multi-stream server.  Latency = time the stabilized frame is reported done
+    minus the arrival of the frame that completed its window."""
+    import importlib
+
+    import numpy as np
+    import torch
+
+    vsc = importlib.import_module("video-stream-consistency_amd")
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    uniq = 6   # distinct frames per stream, cycled (frame k = ring[k % uniq])
+    t0 = time.time()
+    src = []
+    for i, (w, h) in enumerate(SERVE_STREAMS):
+        o, p, _, _ = vsc.synth_video(width=w, height=h, nframes=uniq, nblobs=max(100, int(800 * w * h / (1920 * 1080))),
+                                     motion=8.0, npatches=4, patch_size=max(16, w // 15), seed=1000 + i)
+        src.append((torch.from_numpy(o).to(dev), torch.from_numpy(p).to(dev)))
+    gen_s = time.time() - t0
+    sv = vsc.Server(hbm_budget=128 << 30, max_streams=len(SERVE_STREAMS), hip_streams=4, form="chosen")
+    ids = [sv.open(w, h) for (w, h) in SERVE_STREAMS]
+    sid_to_i = {sid: i for i, sid in enumerate(ids)}
+    n = args.serve_frames
+    rng = np.random.default_rng(7)
+    jit = rng.exponential(2e-3, size=(len(ids), n)) - 2e-3
+    arrive = np.maximum(0.0, np.arange(n)[None, :] / args.serve_fps + jit + rng.uniform(0, 1 / args.serve_fps, (len(ids), 1)))
+    arrive = np.maximum.accumulate(arrive, axis=1)
+    nxt = [0] * len(ids)
+    lat, done_at = [], []
+    per_res = {}
+    expected = len(ids) * (n - 2)
+    got = 0
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    while got < expected:
+        now = time.perf_counter() - start
+        for i, sid in enumerate(ids):
+            while nxt[i] < n and arrive[i, nxt[i]] <= now:
+                k = nxt[i]
+                o, p = src[i]
+                try:
+                    sv.submit(sid, o[k % uniq], p[k % uniq], tag=k)
+                except vsc.VscError as e:
+                    if e.code != vsc.VSC_E_AGAIN:
+                        raise
+                    break
+                nxt[i] += 1
+        sv.tick()
+        t_done = time.perf_counter() - start
+        for sid, tag, frame, _ in sv.poll():
+            i = sid_to_i[sid]
+            l = t_done - arrive[i, tag]
+            lat.append(l)
+            done_at.append(t_done)
+            per_res.setdefault(SERVE_STREAMS[i], []).append(l)
+            got += 1
+        time.sleep(0.0002)
+    sv.close()
+    lat = np.array(lat) * 1000.0
+    span = max(done_at) - float(arrive.min())
+    mpx = sum(w * h for w, h in SERVE_STREAMS) * (n - 2) / 1e6
+    result = {
+        "metric": "multi-stream serving: aggregate stabilized frames/s and p50/p99 frame latency",
+        "value": round(got / span, 2), "unit": "frames/s", "n_gpus": 1, "steps": got, "warmup": 0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32",
+        "data": f"synthetic: {len(ids)} seeded streams, {uniq} distinct frames each cycled, resident in HBM",
+        "config": {"workload": f"{len(ids)} streams (8x640x360, 8x1280x720, 6x1920x1080, 2x3840x2160), "
+                               f"{n} frames each at {args.serve_fps} fps arrivals (zero-mean exponential jitter), "
+                               f"chosen form on the multi-stream server (4 HIP streams)"},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                       "max": round(float(lat.max()), 3)},
+        "latency_ms_by_size": {f"{w}x{h}": {"p50": round(1000.0 * float(np.percentile(v, 50)), 3),
+                                            "p99": round(1000.0 * float(np.percentile(v, 99)), 3)}
+                               for (w, h), v in sorted(per_res.items())},
+        "offered_mpx_per_s": round(sum(w * h for w, h in SERVE_STREAMS) * args.serve_fps / 1e6, 1),
+        "served_mpx_per_s": round(mpx / span, 1),
+        "span_s": round(span, 3), "gen_seconds": round(gen_s, 2),
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -371,6 +462,8 @@ def main():
         return bench_ops(args)
     if args.workload == "pipeline":
         return bench_pipeline(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     import importlib
 
     import numpy as np
