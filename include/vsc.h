@@ -288,6 +288,12 @@ int vsc_flownet_flow_size(const vsc_flownet* net, int* width, int* height);
 size_t vsc_flownet_param_count(const vsc_flownet* net);
 int vsc_flownet_get_params(const vsc_flownet* net, float* host, size_t count);
 int vsc_flownet_set_params(vsc_flownet* net, const float* host, size_t count);
+/* Multiply the weights and biases of the flow-output layers (each decoder's
+ * last conv and the refiner's last conv) by factor.  With seeded, untrained
+ * weights the network's flows reach ~100 px; a small factor gives flows of
+ * the magnitude of real motion, so that the warps' memory access pattern in a
+ * timing run resembles a trained network's. */
+int vsc_flownet_scale_flow_heads(vsc_flownet* net, float factor);
 /* rgba: device RGBA8 frame_width x frame_height (4-byte aligned) */
 int vsc_flownet_extract(vsc_flownet* net, const uint8_t* rgba, int slot, vsc_hip_stream stream);
 /* npairs 1|2; out[i]: device HWC float flow_width x flow_height x out_channels (2|3) */

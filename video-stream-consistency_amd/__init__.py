@@ -47,6 +47,7 @@ EXPORTED = (
     "vsc_stab_consist_out_planar", "vsc_conv3x3_nhwc", "vsc_flownet_create", "vsc_flownet_destroy",
     "vsc_flownet_footprint", "vsc_flownet_flow_size", "vsc_flownet_param_count", "vsc_flownet_get_params",
     "vsc_flownet_set_params", "vsc_flownet_extract", "vsc_flownet_flow", "vsc_flownet_intermediate",
+    "vsc_flownet_scale_flow_heads",
     "vsc_upsampler_create", "vsc_upsampler_destroy", "vsc_upsampler_param_count", "vsc_upsampler_get_params",
     "vsc_upsampler_set_params", "vsc_upsampler_workspace_size", "vsc_upsampler_run", "vsc_upsampler_train_step",
     "vsc_upsampler_loss_grad", "vsc_flow_fb_gate", "vsc_pipeline_default_cfg", "vsc_pipeline_create",
@@ -169,6 +170,7 @@ def lib():
         L.vsc_flownet_extract.argtypes = [vp, vp, i, vp]
         L.vsc_flownet_flow.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(vp), i, vp]
         L.vsc_flownet_intermediate.argtypes = [vp, i, i, i, vp, vp]
+        L.vsc_flownet_scale_flow_heads.argtypes = [vp, f]
         L.vsc_upsampler_create.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(vp)]
         L.vsc_upsampler_destroy.argtypes = [vp]
         L.vsc_upsampler_param_count.argtypes = []
@@ -307,6 +309,9 @@ class FlowNet:
         import numpy as np
         a = np.ascontiguousarray(a, dtype=np.float32)
         check(lib().vsc_flownet_set_params(self.h, a.ctypes.data, a.size))
+
+    def scale_flow_heads(self, factor):
+        check(lib().vsc_flownet_scale_flow_heads(self.h, factor))
 
     def extract(self, rgba, slot):
         check(lib().vsc_flownet_extract(self.h, _ptr(rgba), slot, _stream()))
@@ -725,7 +730,8 @@ class Pipeline:
     }
 
     def __init__(self, width, height, form=None, flow_dtype="f32", downscale=1, upsample="bilinear",
-                 scale_vectors=False, fb_max_inconsistent=0.25, ref_bwd_index=False, seed=0, upsampler_params=None):
+                 scale_vectors=False, fb_max_inconsistent=0.25, ref_bwd_index=False, seed=0, upsampler_params=None,
+                 flow_head_scale=None):
         if form is not None:
             kw = self.FORMS[form]
             flow_dtype, downscale, upsample = kw["flow_dtype"], kw["downscale"], kw["upsample"]
@@ -749,6 +755,10 @@ class Pipeline:
             p = upsampler_params if upsampler_params is not None else load_upsampler_weights()
             p = np.ascontiguousarray(p, dtype=np.float32)
             check(lib().vsc_pipeline_set_upsampler_params(self.h, p.ctypes.data))
+        if flow_head_scale is not None:
+            net = ctypes.c_void_p()
+            check(lib().vsc_pipeline_flownet(self.h, ctypes.byref(net)))
+            check(lib().vsc_flownet_scale_flow_heads(net, flow_head_scale))
 
     @staticmethod
     def footprint(width, height, form="plain"):

@@ -468,6 +468,22 @@ extern "C" int vsc_flownet_set_params(vsc_flownet* n, const float* host, size_t 
     return upload_params(n);
 }
 
+extern "C" int vsc_flownet_scale_flow_heads(vsc_flownet* n, float factor)
+{
+    if (!n || !(factor >= 0.0f)) return set_error(VSC_E_INVALID, "vsc_flownet_scale_flow_heads: bad argument");
+    // the layers whose outputs are flows: each decoder's last conv and the refiner's last conv
+    for (int lvl = 6; lvl >= 3; --lvl) {
+        Layer& l = n->L[n->dec0 + 6 * (6 - lvl) + 5];
+        for (float& x : l.w) x *= factor;
+        for (float& x : l.b) x *= factor;
+    }
+    Layer& r = n->L[n->ref0 + 6];
+    for (float& x : r.w) x *= factor;
+    for (float& x : r.b) x *= factor;
+    if (hipDeviceSynchronize() != hipSuccess) return set_error(VSC_E_HIP, "vsc_flownet_scale_flow_heads: sync");
+    return upload_params(n);
+}
+
 extern "C" int vsc_flownet_extract(vsc_flownet* n, const uint8_t* rgba, int slot, vsc_hip_stream stream)
 {
     if (!n || !rgba) return set_error(VSC_E_INVALID, "vsc_flownet_extract: NULL argument");
