@@ -12,11 +12,13 @@
 // bound by the 81 output planes it writes, not by the products.
 //
 // Block: 8 waves; wave q computes one 16-pixel segment (q & 1) of output row
-// h0 + q/2 for all (2*MD+1)^2 displacements, 72 FP32 accumulators per lane.  Channels
-// stream through LDS in chunks of 32: the f1 tile (4 rows x 32 px) and the f2
-// tile (12 rows x 40 px, MD-pixel halo, zeros outside the image) are stored
-// channels-last with a padded pixel stride, so that each lane's 8-channel MFMA
-// operand is one 16-byte (8-byte for FP8) LDS read and 8 lanes cover all banks.
+// h0 + q/2 for all (2*MD+1)^2 displacements, 72 FP32 accumulators per lane.  The
+// NCHW operands are first copied channels-last (channels padded to 32, zeros)
+// by a bandwidth-bound conversion kernel; channels then stream through LDS in
+// chunks of 32 as straight 8-byte copies: the f1 tile (4 rows x 32 px) and the
+// f2 tile (12 rows x 40 px, MD-pixel halo, zeros outside the image), with a
+// padded pixel stride, so that each lane's 8-channel MFMA operand is one
+// 16-byte (8-byte for FP8) LDS read and 8 lanes cover all banks.
 // After the channel loop, each wave moves the diagonals of its tiles through a
 // small LDS stage and writes every (dy, dx) output row as 32 consecutive floats.
 #include "vsc_internal.h"
@@ -59,7 +61,8 @@ __device__ inline f32x4 cm_mfma(const void* a, const void* b, f32x4 acc)
 
 template <int KIND, int MD, bool LEGACY>
 __global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v, const void* __restrict__ f2v,
-                                                   float* __restrict__ out, int C, int H, int W, const float* scale)
+                                                   float* __restrict__ out, int C, int CP, int H, int W,
+                                                   const float* scale)
 {
     constexpr int EB = (KIND == CM_FP8) ? 1 : 2;          // bytes per element
     constexpr int ST = CmLayout<EB>::STRIDE;              // LDS pixel stride in elements
@@ -76,13 +79,17 @@ __global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v,
     E (*sB)[BC][ST] = reinterpret_cast<E (*)[BC][ST]>(lds + A_BYTES);
     float (*stage)[P * P][CM_PX] = reinterpret_cast<float (*)[P * P][CM_PX]>(lds);
 
+    // operands are channels-last copies [n][y][x][CP] (CP: C padded to a
+    // multiple of 32 with zeros), so a chunk of 32 channels of one pixel is
+    // contiguous: staged with 8-byte vectors (VPP per pixel), no transpose
+    constexpr int VPP = CM_KC * EB / 8;
     const E* f1 = static_cast<const E*>(f1v);
     const E* f2 = static_cast<const E*>(f2v);
     const int n = blockIdx.z;
     const int h0 = blockIdx.y * CM_ROWS, w0 = blockIdx.x * CM_PX;
     const long long HW = (long long)H * W;
-    const E* a = f1 + (long long)n * C * HW;
-    const E* b = f2 + (long long)n * C * HW;
+    const E* a = f1 + (long long)n * HW * CP;
+    const E* b = f2 + (long long)n * HW * CP;
     const int tid = threadIdx.x;
     const int q = tid / 64, lane = tid % 64;
     const int qr = q >> 1, seg = q & 1;         // output row in the block, 16-pixel segment
@@ -94,50 +101,44 @@ __global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v,
 #pragma unroll
         for (int t = 0; t < 2; ++t) acc[d][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    for (int c0 = 0; c0 < C; c0 += CM_KC) {
-        // stage the f1 tile and the f2 tile (channels-last, zero outside the
-        // image and past C).  Global loads are issued from clamped addresses in
-        // batches of SB before their LDS stores, a few memory round trips per
-        // chunk instead of one per element.
-        constexpr int N1 = CM_KC * CM_ROWS * CM_PX, N2 = CM_KC * BR * BC;
-        constexpr int L1 = N1 / 512, L2 = N2 / 512, SB = 15;
-        static_assert(N1 % 512 == 0 && N2 % 512 == 0 && L2 % SB == 0, "staging assumes whole rounds");
-        __syncthreads();
-        {
-            E r1[L1];
+    constexpr int NA = CM_ROWS * CM_PX * VPP, NB = BR * BC * VPP, NU = NA + NB, RU = (NU + 511) / 512;
+    uint2 rg[RU];
+    auto fetch = [&](int c0) {
 #pragma unroll
-            for (int k = 0; k < L1; ++k) {
-                const int e = tid + 512 * k;
-                const int x = e % CM_PX, r = (e / CM_PX) % CM_ROWS, c = e / (CM_PX * CM_ROWS);
-                const int gh = min(h0 + r, H - 1), gw = min(w0 + x, W - 1), gc = min(c0 + c, C - 1);
-                const E v = a[gc * HW + (long long)gh * W + gw];
-                r1[k] = (c0 + c < C && h0 + r < H && w0 + x < W) ? v : (E)0;
+        for (int k = 0; k < RU; ++k) {
+            const int u = tid + 512 * k;
+            uint2 v = make_uint2(0u, 0u);
+            if (u < NA) {
+                const int p = u / VPP, cv = u % VPP, r = p / CM_PX, x = p % CM_PX;
+                const int gh = h0 + r, gw = w0 + x;
+                if (gh < H && gw < W)
+                    v = *reinterpret_cast<const uint2*>(a + ((long long)gh * W + gw) * CP + c0 + cv * (8 / EB));
+            } else if (u < NU) {
+                const int ub = u - NA, p = ub / VPP, cv = ub % VPP, r = p / BC, x = p % BC;
+                const int gh = h0 - CM_HALO + r, gw = w0 - CM_HALO + x;
+                if (gh >= 0 && gh < H && gw >= 0 && gw < W)
+                    v = *reinterpret_cast<const uint2*>(b + ((long long)gh * W + gw) * CP + c0 + cv * (8 / EB));
             }
-#pragma unroll
-            for (int k = 0; k < L1; ++k) {
-                const int e = tid + 512 * k;
-                sA[(e / CM_PX) % CM_ROWS][e % CM_PX][e / (CM_PX * CM_ROWS)] = r1[k];
-            }
+            rg[k] = v;
         }
+    };
+
+    fetch(0);
+    for (int c0 = 0; c0 < CP; c0 += CM_KC) {
+        __syncthreads();
 #pragma unroll
-        for (int k0 = 0; k0 < L2; k0 += SB) {
-            E r2[SB];
-#pragma unroll
-            for (int k = 0; k < SB; ++k) {
-                const int e = tid + 512 * (k0 + k);
-                const int x = e % BC, r = (e / BC) % BR, c = e / (BC * BR);
-                const int ghr = h0 - CM_HALO + r, gwr = w0 - CM_HALO + x;
-                const int gh = min(max(ghr, 0), H - 1), gw = min(max(gwr, 0), W - 1), gc = min(c0 + c, C - 1);
-                const E v = b[gc * HW + (long long)gh * W + gw];
-                r2[k] = (c0 + c < C && ghr >= 0 && ghr < H && gwr >= 0 && gwr < W) ? v : (E)0;
-            }
-#pragma unroll
-            for (int k = 0; k < SB; ++k) {
-                const int e = tid + 512 * (k0 + k);
-                sB[(e / BC) % BR][e % BC][e / (BC * BR)] = r2[k];
+        for (int k = 0; k < RU; ++k) {
+            const int u = tid + 512 * k;
+            if (u < NA) {
+                const int p = u / VPP, cv = u % VPP;
+                *reinterpret_cast<uint2*>(&sA[p / CM_PX][p % CM_PX][cv * (8 / EB)]) = rg[k];
+            } else if (u < NU) {
+                const int ub = u - NA, p = ub / VPP, cv = ub % VPP;
+                *reinterpret_cast<uint2*>(&sB[p / BC][p % BC][cv * (8 / EB)]) = rg[k];
             }
         }
         __syncthreads();
+        if (c0 + CM_KC < CP) fetch(c0 + CM_KC);
         // A operand: pixel lr of this wave's segment, channels 8*lg .. 8*lg+7
         const E* pa = &sA[qr][16 * seg + lr][8 * lg];
 #pragma unroll
@@ -183,26 +184,26 @@ __global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v,
 }
 
 template <int KIND, int MD>
-static void launch_mfma(const void* f1, const void* f2, float* out, int N, int C, int H, int W, bool legacy,
+static void launch_mfma(const void* f1, const void* f2, float* out, int N, int C, int CP, int H, int W, bool legacy,
                         const float* scale, hipStream_t s)
 {
     dim3 grid((W + CM_PX - 1) / CM_PX, (H + CM_ROWS - 1) / CM_ROWS, N);
     if (legacy)
-        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, true>), grid, dim3(512), 0, s, f1, f2, out, C, H, W, scale);
+        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, true>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
     else
-        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, false>), grid, dim3(512), 0, s, f1, f2, out, C, H, W, scale);
+        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, false>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
 }
 
 template <int KIND>
-static int dispatch_md(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, bool legacy,
-                       const float* scale, hipStream_t s)
+static int dispatch_md(const void* f1, const void* f2, float* out, int N, int C, int CP, int H, int W, int md,
+                       bool legacy, const float* scale, hipStream_t s)
 {
     switch (md) {
-        case 0: launch_mfma<KIND, 0>(f1, f2, out, N, C, H, W, legacy, scale, s); break;
-        case 1: launch_mfma<KIND, 1>(f1, f2, out, N, C, H, W, legacy, scale, s); break;
-        case 2: launch_mfma<KIND, 2>(f1, f2, out, N, C, H, W, legacy, scale, s); break;
-        case 3: launch_mfma<KIND, 3>(f1, f2, out, N, C, H, W, legacy, scale, s); break;
-        case 4: launch_mfma<KIND, 4>(f1, f2, out, N, C, H, W, legacy, scale, s); break;
+        case 0: launch_mfma<KIND, 0>(f1, f2, out, N, C, CP, H, W, legacy, scale, s); break;
+        case 1: launch_mfma<KIND, 1>(f1, f2, out, N, C, CP, H, W, legacy, scale, s); break;
+        case 2: launch_mfma<KIND, 2>(f1, f2, out, N, C, CP, H, W, legacy, scale, s); break;
+        case 3: launch_mfma<KIND, 3>(f1, f2, out, N, C, CP, H, W, legacy, scale, s); break;
+        case 4: launch_mfma<KIND, 4>(f1, f2, out, N, C, CP, H, W, legacy, scale, s); break;
         default:
             return set_error(VSC_E_UNSUPPORTED,
                              "vsc_correlation_forward: reduced-precision correlation supports max_displacement <= 4");
@@ -210,13 +211,111 @@ static int dispatch_md(const void* f1, const void* f2, float* out, int N, int C,
     return check_launch("correlation mfma");
 }
 
+// NCHW -> channels-last [n][y][x][CP], channels >= C zero.  One thread per
+// 4 consecutive pixels and 8-byte channel group (G = 4 BF16/F16 or 8 FP8
+// channels): G vector loads of 4 pixels of one channel (coalesced along x
+// across the wave), a register transpose, 4 vector stores of G channels.
+// Needs H*W % 4 == 0 for the vector loads; k_to_nhwc1 is the general form.
+template <typename E>
+__global__ void k_to_nhwc4(const E* __restrict__ in, E* __restrict__ out, int C, int CP, long long HW, int N)
+{
+    constexpr int G = 8 / sizeof(E);
+    using V4 = typename std::conditional<sizeof(E) == 2, uint2, uint32_t>::type;   // 4 pixels of one channel
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long quads = HW / 4;
+    const int groups = CP / G;
+    const long long p4 = e % quads;
+    const long long rest = e / quads;
+    if (rest >= (long long)groups * N) return;
+    const int g = (int)(rest % groups), n = (int)(rest / groups);
+    E v[G][4];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const int c = g * G + k;
+        V4 x;
+        if (c < C) x = *reinterpret_cast<const V4*>(in + ((long long)n * C + c) * HW + p4 * 4);
+        else __builtin_memset(&x, 0, sizeof(x));
+        __builtin_memcpy(v[k], &x, sizeof(x));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        E w[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) w[k] = v[k][j];
+        uint2 pk;
+        __builtin_memcpy(&pk, w, 8);
+        *reinterpret_cast<uint2*>(out + ((long long)n * HW + p4 * 4 + j) * CP + g * G) = pk;
+    }
+}
+
+template <typename E>
+__global__ void k_to_nhwc1(const E* __restrict__ in, E* __restrict__ out, int C, int CP, long long HW, int N)
+{
+    constexpr int G = 8 / sizeof(E);
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int groups = CP / G;
+    const long long p = e % HW;
+    const long long rest = e / HW;
+    if (rest >= (long long)groups * N) return;
+    const int g = (int)(rest % groups), n = (int)(rest / groups);
+    E v[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const int c = g * G + k;
+        v[k] = c < C ? in[((long long)n * C + c) * HW + p] : (E)0;
+    }
+    uint2 pk;
+    __builtin_memcpy(&pk, v, 8);
+    *reinterpret_cast<uint2*>(out + ((long long)n * HW + p) * CP + g * G) = pk;
+}
+
+template <typename E>
+static void to_nhwc(const void* in, void* out, int C, int CP, long long HW, int N, hipStream_t s)
+{
+    constexpr int G = 8 / sizeof(E);
+    if (HW % 4 == 0) {
+        const long long units = (HW / 4) * (CP / G) * N;
+        hipLaunchKernelGGL(k_to_nhwc4<E>, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const E*>(in), static_cast<E*>(out), C, CP, HW, N);
+    } else {
+        const long long units = HW * (CP / G) * N;
+        hipLaunchKernelGGL(k_to_nhwc1<E>, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const E*>(in), static_cast<E*>(out), C, CP, HW, N);
+    }
+}
+
 int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
                        int in_dtype, const float* chan_scale, hipStream_t s)
 {
-    if (in_dtype == VSC_BF16) return dispatch_md<CM_BF16>(f1, f2, out, N, C, H, W, md, legacy, nullptr, s);
-    if (in_dtype == VSC_F16) return dispatch_md<CM_F16>(f1, f2, out, N, C, H, W, md, legacy, nullptr, s);
-    if (in_dtype == VSC_FP8E4M3) return dispatch_md<CM_FP8>(f1, f2, out, N, C, H, W, md, legacy, chan_scale, s);
-    return set_error(VSC_E_DTYPE, "vsc_correlation_forward: unsupported input type");
+    if (in_dtype != VSC_BF16 && in_dtype != VSC_F16 && in_dtype != VSC_FP8E4M3)
+        return set_error(VSC_E_DTYPE, "vsc_correlation_forward: unsupported input type");
+    if (md > 4)
+        return set_error(VSC_E_UNSUPPORTED,
+                         "vsc_correlation_forward: reduced-precision correlation supports max_displacement <= 4");
+    const int eb = in_dtype == VSC_FP8E4M3 ? 1 : 2;
+    const int CP = (C + CM_KC - 1) / CM_KC * CM_KC;
+    const long long HW = (long long)H * W;
+    const size_t one = (size_t)N * HW * CP * eb;
+    void* ws = nullptr;
+    int rc = check_hip(hipMallocAsync(&ws, 2 * one, s), "vsc_correlation_forward workspace");
+    if (rc) return rc;
+    void* t1 = ws;
+    void* t2 = static_cast<char*>(ws) + one;
+    if (eb == 2) {
+        to_nhwc<uint16_t>(f1, t1, C, CP, HW, N, s);
+        to_nhwc<uint16_t>(f2, t2, C, CP, HW, N, s);
+    } else {
+        to_nhwc<uint8_t>(f1, t1, C, CP, HW, N, s);
+        to_nhwc<uint8_t>(f2, t2, C, CP, HW, N, s);
+    }
+    rc = check_launch("correlation to channels-last");
+    if (!rc) {
+        if (in_dtype == VSC_BF16) rc = dispatch_md<CM_BF16>(t1, t2, out, N, C, CP, H, W, md, legacy, nullptr, s);
+        else if (in_dtype == VSC_F16) rc = dispatch_md<CM_F16>(t1, t2, out, N, C, CP, H, W, md, legacy, nullptr, s);
+        else rc = dispatch_md<CM_FP8>(t1, t2, out, N, C, CP, H, W, md, legacy, chan_scale, s);
+    }
+    const int rc2 = check_hip(hipFreeAsync(ws, s), "vsc_correlation_forward workspace free");
+    return rc ? rc : rc2;
 }
 
 // ---------------------------------------------------------------------------
