@@ -232,20 +232,51 @@ __device__ inline uint32_t pack_h2(float a, float b)
     return (uint32_t)f32_to_f16(a) | ((uint32_t)f32_to_f16(b) << 16);
 }
 
-__global__ __launch_bounds__(256) void k_ups_fused(const float* __restrict__ lo, int lw, int lh, int lch,
+// one 16-pixel group of a 16-channel layer: K = 144 (9 taps x 16) in five
+// steps; off[m] is the lane's LDS offset of tap 2m + (lg >> 1) relative to
+// the group pixel's top-left tap (negative: the tap does not exist)
+__device__ inline up_f32x4 ups_l16(const uint16_t* src, const up_u32x4 (&w)[5], int base, const int (&off)[5])
+{
+    up_f32x4 acc = up_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    up_u32x4 xb[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        xb[m] = up_u32x4{0u, 0u, 0u, 0u};
+        if (off[m] >= 0) xb[m] = *reinterpret_cast<const up_u32x4*>(&src[base + off[m]]);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) acc = up_mma(w[m], xb[m], acc);
+    return acc;
+}
+
+__global__ __launch_bounds__(512) void k_ups_fused(const float* __restrict__ lo, int lw, int lh, int lch,
                                                    const uint8_t* __restrict__ rgba, int W, int H, float scale,
                                                    const uint16_t* __restrict__ wf, const float* __restrict__ prm,
                                                    const float* gate, float* __restrict__ out, int out_ch,
                                                    int tiles_x)
 {
+    constexpr int NW = 8;   // waves per block; each keeps two 16-pixel groups in flight
     __shared__ __attribute__((aligned(16))) uint2 sx[UF_XH * UF_XW];
     __shared__ __attribute__((aligned(16))) uint16_t s1[UF_1H * UF_1W * UF_AS];
     __shared__ __attribute__((aligned(16))) uint16_t s2[UF_2H * UF_2W * UF_AS];
+    __shared__ float2 sbase[UF_TH * UF_TW];          // FP32 bilinear flow of the output tile
+    __shared__ int tx0[UF_XW], tx1[UF_XW], ty0[UF_XH], ty1[UF_XH];
+    __shared__ float tfx[UF_XW], tfy[UF_XH];
     const int tid = threadIdx.x, q = tid / 64, lane = tid % 64, l16 = lane % 16, lg = lane / 16;
     const int y0 = (blockIdx.x / tiles_x) * UF_TH, x0 = (blockIdx.x % tiles_x) * UF_TW;
-    View fl{const_cast<float*>(lo), lw, lh, lch, 1, (long long)lch};
+    // bilinear taps of the tile's columns and rows (kernel_bilinear mapping)
+    if (tid < UF_XW) {
+        const int ix = min(max(x0 - 3 + tid, 0), W - 1);
+        const BiTap t = bilinear_tap(ix, 0, lw, lh, W, H);
+        tx0[tid] = t.ix; tx1[tid] = t.ix1; tfx[tid] = t.fx;
+    } else if (tid >= 64 && tid < 64 + UF_XH) {
+        const int iy = min(max(y0 - 3 + tid - 64, 0), H - 1);
+        const BiTap t = bilinear_tap(0, iy, lw, lh, W, H);
+        ty0[tid - 64] = t.iy; ty1[tid - 64] = t.iy1; tfy[tid - 64] = t.fy;
+    }
 
-    // weights (A operand rows co = l16) and biases for this lane's 4 output rows
+    // weights (A operand rows co = l16), biases for this lane's 4 output rows,
+    // and the lane's tap offsets in the activation tiles
     up_u32x4 w1[2], w2[5], w3[5];
 #pragma unroll
     for (int m = 0; m < 2; ++m) w1[m] = *reinterpret_cast<const up_u32x4*>(wf + H_W1 + l16 * H_K1 + 32 * m + 8 * lg);
@@ -261,90 +292,118 @@ __global__ __launch_bounds__(256) void k_ups_fused(const float* __restrict__ lo,
         b2[r] = prm[U_B2 + 4 * lg + r];
         b3[r] = 4 * lg + r < 2 ? prm[U_B3 + 4 * lg + r] : 0.0f;
     }
+    int off1[5], off2[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const int tap = 2 * m + (lg >> 1);
+        off1[m] = tap < 9 ? ((tap / 3) * UF_1W + tap % 3) * UF_AS + 8 * (lg & 1) : -1;
+        off2[m] = tap < 9 ? ((tap / 3) * UF_2W + tap % 3) * UF_AS + 8 * (lg & 1) : -1;
+    }
+    const int t0 = 2 * lg, t1 = 2 * lg + 1;
+    const int offx0 = (t0 / 3) * UF_XW + t0 % 3, offx1 = (t1 / 3) * UF_XW + t1 % 3, offx8 = 2 * UF_XW + 2;
+    __syncthreads();
 
     // input tile: (u, v) = bilinear x scale, luma; zero outside the frame
-    for (int e = tid; e < UF_XH * UF_XW; e += 256) {
-        const int iy = y0 - 3 + e / UF_XW, ix = x0 - 3 + e % UF_XW;
+    for (int e = tid; e < UF_XH * UF_XW; e += 64 * NW) {
+        const int r = e / UF_XW, c = e % UF_XW;
+        const int iy = y0 - 3 + r, ix = x0 - 3 + c;
         uint2 v = make_uint2(0u, 0u);
         if (iy >= 0 && iy < H && ix >= 0 && ix < W) {
-            const BiTap t = bilinear_tap(ix, iy, lw, lh, W, H);
-            const float u = bilinear_sample(fl, t, 0) * scale, w = bilinear_sample(fl, t, 1) * scale;
+            // bilinear_sample's operation order, both channels from 8-byte loads
+            const int a0 = tx0[c], a1 = tx1[c], r0 = ty0[r] * lw, r1 = ty1[r] * lw;
+            const float fx = tfx[c], fy = tfy[r];
+            float2 v00, v10, v01, v11;
+            if (lch == 2) {
+                const float2* f2 = reinterpret_cast<const float2*>(lo);
+                v00 = f2[r0 + a0]; v10 = f2[r0 + a1]; v01 = f2[r1 + a0]; v11 = f2[r1 + a1];
+            } else {
+                v00 = make_float2(lo[(r0 + a0) * lch], lo[(r0 + a0) * lch + 1]);
+                v10 = make_float2(lo[(r0 + a1) * lch], lo[(r0 + a1) * lch + 1]);
+                v01 = make_float2(lo[(r1 + a0) * lch], lo[(r1 + a0) * lch + 1]);
+                v11 = make_float2(lo[(r1 + a1) * lch], lo[(r1 + a1) * lch + 1]);
+            }
+            const float gx = 1.0f - fx, gy = 1.0f - fy;
+            const float u = ((((v00.x * gx) * gy + (v10.x * fx) * gy) + (v01.x * gx) * fy) + (v11.x * fx) * fy) * scale;
+            const float w = ((((v00.y * gx) * gy + (v10.y * fx) * gy) + (v01.y * gx) * fy) + (v11.y * fx) * fy) * scale;
             const uint8_t* px = rgba + ((long long)iy * W + ix) * 4;
             const float luma = (0.299f * (float)px[0] + 0.587f * (float)px[1] + 0.114f * (float)px[2]) * (1.0f / 255.0f);
             v = make_uint2(pack_h2(u, w), pack_h2(luma, 0.0f));
+            if (r >= 3 && r < 3 + UF_TH && c >= 3 && c < 3 + UF_TW) sbase[(r - 3) * UF_TW + c - 3] = make_float2(u, w);
         }
         sx[e] = v;
     }
     __syncthreads();
 
-    // layer 1 on the 20 x 36 tile
+    // layer 1 on the 20 x 36 tile, ReLU, zero outside the frame
     constexpr int N1 = UF_1H * UF_1W, G1 = (N1 + 15) / 16;
-    for (int g = q; g < G1; g += 4) {
-        const int p = min(16 * g + l16, N1 - 1), r = p / UF_1W, c = p % UF_1W;
-        up_f32x4 acc = up_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        {
-            const int t0 = 2 * lg, t1 = 2 * lg + 1;
-            const uint2 a = sx[(r + t0 / 3) * UF_XW + c + t0 % 3], b = sx[(r + t1 / 3) * UF_XW + c + t1 % 3];
-            acc = up_mma(w1[0], up_u32x4{a.x, a.y, b.x, b.y}, acc);
-        }
-        {
-            const uint2 a = lg == 0 ? sx[(r + 2) * UF_XW + c + 2] : make_uint2(0u, 0u);
-            acc = up_mma(w1[1], up_u32x4{a.x, a.y, 0u, 0u}, acc);
-        }
-        const int iy = y0 - 2 + r, ix = x0 - 2 + c;
+    auto l1 = [&](int p) {
+        const int bx = (p / UF_1W) * UF_XW + p % UF_1W;
+        const uint2 a = sx[bx + offx0], b = sx[bx + offx1];
+        const uint2 e = lg == 0 ? sx[bx + offx8] : make_uint2(0u, 0u);
+        up_f32x4 acc = up_mma(w1[0], up_u32x4{a.x, a.y, b.x, b.y}, up_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+        return up_mma(w1[1], up_u32x4{e.x, e.y, 0u, 0u}, acc);
+    };
+    auto put1 = [&](int g, const up_f32x4& acc) {
+        const int p = 16 * g + l16;
+        if (p >= N1) return;
+        const int r = p / UF_1W, c = p % UF_1W, iy = y0 - 2 + r, ix = x0 - 2 + c;
         const bool in = iy >= 0 && iy < H && ix >= 0 && ix < W;
         float o[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) o[k] = in ? fmaxf(acc[k] + b1[k], 0.0f) : 0.0f;
-        if (16 * g + l16 < N1)
-            *reinterpret_cast<uint2*>(&s1[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
+        *reinterpret_cast<uint2*>(&s1[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
+    };
+    for (int g = q; g < G1; g += 2 * NW) {
+        const int g2 = g + NW;
+        const up_f32x4 a0 = l1(min(16 * g + l16, N1 - 1));
+        const up_f32x4 a1 = l1(min(16 * g2 + l16, N1 - 1));
+        put1(g, a0);
+        if (g2 < G1) put1(g2, a1);
     }
     __syncthreads();
 
     // layer 2 on the 18 x 34 tile
     constexpr int N2 = UF_2H * UF_2W, G2 = (N2 + 15) / 16;
-    for (int g = q; g < G2; g += 4) {
-        const int p = min(16 * g + l16, N2 - 1), r = p / UF_2W, c = p % UF_2W;
-        up_f32x4 acc = up_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            const int tap = 2 * m + (lg >> 1);
-            up_u32x4 xb = up_u32x4{0u, 0u, 0u, 0u};
-            if (tap < 9) xb = *reinterpret_cast<const up_u32x4*>(&s1[((r + tap / 3) * UF_1W + c + tap % 3) * UF_AS + 8 * (lg & 1)]);
-            acc = up_mma(w2[m], xb, acc);
-        }
-        const int iy = y0 - 1 + r, ix = x0 - 1 + c;
+    auto put2 = [&](int g, const up_f32x4& acc) {
+        const int p = 16 * g + l16;
+        if (p >= N2) return;
+        const int r = p / UF_2W, c = p % UF_2W, iy = y0 - 1 + r, ix = x0 - 1 + c;
         const bool in = iy >= 0 && iy < H && ix >= 0 && ix < W;
         float o[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) o[k] = in ? fmaxf(acc[k] + b2[k], 0.0f) : 0.0f;
-        if (16 * g + l16 < N2)
-            *reinterpret_cast<uint2*>(&s2[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
+        *reinterpret_cast<uint2*>(&s2[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
+    };
+    auto base2 = [&](int p) { return ((p / UF_2W) * UF_1W + p % UF_2W) * UF_AS; };
+    for (int g = q; g < G2; g += 2 * NW) {
+        const int g2 = g + NW;
+        const up_f32x4 a0 = ups_l16(s1, w2, base2(min(16 * g + l16, N2 - 1)), off1);
+        const up_f32x4 a1 = ups_l16(s1, w2, base2(min(16 * g2 + l16, N2 - 1)), off1);
+        put2(g, a0);
+        if (g2 < G2) put2(g2, a1);
     }
     __syncthreads();
 
     // layer 3 on the 16 x 32 output tile, added to the FP32 bilinear flow
     const float gv = gate ? *gate : 1.0f;
     constexpr int G3 = UF_TH * UF_TW / 16;
-    for (int g = q; g < G3; g += 4) {
-        const int p = 16 * g + l16, r = p / UF_TW, c = p % UF_TW;
-        up_f32x4 acc = up_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            const int tap = 2 * m + (lg >> 1);
-            up_u32x4 xb = up_u32x4{0u, 0u, 0u, 0u};
-            if (tap < 9) xb = *reinterpret_cast<const up_u32x4*>(&s2[((r + tap / 3) * UF_2W + c + tap % 3) * UF_AS + 8 * (lg & 1)]);
-            acc = up_mma(w3[m], xb, acc);
-        }
-        const int iy = y0 + r, ix = x0 + c;
+    auto put3 = [&](int g, const up_f32x4& acc) {
+        const int p = 16 * g + l16, r = p / UF_TW, c = p % UF_TW, iy = y0 + r, ix = x0 + c;
         if (lg == 0 && iy < H && ix < W) {
-            const BiTap t = bilinear_tap(ix, iy, lw, lh, W, H);
-            const float u = bilinear_sample(fl, t, 0) * scale, w = bilinear_sample(fl, t, 1) * scale;
+            const float2 bs = sbase[p];
             float* o = out + ((long long)iy * W + ix) * out_ch;
-            o[0] = u + gv * (acc[0] + b3[0]);
-            o[1] = w + gv * (acc[1] + b3[1]);
+            o[0] = bs.x + gv * (acc[0] + b3[0]);
+            o[1] = bs.y + gv * (acc[1] + b3[1]);
             if (out_ch == 3) o[2] = 0.0f;
         }
+    };
+    auto base3 = [&](int p) { return ((p / UF_TW) * UF_2W + p % UF_TW) * UF_AS; };
+    for (int g = q; g < G3; g += 2 * NW) {
+        const int g2 = g + NW;
+        const up_f32x4 a0 = ups_l16(s2, w3, base3(16 * g + l16), off2);
+        const up_f32x4 a1 = ups_l16(s2, w3, base3(16 * min(g2, G3 - 1) + l16), off2);
+        put3(g, a0);
+        if (g2 < G3) put3(g2, a1);
     }
 }
 
@@ -642,7 +701,7 @@ extern "C" int vsc_upsampler_run(vsc_upsampler* u, const float* flow_lo, int lo_
         (out_ch != 2 && out_ch != 3))
         return set_error(VSC_E_INVALID, "vsc_upsampler_run: bad shape");
     const int tx = (width + UF_TW - 1) / UF_TW, ty = (height + UF_TH - 1) / UF_TH;
-    hipLaunchKernelGGL(k_ups_fused, dim3(tx * ty), dim3(256), 0, as_stream(stream), flow_lo, lo_w, lo_h, lo_ch, rgba,
+    hipLaunchKernelGGL(k_ups_fused, dim3(tx * ty), dim3(512), 0, as_stream(stream), flow_lo, lo_w, lo_h, lo_ch, rgba,
                        width, height, scale, u->w16, u->p, gate, flow_out, out_ch, tx);
     return check_launch("upsampler fused");
 }
