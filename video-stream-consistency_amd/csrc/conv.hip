@@ -20,6 +20,8 @@
 // 8- or 16-byte vector in NHWC.
 #include "vsc_internal.h"
 
+#include <algorithm>
+
 namespace vsc {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 cv_bf16x8;
@@ -42,6 +44,12 @@ struct ConvArgs {
     int Ho, Wo, Cout;
     int pad, pstep, act;
     int tiles_x, tiles_y;
+    // split over input channels: blockIdx.y = part * ctiles + channel tile; part p
+    // covers chunks [p*cpp, (p+1)*cpp) and writes raw FP32 sums to
+    // partial + p*part_stride (k_conv_finalize adds them in part order)
+    int ctiles, cpp;
+    float* partial;
+    long long part_stride;
 };
 
 template <int EK>
@@ -105,7 +113,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
 
     const int tid = threadIdx.x, q = tid / 64, lane = tid % 64;
     const int l16 = lane % 16, lg = lane / 16;
-    const int n = blockIdx.z, nt0 = blockIdx.y * NT;
+    const int n = blockIdx.z, kp = blockIdx.y / a.ctiles, nt0 = (blockIdx.y % a.ctiles) * NT;
+    const int cbeg = kp * a.cpp * KC, cend = min(a.Cin, (kp + 1) * a.cpp * KC);
     int t = blockIdx.x;
     const int tx = t % a.tiles_x; t /= a.tiles_x;
     const int ty = t % a.tiles_y; t /= a.tiles_y;
@@ -145,8 +154,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
         }
     };
 
-    fetch(0);
-    for (int c0 = 0; c0 < a.Cin; c0 += KC) {
+    fetch(cbeg);
+    for (int c0 = cbeg; c0 < cend; c0 += KC) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
@@ -159,7 +168,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
             if (u < NWU) *reinterpret_cast<cv_u32x4*>(&sW[(u / KV) * ROW + (u % KV) * VEC]) = rw[k];
         }
         __syncthreads();
-        if (c0 + KC < a.Cin) fetch(c0 + KC);
+        if (c0 + KC < cend) fetch(c0 + KC);
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
             const int ky = tap / 3, kx = tap % 3;
@@ -191,6 +200,11 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
             const int co = nt0 + i * 16 + 4 * lg;
             const int nv = min(4, a.Cout - co);
             if (nv <= 0) continue;
+            if (a.partial) {
+                float* d = a.partial + kp * a.part_stride + pix * a.Cout + co;
+                for (int r = 0; r < nv; ++r) d[r] = acc[i][m][r];
+                continue;
+            }
             float v[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -206,16 +220,58 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
     }
 }
 
+// sum of the split partials in part order, then the epilogue of k_conv3x3
+template <int EKO>
+__global__ void k_conv_finalize(ConvArgs a, int parts)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)a.N * a.Ho * a.Wo * a.Cout;
+    if (e >= total) return;
+    const int co = (int)(e % a.Cout);
+    const long long pix = e / a.Cout;
+    float x = 0.0f;
+    for (int k = 0; k < parts; ++k) x += a.partial[k * a.part_stride + e];
+    if (a.bias) x += a.bias[co];
+    if (a.act == 1) x = x > 0.0f ? x : x * 0.1f;
+    else if (a.act == 2) x = x > 0.0f ? x : 0.0f;
+    if (a.gate) x *= *a.gate;
+    using E = Elem<EKO>;
+    if (a.add) x += E::ld(static_cast<const typename E::T*>(a.add)[pix * a.add_cs + a.add_c0 + co]);
+    static_cast<typename E::T*>(a.out)[pix * a.out_cs + a.out_c0 + co] = E::st(x);
+}
+
 template <int EK, int EKO, int S, int NT, int MR>
 static void conv_go(const ConvArgs& a, hipStream_t s)
 {
+    constexpr int KC = ConvK<EK>::KC;
     const int ps = a.pstep;
     ConvArgs b = a;
     const int hs = (a.Ho + ps - 1) / ps, ws = (a.Wo + ps - 1) / ps;
     b.tiles_y = (hs + 4 * MR - 1) / (4 * MR);
     b.tiles_x = (ws + 15) / 16;
-    dim3 grid(b.tiles_x * b.tiles_y * ps * ps, (a.Cout + NT - 1) / NT, a.N);
+    b.ctiles = (a.Cout + NT - 1) / NT;
+    const int chunks = (a.Cin + KC - 1) / KC;
+    // split the channel loop when the grid leaves most CUs idle and the
+    // caller provided room for the partial sums.  The split depends on one
+    // batch item's geometry only (room reserved for two items), so an item's
+    // result has the same bits alone and in a batch of two.
+    const long long item_blocks = (long long)b.tiles_x * b.tiles_y * ps * ps * b.ctiles;
+    const long long item_stride = (long long)a.Ho * a.Wo * a.Cout;
+    const long long stride = item_stride * a.N;
+    int parts = 1;
+    if (a.partial && chunks > 1 && item_blocks < 256 && a.N <= 2) {
+        parts = (int)std::min<long long>(chunks, (256 + item_blocks - 1) / item_blocks);
+        parts = (int)std::min<long long>(parts, a.part_stride / std::max<long long>(2 * item_stride, 1));
+        if (parts < 2) parts = 1;
+    }
+    b.cpp = (chunks + parts - 1) / parts;
+    parts = (chunks + b.cpp - 1) / b.cpp;
+    b.partial = parts > 1 ? a.partial : nullptr;
+    b.part_stride = stride;
+    dim3 grid(b.tiles_x * b.tiles_y * ps * ps, b.ctiles * parts, a.N);
     hipLaunchKernelGGL((k_conv3x3<EK, EKO, S, NT, MR>), grid, dim3(256), 0, s, b);
+    if (parts > 1)
+        hipLaunchKernelGGL(k_conv_finalize<EKO>, dim3((unsigned)((stride + 255) / 256)), dim3(256), 0, s, b, parts);
 }
 
 // tile choice: the widest channel tile that the layer fills, and the tallest
@@ -278,6 +334,8 @@ int launch_conv3x3(const ConvDesc& d, hipStream_t s)
     a.out = d.out.p; a.out_cs = d.out.cs; a.out_c0 = d.out.c0;
     a.add = d.add.p; a.add_cs = d.add.cs; a.add_c0 = d.add.c0;
     a.gate = d.gate;
+    a.partial = d.partial;
+    a.part_stride = (long long)d.partial_cap;   // capacity in floats; conv_go turns it into the part stride
     a.Ho = Ho; a.Wo = Wo; a.Cout = Cout;
     a.pad = d.pad; a.pstep = d.dil; a.act = d.act;
     if (ek == EK_F32) conv_stride<EK_F32, EK_F32>(a, d.stride, s);

@@ -34,6 +34,7 @@ constexpr int kDecC = 456;                                  // dec channels: 448
 constexpr int kRefIn0 = 416, kRefInC = 40;
 constexpr int kRefOut[7] = {128, 128, 128, 96, 64, 32, 2};
 constexpr int kRefDil[7] = {1, 2, 4, 8, 16, 1, 1};
+constexpr size_t kPartialFloats = (size_t)4 << 20;         // 16 MB of FP32 partial sums
 
 inline int rup(int x, int m) { return (x + m - 1) / m * m; }
 
@@ -83,6 +84,7 @@ struct vsc_flownet {
     float* flow[7] = {};                  // per level [2][h][w][2]
     float* refined = nullptr;             // [2][h3][w3][2]
     void* rbuf[2] = {nullptr, nullptr};   // refiner ping-pong [2][h3][w3][128]
+    float* partial = nullptr;             // split-channel partial sums of the small layers
     std::vector<void*> owned;
     int last_pairs = 0;
 };
@@ -218,6 +220,7 @@ Sizes sizes_of(const vsc_flownet* n)
     z.weights = 0;
     for (const Layer& l : n->L)
         z.weights += (l.k == 4 ? l.w.size() * 4 : (size_t)l.cout * 9 * l.cin_pad * eb) + l.b.size() * 4 + 512;
+    z.weights += kPartialFloats * 4;
     return z;
 }
 
@@ -262,6 +265,7 @@ int alloc_all(vsc_flownet* n)
     n->refined = static_cast<float*>(dalloc(n, z.refined, rc));
     n->rbuf[0] = dalloc(n, z.rbuf, rc);
     n->rbuf[1] = dalloc(n, z.rbuf, rc);
+    n->partial = static_cast<float*>(dalloc(n, kPartialFloats * 4, rc));
     for (Layer& l : n->L) {
         l.dw = dalloc(n, l.k == 4 ? l.w.size() * 4 : (size_t)l.cout * 9 * l.cin_pad * n->eb, rc);
         l.db = static_cast<float*>(dalloc(n, l.b.size() * 4, rc));
@@ -281,6 +285,8 @@ int conv(vsc_flownet* n, const Layer& l, Nhwc in, Nhwc out, Nhwc add, int ek_out
     d.wt = l.dw; d.bias = l.db;
     d.stride = l.stride; d.dil = l.dil; d.pad = l.dil; d.act = l.act;
     d.ek_in = n->ek; d.ek_out = ek_out;
+    d.partial = n->partial;
+    d.partial_cap = kPartialFloats;
     return launch_conv3x3(d, s);
 }
 

@@ -283,6 +283,8 @@ struct ConvDesc {
     const float* gate;   // nullptr, or a device scalar multiplying the activated result before `add`
     int stride, dil, pad, act;
     int ek_in, ek_out;
+    float* partial;      // optional FP32 scratch: small layers split their channel loop into it
+    size_t partial_cap;  // scratch capacity in floats
 };
 int launch_conv3x3(const ConvDesc& d, hipStream_t s);
 
