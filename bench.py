@@ -50,6 +50,7 @@ def parse():
                          "serve: 24 concurrent streams of mixed resolutions at 30 fps (multi-stream server)")
     ap.add_argument("--serve-frames", type=int, default=60, help="frames per stream (serve workload)")
     ap.add_argument("--serve-fps", type=float, default=30.0, help="arrival rate per stream (serve workload)")
+    ap.add_argument("--serve-eager", action="store_true", help="serve workload: launch eagerly instead of replaying graphs")
     ap.add_argument("--plain-steps", type=int, default=0, help="timed frames of the plain form (default steps/3)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
@@ -392,7 +393,8 @@ def bench_serve(args):
                                      motion=8.0, npatches=4, patch_size=max(16, w // 15), seed=1000 + i)
         src.append((torch.from_numpy(o).to(dev), torch.from_numpy(p).to(dev)))
     gen_s = time.time() - t0
-    sv = vsc.Server(hbm_budget=128 << 30, max_streams=len(SERVE_STREAMS), hip_streams=4, form="chosen")
+    sv = vsc.Server(hbm_budget=128 << 30, max_streams=len(SERVE_STREAMS), hip_streams=4, form="chosen",
+                    use_graphs=not args.serve_eager)
     ids = [sv.open(w, h) for (w, h) in SERVE_STREAMS]
     sid_to_i = {sid: i for i, sid in enumerate(ids)}
     n = args.serve_frames
@@ -441,7 +443,8 @@ def bench_serve(args):
         "data": f"synthetic: {len(ids)} seeded streams, {uniq} distinct frames each cycled, resident in HBM",
         "config": {"workload": f"{len(ids)} streams (8x640x360, 8x1280x720, 6x1920x1080, 2x3840x2160), "
                                f"{n} frames each at {args.serve_fps} fps arrivals (zero-mean exponential jitter), "
-                               f"chosen form on the multi-stream server (4 HIP streams)"},
+                               f"chosen form on the multi-stream server (4 HIP streams, "
+                               f"{'eager launches' if args.serve_eager else 'per-stream hipGraph replay'})"},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(lat.max()), 3)},
         "latency_ms_by_size": {f"{w}x{h}": {"p50": round(1000.0 * float(np.percentile(v, 50)), 3),

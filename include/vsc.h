@@ -420,7 +420,7 @@ int vsc_pipeline_gate(const vsc_pipeline* p, float* gate, uint32_t* count, vsc_h
  * over hip_streams HIP streams (default priority) so their kernels overlap;
  * streams opened or frames submitted while earlier work runs join the next
  * tick.  vsc_server_poll reports finished frames in order per stream; a
- * reported frame's output stays in the stream's 4-slot output ring until the
+ * reported frame's output stays in the stream's 6-slot output ring until the
  * next tick (read it with vsc_server_read before ticking again).  A window is
  * launched only when its output slot has been reported, and frame t is taken
  * only when window t-2 has been launched; otherwise submit returns
@@ -433,6 +433,8 @@ typedef struct {
     size_t hbm_budget;        /* bytes of device memory the server may commit */
     int hip_streams;          /* HIP streams in the pool (1..16) */
     vsc_pipeline_cfg form;
+    int use_graphs;           /* 1: replay each stream's per-frame work from hipGraphs (6 per stream, one
+                                 per frame phase, captured once, re-captured after a parameter change) */
 } vsc_server_cfg;
 
 typedef struct {

@@ -117,3 +117,38 @@ def test_server_params_apply_from_next_tick(vsc, cuda):
         prm = hp if t >= 3 else None
         ref = pipe.frame((o[t - 1], o[t], o[t + 1], p[t - 1], p[t], p[t + 1]), t, params=prm).cpu()
         assert torch.equal(ref, got[t]), f"frame {t}"
+
+
+@pytest.mark.parametrize("form", ["chosen", "plain"])
+def test_server_graph_replay_matches_eager(vsc, cuda, form):
+    """Each stream's per-frame work replayed from captured hipGraphs (6 per
+    stream, one per frame phase) gives the eager launches' bits on every
+    frame: several replays of each graph, eager work and synchronisations in
+    between, and a parameter change that forces a re-capture."""
+    import torch
+    W, H = 160, 96
+    n = 22
+    orig, proc = _video(vsc, W, H, n, seed=17)
+    o = torch.from_numpy(orig).to(cuda)
+    p = torch.from_numpy(proc).to(cuda)
+    hp = vsc.HyperParams()
+    hp.gamma = 1.25
+    res = {}
+    for graphs in (False, True):
+        sv = vsc.Server(form=form, use_graphs=graphs)
+        sid = sv.open(W, H)
+        got = {}
+        for t in range(n):
+            if t == 13:
+                sv.set_params(sid, hp)
+            sv.submit(sid, o[t], p[t], tag=t)
+            sv.tick()
+            sv.drain()
+            for _, _, frame, _ in sv.poll():
+                got[frame] = sv.read(sid, frame).cpu()
+            torch.cuda.synchronize()
+        sv.close()
+        res[graphs] = got
+    assert sorted(res[True]) == list(range(1, n - 1))
+    for f in res[False]:
+        assert torch.equal(res[False][f], res[True][f]), f"frame {f}"

@@ -89,7 +89,7 @@ class Window(ctypes.Structure):
 
 class ServerCfg(ctypes.Structure):
     _fields_ = [("max_streams", ctypes.c_int), ("hbm_budget", ctypes.c_size_t), ("hip_streams", ctypes.c_int),
-                ("form", PipelineCfg)]
+                ("form", PipelineCfg), ("use_graphs", ctypes.c_int)]
 
 
 class ServerDone(ctypes.Structure):
@@ -823,10 +823,12 @@ class Server:
     """Multi-stream frame server (vsc_server_*): admission by device-memory
     budget, continuous batching of ready windows over a pool of HIP streams."""
 
-    def __init__(self, hbm_budget=64 << 30, max_streams=64, hip_streams=4, form="chosen", upsampler_params=None):
+    def __init__(self, hbm_budget=64 << 30, max_streams=64, hip_streams=4, form="chosen", upsampler_params=None,
+                 use_graphs=True):
         c = ServerCfg()
         lib().vsc_server_default_cfg(ctypes.byref(c))
         c.max_streams, c.hbm_budget, c.hip_streams = max_streams, hbm_budget, hip_streams
+        c.use_graphs = int(bool(use_graphs))
         fk = Pipeline.FORMS[form]
         c.form.flow_dtype = {"f32": VSC_F32, "bf16": VSC_BF16}[fk["flow_dtype"]]
         c.form.downscale = fk["downscale"]

@@ -261,3 +261,24 @@ extern "C" int vsc_pipeline_gate(const vsc_pipeline* p, float* gate_dev, uint32_
         return VSC_E_HIP;
     return VSC_OK;
 }
+
+namespace vsc {
+
+// The host-side state changes of vsc_pipeline_frame(frame) in steady state
+// (previous frame was frame - 1, features of frame - 1 and frame cached),
+// without launching anything: used when the frame's work is replayed from a
+// captured graph.  Returns VSC_E_INVALID when the pipeline is not in that state.
+int pipeline_advance(vsc_pipeline* p, long long frame)
+{
+    const int sp = (int)(((frame - 1) % 3 + 3) % 3), sc = (int)((frame % 3 + 3) % 3), sn = (int)(((frame + 1) % 3 + 3) % 3);
+    if (p->prev_fwd_frame != frame - 1 || p->slot_frame[sp] != frame - 1 || p->slot_frame[sc] != frame)
+        return set_error(VSC_E_INVALID, "pipeline_advance: not in steady state");
+    float* t = p->lo[2];
+    p->lo[2] = p->lo[0];
+    p->lo[0] = t;
+    p->slot_frame[sn] = frame + 1;
+    p->prev_fwd_frame = frame;
+    return VSC_OK;
+}
+
+}  // namespace vsc

@@ -39,7 +39,8 @@ struct Stream {
     vsc_pipeline* pipe = nullptr;
     uint8_t* orig[3] = {nullptr, nullptr, nullptr};
     uint8_t* proc[3] = {nullptr, nullptr, nullptr};
-    uint8_t* out[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint8_t* out[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t ge[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipStream_t hs = nullptr;       // borrowed from the pool
     long long next_frame = 0;       // index of the next submitted frame
     long long ready = -1;           // highest window centre ready to run
@@ -65,11 +66,26 @@ struct vsc_server {
 namespace vsc {
 namespace {
 
-constexpr int R_OUT = 4;
+// 6 output slots: every per-frame pointer then repeats with period 6 in the
+// frame index (input ring t % 3, ping-pongs t % 2), so 6 graphs replay forever
+constexpr int R_OUT = 6;
+constexpr int N_PHASES = 6;
 size_t ring_bytes(int W, int H) { return (size_t)W * H * 4 * (6 + R_OUT); }
+
+void drop_graphs(Stream& st)
+{
+    bool any = false;
+    for (int i = 0; i < N_PHASES; ++i) any = any || st.ge[i];
+    if (any && st.hs) (void)hipStreamSynchronize(st.hs);
+    for (int i = 0; i < N_PHASES; ++i) {
+        if (st.ge[i]) (void)hipGraphExecDestroy(st.ge[i]);
+        st.ge[i] = nullptr;
+    }
+}
 
 void release(Stream& st)
 {
+    drop_graphs(st);
     for (Done& d : st.inflight) (void)hipEventDestroy(d.ev);
     st.inflight.clear();
     if (st.pipe) vsc_pipeline_destroy(st.pipe);
@@ -96,6 +112,7 @@ extern "C" void vsc_server_default_cfg(vsc_server_cfg* c)
     c->form.flow_dtype = VSC_BF16;
     c->form.downscale = 2;
     c->form.upsample = 1;
+    c->use_graphs = 1;
 }
 
 extern "C" int vsc_server_create(const vsc_server_cfg* cfg, const float* upsampler_params, vsc_server** out)
@@ -208,6 +225,7 @@ extern "C" int vsc_server_set_params(vsc_server* sv, int id, const vsc_params* p
     Stream* st = get(sv, id);
     if (!st || !p) return set_error(VSC_E_INVALID, "vsc_server_set_params: no such stream or NULL params");
     st->params = *p;   // read by the next tick (hyperParams are copied per frame, videostabilizer.cpp:192)
+    drop_graphs(*st);  // captured graphs hold the old values as kernel arguments
     return VSC_OK;
 }
 
@@ -248,9 +266,32 @@ extern "C" int vsc_server_tick(vsc_server* sv, int* launched)
             const long long c = st.done_to + 1;
             vsc_window w{st.orig[(c - 1) % 3], st.orig[c % 3], st.orig[(c + 1) % 3],
                          st.proc[(c - 1) % 3], st.proc[c % 3], st.proc[(c + 1) % 3]};
-            int rc = vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
+            int rc;
+            const int ph = (int)(c % N_PHASES);
+            if (sv->cfg.use_graphs && c >= 3 && st.ge[ph]) {
+                // steady state: replay the phase's graph, step the host state
+                if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
+                if ((rc = pipeline_advance(st.pipe, c))) return rc;
+            } else if (sv->cfg.use_graphs && c >= 3) {
+                hipGraph_t g = nullptr;
+                if (check_hip(hipStreamBeginCapture(st.hs, hipStreamCaptureModeThreadLocal), "vsc_server_tick capture"))
+                    return VSC_E_HIP;
+                rc = vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
                                         reinterpret_cast<vsc_hip_stream>(st.hs));
-            if (rc) return rc;
+                const int rce = check_hip(hipStreamEndCapture(st.hs, &g), "vsc_server_tick capture end");
+                if (rc || rce) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc ? rc : rce;
+                }
+                rc = check_hip(hipGraphInstantiate(&st.ge[ph], g, nullptr, nullptr, 0), "vsc_server_tick instantiate");
+                (void)hipGraphDestroy(g);
+                if (rc) return rc;
+                if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
+            } else {
+                rc = vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
+                                        reinterpret_cast<vsc_hip_stream>(st.hs));
+                if (rc) return rc;
+            }
             Done d{st.tags.front(), c, (int)(c % R_OUT), nullptr};
             st.tags.pop_front();
             if (check_hip(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming), "vsc_server_tick") ||

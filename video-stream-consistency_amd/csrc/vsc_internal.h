@@ -288,6 +288,9 @@ struct ConvDesc {
 };
 int launch_conv3x3(const ConvDesc& d, hipStream_t s);
 
+// host-state step of a pipeline frame replayed from a graph (pipeline.hip)
+int pipeline_advance(vsc_pipeline* p, long long frame);
+
 // reduced-precision (BF16 / F16 / FP8 e4m3) correlation on MFMA, corr_mfma.hip
 int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
                        int in_dtype, const float* chan_scale, hipStream_t s);
