@@ -259,7 +259,7 @@ static void conv_go(const ConvArgs& a, hipStream_t s)
     const long long item_stride = (long long)a.Ho * a.Wo * a.Cout;
     const long long stride = item_stride * a.N;
     int parts = 1;
-    if (a.partial && chunks > 1 && item_blocks < 256 && a.N <= 2) {
+    if (a.partial && chunks > 1 && item_blocks < 128 && a.N <= 2) {
         parts = (int)std::min<long long>(chunks, (256 + item_blocks - 1) / item_blocks);
         parts = (int)std::min<long long>(parts, a.part_stride / std::max<long long>(2 * item_stride, 1));
         if (parts < 2) parts = 1;
