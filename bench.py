@@ -33,6 +33,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FLOW_HEAD_SCALE = 0.05
+# consistency solver arithmetic per 1080p frame: 14 operations per pixel, channel
+# and sweep; 150 sweeps at 1920x1080 and 75 at 960x540 (default parameters)
+SOLVER_FLOPS_PER_FRAME = 14 * 3 * (1920 * 1080 * 150 + 960 * 540 * 75)
 RING = 24
 W, H = 1920, 1080
 
@@ -71,10 +74,10 @@ def calibrate_copy(mib):
     return 0
 
 
-def traffic_from_profile(kernel):
+def traffic_from_profile(kernel, workload="pipeline"):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
     (copy-calibrated) for this workload, from the committed profile summary."""
-    path = os.path.join(ROOT, "profiles", "stabilize_1080p_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "pipeline_1080p_hbm_traffic.md" if workload == "pipeline" else "stabilize_1080p_hbm_traffic.md")
     try:
         for line in open(path):
             if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
@@ -356,11 +359,19 @@ def bench_pipeline(args):
             "launches": n_solver, "avg_launch_us": round(1000.0 * ms_solver / n_solver, 3),
             "algorithmic_bytes_per_launch": round(by_solver / n_solver),
             "share_of_frame": round(ms_solver / float(lat.sum()), 4),
+            # after temporal blocking the solver issues ~14 FP32 operations per pixel,
+            # channel and sweep in the reference's order (no contraction): its
+            # arithmetic rate against the FP32 vector peak, beside the HBM figure
+            "valu": {"algorithmic_tflops": round(SOLVER_FLOPS_PER_FRAME * nprof / (ms_solver * 1e-3) / 1e12, 2),
+                     "peak_tflops": 157.3,
+                     "frac": round(SOLVER_FLOPS_PER_FRAME * nprof / (ms_solver * 1e-3) / 157.3e12, 4)},
         }
     if n_front:
         ach = by_front / (ms_front * 1e-3) / 1e9
         result["front_pass"] = {"achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
-                                "avg_launch_us": round(1000.0 * ms_front / n_front, 3)}
+                                "avg_launch_us": round(1000.0 * ms_front / n_front, 3),
+                                "hbm_traffic_gbs": round((traffic_from_profile("k_front") or 0) /
+                                                         (ms_front / n_front * 1e-3) / 1e9, 1)}
     result["gen_seconds"] = round(gen_s, 2)
     if not args.no_cpu:
         result["cpu_baseline"], result["quality"] = cpu_baseline_and_quality(args, vsc, orig, proc, fwd, bwd, dev)
@@ -548,7 +559,7 @@ def main():
         result["roofline"] = {
             "kernel": "k_consist_blocked (temporally blocked consistency solver)",
             "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile("k_consist_blocked"),
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile("k_consist_blocked", "stabilize"),
             "launches": n_solver, "avg_launch_us": round(1000.0 * ms_solver / n_solver, 3),
             "algorithmic_bytes_per_launch": round(by_solver / n_solver),
             "share_of_frame": round(ms_solver / float(lat.sum()), 4),
