@@ -435,15 +435,27 @@ int launch_consist_blocked(const float* lapP, const float* A, const float* Wt, c
                            float mom, hipStream_t s)
 {
     if (sweeps < 1 || sweeps > SB_HALO) return set_error(VSC_E_INVALID, "consist_blocked: sweeps out of range");
-    constexpr int TH = SB_STRIPS * SB_R;
-    constexpr int OW = SB_TW - 2 * SB_HALO, OH = TH - 2 * SB_HALO;
-    const int tiles_x = (W + OW - 1) / OW, tiles_y = (H + OH - 1) / OH;
+    constexpr int OW = SB_TW - 2 * SB_HALO;
+    const int tiles_x = (W + OW - 1) / OW;
+    // full-height tiles (SB_STRIPS strips) unless they leave CUs idle: then
+    // half-height tiles, whose 256-thread blocks run two per CU
+    int strips = SB_STRIPS;
+    if (SB_STRIPS > 2) {
+        const int oh = SB_STRIPS * SB_R - 2 * SB_HALO;
+        if (tiles_x * ((H + oh - 1) / oh) * 3 < 256) strips = 2;
+    }
+    const int OH = strips * SB_R - 2 * SB_HALO;
+    const int tiles_y = (H + OH - 1) / OH;
     const int per_xcd = (tiles_x * tiles_y * 3 + 7) / 8;
     // algorithmic bytes: read O, U, lapP, A, w and write O, U once per pixel and channel
     const int slot = prof_start(PROF_CONSIST_BLOCKED, 7.0 * 3 * 4 * (double)W * H, s);
-    hipLaunchKernelGGL((k_consist_blocked<SB_R, SB_HALO, SB_STRIPS>), dim3(8 * per_xcd), dim3(SB_TW * SB_STRIPS), 0, s,
-                       lapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, tiles_x, tiles_y,
-                       per_xcd);
+    if (strips == SB_STRIPS)
+        hipLaunchKernelGGL((k_consist_blocked<SB_R, SB_HALO, SB_STRIPS>), dim3(8 * per_xcd), dim3(SB_TW * SB_STRIPS), 0,
+                           s, lapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, tiles_x, tiles_y,
+                           per_xcd);
+    else
+        hipLaunchKernelGGL((k_consist_blocked<SB_R, SB_HALO, 2>), dim3(8 * per_xcd), dim3(SB_TW * 2), 0, s, lapP, A, Wt,
+                           Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, tiles_x, tiles_y, per_xcd);
     prof_stop(slot, s);
     return check_launch("consist_blocked");
 }
