@@ -461,9 +461,18 @@ int vsc_server_poll(vsc_server* sv, vsc_server_done* done, int max, int* count);
 /* wait for all launched work */
 int vsc_server_drain(vsc_server* sv);
 /* Copy stabilized frame `frame` (still in the output ring) of a stream into
- * dst (device RGBA8), ordered after the stream's work and before later work on
- * `stream`. */
+ * dst (device or pinned host RGBA8), ordered after the stream's work and
+ * before later work on `stream`. */
 int vsc_server_read(vsc_server* sv, int stream_id, long long frame, uint8_t* dst, vsc_hip_stream stream);
+
+/* ---------------------------------------------------------------------------
+ * Flow files (ReadFlowFile, flowIO.cpp:31-78): the Middlebury .flo format,
+ * host memory.  vsc_flo_read with flow == NULL returns the size only; errors
+ * (VSC_E_INVALID, message in vsc_last_error) for a wrong tag, a width or height
+ * outside [1, 99999], a short file or trailing bytes, as the reference rejects
+ * them.  Flows are HWC (u, v) float32, little-endian.                        */
+int vsc_flo_read(const char* path, float* flow, size_t capacity_floats, int* width, int* height);
+int vsc_flo_write(const char* path, const float* flow, int width, int height);
 
 /* ---------------------------------------------------------------------------
  * Per-kernel timing (replaces the per-stage QElapsedTimer accounting of

@@ -152,3 +152,31 @@ def test_server_graph_replay_matches_eager(vsc, cuda, form):
     assert sorted(res[True]) == list(range(1, n - 1))
     for f in res[False]:
         assert torch.equal(res[False][f], res[True][f]), f"frame {f}"
+
+
+def test_server_read_into_pinned_host(vsc, cuda):
+    """Egress straight into pinned host memory gives the device read's bytes;
+    an unpinned host destination is refused before any copy."""
+    import torch
+    W, H = SIZES[1]
+    orig, proc = _video(vsc, W, H, 5, seed=21)
+    o = torch.from_numpy(orig).to(cuda)
+    p = torch.from_numpy(proc).to(cuda)
+    sv = vsc.Server(form="chosen")
+    sid = sv.open(W, H)
+    n = 0
+    for t in range(5):
+        sv.submit(sid, o[t], p[t], tag=t)
+        sv.tick()
+        sv.drain()
+        for _, _, frame, _ in sv.poll():
+            dev = sv.read(sid, frame)
+            host = torch.empty((H, W, 4), dtype=torch.uint8, pin_memory=True)
+            sv.read(sid, frame, out=host)
+            torch.cuda.synchronize()
+            assert torch.equal(dev.cpu(), host), f"frame {frame}"
+            n += 1
+    with pytest.raises(vsc.VscError):
+        sv.read(sid, 3, out=torch.empty((H, W, 4), dtype=torch.uint8))
+    sv.close()
+    assert n == 3

@@ -55,7 +55,7 @@ EXPORTED = (
     "vsc_pipeline_stream", "vsc_pipeline_flows", "vsc_pipeline_frame", "vsc_pipeline_gate",
     "vsc_server_default_cfg", "vsc_server_create", "vsc_server_destroy", "vsc_server_stream_footprint",
     "vsc_server_used_bytes", "vsc_server_open", "vsc_server_close", "vsc_server_set_params", "vsc_server_submit",
-    "vsc_server_tick", "vsc_server_poll", "vsc_server_drain", "vsc_server_read",
+    "vsc_server_tick", "vsc_server_poll", "vsc_server_drain", "vsc_server_read", "vsc_flo_read", "vsc_flo_write",
 )
 
 
@@ -212,6 +212,8 @@ def lib():
         L.vsc_server_poll.argtypes = [vp, ctypes.POINTER(ServerDone), i, ctypes.POINTER(i)]
         L.vsc_server_drain.argtypes = [vp]
         L.vsc_server_read.argtypes = [vp, i, ctypes.c_longlong, vp, vp]
+        L.vsc_flo_read.argtypes = [ctypes.c_char_p, vp, ctypes.c_size_t, ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.vsc_flo_write.argtypes = [ctypes.c_char_p, vp, i, i]
         L.vsc_profile_begin.argtypes = [i]
         L.vsc_profile_end.argtypes = [ctypes.c_char_p, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -878,11 +880,17 @@ class Server:
         check(lib().vsc_server_poll(self.h, arr, max_items, ctypes.byref(n)))
         return [(arr[k].stream, arr[k].tag, arr[k].frame, arr[k].out_rgba) for k in range(n.value)]
 
-    def read(self, sid, frame):
-        """Stabilized frame `frame` of stream sid as a new device tensor (stream-ordered)."""
+    def read(self, sid, frame, out=None):
+        """Stabilized frame `frame` of stream sid (stream-ordered on torch's current
+        stream) into `out` (a contiguous uint8 [H, W, 4] device or pinned host
+        tensor) or a new device tensor."""
         torch = _torch()
         w, h = self.sizes[sid]
-        t = torch.empty((h, w, 4), dtype=torch.uint8, device="cuda")
+        t = torch.empty((h, w, 4), dtype=torch.uint8, device="cuda") if out is None else out
+        if t.shape != (h, w, 4) or t.dtype != torch.uint8 or not t.is_contiguous():
+            raise VscError(-1, "out must be a contiguous uint8 [H, W, 4] tensor")
+        if not t.is_cuda and not t.is_pinned():
+            raise VscError(-1, "a host destination must be pinned memory")
         check(lib().vsc_server_read(self.h, sid, int(frame), _ptr(t), _stream()))
         return t
 
@@ -899,6 +907,22 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def read_flo(path):
+    """ReadFlowFile (flowIO.cpp:31-78): a .flo file as a float32 [H, W, 2] array."""
+    w, h = ctypes.c_int(), ctypes.c_int()
+    check(lib().vsc_flo_read(str(path).encode(), None, 0, ctypes.byref(w), ctypes.byref(h)))
+    a = np.empty((h.value, w.value, 2), dtype=np.float32)
+    check(lib().vsc_flo_read(str(path).encode(), a.ctypes.data, a.size, ctypes.byref(w), ctypes.byref(h)))
+    return a
+
+
+def write_flo(path, flow):
+    a = np.ascontiguousarray(flow, dtype=np.float32)
+    if a.ndim != 3 or a.shape[2] != 2:
+        raise VscError(-1, "flow must be [H, W, 2]")
+    check(lib().vsc_flo_write(str(path).encode(), a.ctypes.data, a.shape[1], a.shape[0]))
 
 
 def profile_begin(capacity=4096):

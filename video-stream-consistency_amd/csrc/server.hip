@@ -350,7 +350,7 @@ extern "C" int vsc_server_read(vsc_server* sv, int id, long long frame, uint8_t*
         return set_error(VSC_E_INVALID, "vsc_server_read: frame not reported or no longer in the output ring");
     const size_t fb = (size_t)st->W * st->H * 4;
     hipEvent_t ev = nullptr;
-    if (check_hip(hipMemcpyAsync(dst, st->out[frame % R_OUT], fb, hipMemcpyDeviceToDevice, st->hs), "vsc_server_read") ||
+    if (check_hip(hipMemcpyAsync(dst, st->out[frame % R_OUT], fb, hipMemcpyDefault, st->hs), "vsc_server_read") ||
         check_hip(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "vsc_server_read") ||
         check_hip(hipEventRecord(ev, st->hs), "vsc_server_read") ||
         check_hip(hipStreamWaitEvent(as_stream(stream), ev, 0), "vsc_server_read")) {
