@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--serve-fps", type=float, default=30.0, help="arrival rate per stream (serve workload)")
     ap.add_argument("--serve-eager", action="store_true", help="serve workload: launch eagerly instead of replaying graphs")
     ap.add_argument("--plain-steps", type=int, default=0, help="timed frames of the plain form (default steps/3)")
+    ap.add_argument("--solver", choices=["fma", "exact"], default="fma",
+                    help="stabilize workload: solver in FMA form or in the reference op order")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -206,8 +208,8 @@ def cpu_baseline_and_quality(args, vsc, orig, proc, fwd, bwd, dev):
     O = [torch.from_numpy(o).to(dev) for o in orig[:3]]
     P = [torch.from_numpy(p).to(dev) for p in proc[:3]]
 
-    def gpu_frame(f_fwd, f_bwd):
-        st = vsc.VideoStabilizer(W, H)
+    def gpu_frame(f_fwd, f_bwd, mode=vsc.VSC_FRAME_FMA):
+        st = vsc.VideoStabilizer(W, H, mode=mode)
         st.set_last_rgba(P[2])
         of = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
         st.do_one_step(O[0:3], P[0:3], f_fwd, f_bwd, out_float=of)
@@ -216,6 +218,7 @@ def cpu_baseline_and_quality(args, vsc, orig, proc, fwd, bwd, dev):
         return r
 
     gpu_full = gpu_frame(torch.from_numpy(fwd[1]).to(dev), torch.from_numpy(bwd[1]).to(dev))
+    gpu_exact = gpu_frame(torch.from_numpy(fwd[1]).to(dev), torch.from_numpy(bwd[1]).to(dev), vsc.VSC_FRAME_FUSED)
     # chosen form's upsampling on ground truth: half-resolution flows (2x2 box / 2) upsampled x2
     ups = vsc.Upsampler(params=vsc.load_upsampler_weights())
     g0 = torch.zeros(1, device=dev)
@@ -245,11 +248,17 @@ def cpu_baseline_and_quality(args, vsc, orig, proc, fwd, bwd, dev):
     f64 = orc.stab_frame(*(x.astype(np.float64) for x in (Of[0], Of[1], Of[2], Pf[0], Pf[1], Pf[2])),
                          fwd[1].astype(np.float64), bwd[1].astype(np.float64), Pf[2].astype(np.float64),
                          dtype=np.float64)
+    def max_lsb(a):
+        return int(np.max(np.abs(orc.float_to_rgba(a)[..., :3].astype(int) -
+                                 orc.float_to_rgba(cpu1)[..., :3].astype(int))))
+
     q = {
-        "reference": "FP64 oracle stabilization of frame 1 with full-resolution ground-truth flows",
+        "reference": "FP64 oracle stabilization of frame 1 with full-resolution ground-truth flows; LSB "
+                     "against the oracle's FP32 stabilization in the reference op order",
         "plain_fp32_psnr_db": round(psnr(gpu_full, f64), 2),
-        "plain_u8_max_lsb_vs_fp32_oracle": int(np.max(np.abs(orc.float_to_rgba(gpu_full)[..., :3].astype(int) -
-                                                            orc.float_to_rgba(cpu1)[..., :3].astype(int)))),
+        "plain_u8_max_lsb_vs_fp32_oracle": max_lsb(gpu_full),
+        "exact_solver_psnr_db": round(psnr(gpu_exact, f64), 2),
+        "exact_solver_u8_max_lsb_vs_fp32_oracle": max_lsb(gpu_exact),
         "chosen_learned_upsampler_psnr_db": round(psnr(gpu_learned, f64), 2),
         "bilinear_upsampler_psnr_db": round(psnr(gpu_bilinear, f64), 2),
         "upsampler_epe_px": {"learned": round(epe_l, 4), "bilinear": round(epe_b, 4)},
@@ -319,6 +328,12 @@ def bench_pipeline(args):
     el_p, _ = run(plain, plain_steps, max(3, args.warmup // 4), 1)
     plain.close()
     fps_plain = plain_steps / el_p
+    # the plain form with the solver in the reference op order (bit-exact with the oracle)
+    plain_x = vsc.Pipeline(W, H, form="plain", seed=1, flow_head_scale=FLOW_HEAD_SCALE, solver="exact")
+    plain_x.set_last_rgba(P[2])
+    el_x, _ = run(plain_x, plain_steps, max(3, args.warmup // 4), 1)
+    plain_x.close()
+    fps_exact = plain_steps / el_x
 
     result = {
         "metric": "1080p stabilized frames/s, chosen vs plain FP32 form at stated PSNR; p99 latency",
@@ -340,12 +355,17 @@ def bench_pipeline(args):
                                "(new frame's features + batched fwd/bwd decode), flow upsampling, fused "
                                "stabilization (2-level pyramid, 75 + 150 blocked sweeps), RGBA8 out",
                    "form": "chosen: BF16 PWCNet-light at FLOWDOWNSCALE=2, learned FP16 upsampler with the "
-                           "fwd-bwd fallback, FP32 stabilization",
+                           "fwd-bwd fallback, FP32 stabilization (FMA-form solver)",
                    "width": W, "height": H, "pyramid_levels": 2, "num_iter": 150},
         "plain_form": {"value": round(fps_plain, 2), "unit": "frames/s", "steps": plain_steps,
                        "ms_per_step": round(1000.0 * el_p / plain_steps, 4),
-                       "form": "FP32 PWCNet-light at full resolution, FP32 stabilization"},
+                       "form": "FP32 PWCNet-light at full resolution, FP32 stabilization (FMA-form solver)"},
         "speedup_vs_plain": round(fps / fps_plain, 3),
+        "plain_form_exact_solver": {"value": round(fps_exact, 2), "unit": "frames/s", "steps": plain_steps,
+                                    "ms_per_step": round(1000.0 * el_x / plain_steps, 4),
+                                    "form": "the plain form with the solver in the reference op order "
+                                            "(bit-exact with the oracle)",
+                                    "speedup_of_chosen": round(fps / fps_exact, 3)},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                        "p99": round(float(np.percentile(lat, 99)), 4), "frames": int(nprof)},
         "fallback_gate_last_frame": {"gate": gate, "inconsistent_px": count},
@@ -359,9 +379,10 @@ def bench_pipeline(args):
             "launches": n_solver, "avg_launch_us": round(1000.0 * ms_solver / n_solver, 3),
             "algorithmic_bytes_per_launch": round(by_solver / n_solver),
             "share_of_frame": round(ms_solver / float(lat.sum()), 4),
-            # after temporal blocking the solver issues ~14 FP32 operations per pixel,
-            # channel and sweep in the reference's order (no contraction): its
-            # arithmetic rate against the FP32 vector peak, beside the HBM figure
+            # the solver's algorithmic arithmetic (the reference formula's 14 FP32
+            # operations per pixel, channel and sweep; the FMA form issues 7
+            # instructions for them): its rate against the FP32 vector peak,
+            # beside the HBM figure
             "valu": {"algorithmic_tflops": round(SOLVER_FLOPS_PER_FRAME * nprof / (ms_solver * 1e-3) / 1e12, 2),
                      "peak_tflops": 157.3,
                      "frac": round(SOLVER_FLOPS_PER_FRAME * nprof / (ms_solver * 1e-3) / 157.3e12, 4)},
@@ -502,7 +523,8 @@ def main():
     B = [torch.from_numpy(b).to(dev) for b in bwd]
     out = torch.empty((H, W, 4), dtype=torch.uint8, device=dev)
 
-    st = vsc.VideoStabilizer(W, H)
+    mode = vsc.VSC_FRAME_FMA if args.solver == "fma" else vsc.VSC_FRAME_FUSED
+    st = vsc.VideoStabilizer(W, H, mode=mode)
     st.set_last_rgba(P[2])
 
     def step(i):
@@ -550,7 +572,8 @@ def main():
                 f"frames resident in HBM",
         "config": {"workload": "stabilize 1920x1080 frames, flows given (plain FP32 form): fused front pass, "
                                "2-level pyramid, 75 + 150 temporally blocked Jacobi sweeps, RGBA8 out",
-                   "form": "fp32", "width": W, "height": H, "pyramid_levels": 2, "num_iter": 150},
+                   "form": "fp32", "solver": args.solver, "width": W, "height": H, "pyramid_levels": 2,
+                   "num_iter": 150},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                        "p99": round(float(np.percentile(lat, 99)), 4), "frames": int(nprof)},
     }
@@ -577,7 +600,7 @@ def main():
         Of = [orc.rgba_to_float(o) for o in orig[:6]]
         Pf = [orc.rgba_to_float(p) for p in proc[:6]]
         # GPU output of frame 1 with last = processed frame 2 (fresh stream)
-        st2 = vsc.VideoStabilizer(W, H)
+        st2 = vsc.VideoStabilizer(W, H, mode=mode)
         st2.set_last_rgba(P[2])
         of = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
         st2.do_one_step(O[0:3], P[0:3], F[1], B[1], out_float=of)

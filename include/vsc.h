@@ -226,6 +226,11 @@ typedef struct {
 
 #define VSC_FRAME_FUSED 0     /* fused front pass + temporally blocked solver */
 #define VSC_FRAME_STAGED 1    /* one launch per reference stage (parity aid) */
+#define VSC_FRAME_FMA 2       /* as FUSED, solver update in FMA form: per-pixel
+                                 constants q = cnt + w, k = step*(lapP - w*A) and
+                                 7 operations per pixel and sweep instead of 13;
+                                 differs from the reference op order by rounding
+                                 (<= 1 LSB in the RGBA8 output) */
 
 /* Stabilization body of VideoStabilizer::doOneStep (videostabilizer.cpp:177-238):
  * 5 warps, adaptive combination (alpha), consistency weights (beta, gamma),
@@ -376,6 +381,8 @@ typedef struct {
     float fb_max_inconsistent;  /* learned path: bilinear fallback when the inconsistent fraction exceeds this (< 0: never) */
     int ref_bwd_index;          /* 1: backward flow = net(next -> cur) as videostabilizer.cpp:271-272; 0: cur -> prev */
     unsigned long long seed;    /* flow network (and untrained upsampler) initialisation */
+    int solver;                 /* stabilization solver: 0 reference op order (VSC_FRAME_FUSED),
+                                   1 FMA form (VSC_FRAME_FMA, <= 1 LSB from it) */
 } vsc_pipeline_cfg;
 
 /* The k = 1 frame window (device RGBA8 frames, 4-byte aligned). */

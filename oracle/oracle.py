@@ -72,6 +72,10 @@ def lib():
             g = getattr(L, "vsco_stab_frame_" + name)
             g.restype = None
             g.argtypes = [p] * 8 + [_i, p, _f, _f, _f, _i, _i, _f, _f, p, _i, _i]
+        L.vsco_stab_consist_out_fma_f32.restype = None
+        L.vsco_stab_consist_out_fma_f32.argtypes = [_f32p, _f32p, _f32p, _i, _f, _f, _f32p, _i, _i]
+        L.vsco_stab_frame_fma_f32.restype = None
+        L.vsco_stab_frame_fma_f32.argtypes = [_f32p] * 8 + [_i, _f32p, _f, _f, _f, _i, _i, _f, _f, _f32p, _i, _i]
         L.vsco_average_frames_f32.restype = None
         L.vsco_average_frames_f32.argtypes = [_f32p, _f32p, _f32p, _i, _i, _i]
         L.vsco_rgba_to_float.restype = None
@@ -165,11 +169,22 @@ def stab_consist_wt(adIn, cIn, beta, gamma, dtype=np.float32):
     return wt
 
 
-def stab_consist_out(P, A, Wt, O_init, iters, step=0.15, mom=0.15, dtype=np.float32):
+def _solver_fn(base, solver, dtype):
+    if solver == "fma":
+        if dtype != np.float32:
+            raise ValueError("the FMA-form solver is restated in float32 only")
+        return getattr(lib(), base + "fma_f32")
+    if solver != "exact":
+        raise ValueError(f"unknown solver {solver!r}")
+    return getattr(lib(), base + _sfx(dtype))
+
+
+def stab_consist_out(P, A, Wt, O_init, iters, step=0.15, mom=0.15, dtype=np.float32, solver="exact"):
+    """solver "exact": the reference op order; "fma": the FMA form."""
     P, A, Wt = (_c(a, dtype) for a in (P, A, Wt))
     O = _c(O_init, dtype).copy()
     H, W, _ = P.shape
-    getattr(lib(), "vsco_stab_consist_out_" + _sfx(dtype))(P, A, Wt, iters, step, mom, O, W, H)
+    _solver_fn("vsco_stab_consist_out_", solver, dtype)(P, A, Wt, iters, step, mom, O, W, H)
     return O
 
 
@@ -215,8 +230,9 @@ DEFAULT_PARAMS = dict(alpha=6800.0, beta=6800.0, gamma=2.0, pyramid_levels=2, nu
 
 
 def stab_frame(o_prev, o_cur, o_next, p_prev, p_cur, p_next, fwd, bwd, last, params=None,
-               dtype=np.float32):
-    """Stabilization body of one doOneStep call (float HWC frames)."""
+               dtype=np.float32, solver="exact"):
+    """Stabilization body of one doOneStep call (float HWC frames); solver
+    "exact" (reference op order) or "fma" (FMA form, float32)."""
     prm = dict(DEFAULT_PARAMS)
     if params:
         prm.update(params)
@@ -226,13 +242,13 @@ def stab_frame(o_prev, o_cur, o_next, p_prev, p_cur, p_next, fwd, bwd, last, par
     last = _c(last, dtype)
     H, W, _ = ims[0].shape
     out = np.empty((H, W, 3), dtype=dtype)
-    getattr(lib(), "vsco_stab_frame_" + _sfx(dtype))(
+    _solver_fn("vsco_stab_frame_", solver, dtype)(
         *ims, fwd, bwd, fwd.shape[2], last, prm["alpha"], prm["beta"], prm["gamma"],
         prm["pyramid_levels"], prm["num_iter"], prm["step_size"], prm["mom_fac"], out, W, H)
     return out
 
 
-def stabilize_sequence(orig_rgba, proc_rgba, fwd_flows, bwd_flows, params=None, dtype=np.float32):
+def stabilize_sequence(orig_rgba, proc_rgba, fwd_flows, bwd_flows, params=None, dtype=np.float32, solver="exact"):
     """The frame loop of FileStabilizer::stabilizeAll with k = 1, batch 1
     (videostabilizer.cpp:136-164,167-265): frames 0 and N-1 are emitted as the
     processed frames; frame i in 1..N-2 is stabilized with the window
@@ -249,7 +265,7 @@ def stabilize_sequence(orig_rgba, proc_rgba, fwd_flows, bwd_flows, params=None, 
     last = Pr[2].copy()
     for i in range(1, n - 1):
         st = stab_frame(O[i - 1], O[i], O[i + 1], Pr[i - 1], Pr[i], Pr[i + 1], fwd_flows[i], bwd_flows[i], last,
-                        params, dtype)
+                        params, dtype, solver)
         outs[i] = float_to_rgba(st)
         floats[i] = st
         last = st

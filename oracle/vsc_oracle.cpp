@@ -13,6 +13,11 @@
  *   *_f64 : an FP64 twin (same formulas in double, libm exp) used as the error
  *           budget ("FP64 oracle") for PSNR / relative-error reporting.
  *
+ * The solver also has an FMA form (stab_consist_out_fma, *_fma_f32): the same
+ * update rearranged into fused multiply-adds, the arithmetic of the library's
+ * VSC_FRAME_FMA mode; it is pinned against the reference-order solver and the
+ * FP64 twin (tests/test_oracle_stab.py).
+ *
  * Semantics that differ from the reference on purpose (documented deviations):
  *   - the consistency solver (kernel_consist_out) is evaluated with Jacobi
  *     (double-buffered) semantics: every sweep reads only the previous sweep.
@@ -390,6 +395,61 @@ static void stab_consist_out(const T* P, const T* A, const T* Wt, int iters, T s
     if (cur != O) std::memcpy(O, cur, n * sizeof(T));
 }
 
+/* The same solver in FMA form (the library's VSC_FRAME_FMA mode): the
+ * algebraically equal update
+ *   g = sumO - (cnt + w)*O - (lapP - w*A)
+ * with per-pixel constants q = cnt + w and k = s*fma(-w, A, lapP) (lapP as
+ * above), and per sweep
+ *   X = fma(-q, O, sumO); U' = fma(s, X, -k); O' = O + U' (then fma(m, U, O')
+ *   for k > 0).
+ * Each fma rounds once (std::fmaf), so the result differs from the reference
+ * op order by rounding only. */
+template <typename T>
+static void stab_consist_out_fma(const T* P, const T* A, const T* Wt, int iters, T step, T mom, T* O, int W, int H)
+{
+    const size_t n = (size_t)W * H * 3;
+    std::vector<T> Ob(n), U(n, (T)0), Un(n), Q(n), K(n);
+#pragma omp parallel for schedule(static)
+    for (int iy = 0; iy < H; ++iy)
+        for (int ix = 0; ix < W; ++ix)
+            for (int c = 0; c < 3; ++c) {
+                const size_t i = ((size_t)iy * W + ix) * 3 + c;
+                int cnt = 0;
+                T lp = 0;
+                if ((ix + 1) < (W - 1)) { lp += P[i + 3]; cnt += 1; }
+                if ((ix - 1) >= 0) { lp += P[i - 3]; cnt += 1; }
+                if ((iy + 1) < (H - 1)) { lp += P[i + (size_t)W * 3]; cnt += 1; }
+                if ((iy - 1) >= 0) { lp += P[i - (size_t)W * 3]; cnt += 1; }
+                lp -= (T)cnt * P[i];
+                Q[i] = (T)cnt + Wt[i];
+                K[i] = step * std::fma(-Wt[i], A[i], lp);
+            }
+    T* cur = O;
+    T* nxt = Ob.data();
+    for (int k = 0; k < iters; ++k) {
+#pragma omp parallel for schedule(static)
+        for (int iy = 0; iy < H; ++iy)
+            for (int ix = 0; ix < W; ++ix)
+                for (int c = 0; c < 3; ++c) {
+                    const size_t i = ((size_t)iy * W + ix) * 3 + c;
+                    T sum = 0;
+                    if ((ix + 1) < (W - 1)) sum += cur[i + 3];
+                    if ((ix - 1) >= 0) sum += cur[i - 3];
+                    if ((iy + 1) < (H - 1)) sum += cur[i + (size_t)W * 3];
+                    if ((iy - 1) >= 0) sum += cur[i - (size_t)W * 3];
+                    const T X = std::fma(-Q[i], cur[i], sum);
+                    const T sg = std::fma(step, X, -K[i]);
+                    T o = cur[i] + sg;
+                    if (k > 0) o = std::fma(mom, U[i], o);
+                    Un[i] = sg;
+                    nxt[i] = o;
+                }
+        std::swap(cur, nxt);
+        U.swap(Un);
+    }
+    if (cur != O) std::memcpy(O, cur, n * sizeof(T));
+}
+
 /* kernel_bilinear, flowconsistency.cu:50-75: xx = (ox*inW)/outW (no half-pixel
  * offset), neighbours clamped to the last row/column, channels = out channels,
  * read with the input's channel stride. */
@@ -440,6 +500,9 @@ VSCO_API void vsco_stab_consist_out_f32(const float* P, const float* A, const fl
 VSCO_API void vsco_stab_consist_out_f64(const double* P, const double* A, const double* Wt, int iters, double step,
                                         double mom, double* O, int W, int H)
 { stab_consist_out<double>(P, A, Wt, iters, step, mom, O, W, H); }
+VSCO_API void vsco_stab_consist_out_fma_f32(const float* P, const float* A, const float* Wt, int iters, float step,
+                                            float mom, float* O, int W, int H)
+{ stab_consist_out_fma<float>(P, A, Wt, iters, step, mom, O, W, H); }
 VSCO_API void vsco_stab_bilinear_f32(const float* in, int inW, int inH, int inC, float* out, int outW, int outH, int outC)
 { stab_bilinear<float>(in, inW, inH, inC, out, outW, outH, outC); }
 VSCO_API void vsco_stab_bilinear_f64(const double* in, int inW, int inH, int inC, double* out, int outW, int outH, int outC)
@@ -511,6 +574,7 @@ struct FrameParams {
     float alpha, beta, gamma;
     int pyramid_levels, num_iter;
     float step_size, mom_fac;
+    bool fma;  /* solver in FMA form (stab_consist_out_fma) */
 };
 
 /* Inputs: the 3-frame window of originals and processed frames as RGB float
@@ -565,8 +629,12 @@ static void stab_frame(const T* oPrev, const T* oCur, const T* oNext, const T* p
     for (int j = L - 1; j >= 0; --j) {
         if (j != L - 1)
             stab_bilinear<T>(pO[j + 1].data(), lw_[j + 1], lh_[j + 1], 3, pO[j].data(), lw_[j], lh_[j], 3);
-        stab_consist_out<T>(pP[j].data(), pA[j].data(), pW[j].data(), prm.num_iter / (j + 1), (T)prm.step_size,
-                            (T)prm.mom_fac, pO[j].data(), lw_[j], lh_[j]);
+        if (prm.fma)
+            stab_consist_out_fma<T>(pP[j].data(), pA[j].data(), pW[j].data(), prm.num_iter / (j + 1),
+                                    (T)prm.step_size, (T)prm.mom_fac, pO[j].data(), lw_[j], lh_[j]);
+        else
+            stab_consist_out<T>(pP[j].data(), pA[j].data(), pW[j].data(), prm.num_iter / (j + 1), (T)prm.step_size,
+                                (T)prm.mom_fac, pO[j].data(), lw_[j], lh_[j]);
     }
     std::memcpy(out, pO[0].data(), n * sizeof(T));
 }
@@ -576,7 +644,16 @@ VSCO_API void vsco_stab_frame_f32(const float* oPrev, const float* oCur, const f
                                   const float* last, float alpha, float beta, float gamma, int levels, int iters,
                                   float step, float mom, float* out, int W, int H)
 {
-    FrameParams p{alpha, beta, gamma, levels, iters, step, mom};
+    FrameParams p{alpha, beta, gamma, levels, iters, step, mom, false};
+    stab_frame<float>(oPrev, oCur, oNext, pPrev, pCur, pNext, fwd, bwd, flow_ch, last, p, out, W, H);
+}
+
+VSCO_API void vsco_stab_frame_fma_f32(const float* oPrev, const float* oCur, const float* oNext, const float* pPrev,
+                                      const float* pCur, const float* pNext, const float* fwd, const float* bwd,
+                                      int flow_ch, const float* last, float alpha, float beta, float gamma, int levels,
+                                      int iters, float step, float mom, float* out, int W, int H)
+{
+    FrameParams p{alpha, beta, gamma, levels, iters, step, mom, true};
     stab_frame<float>(oPrev, oCur, oNext, pPrev, pCur, pNext, fwd, bwd, flow_ch, last, p, out, W, H);
 }
 
@@ -585,7 +662,7 @@ VSCO_API void vsco_stab_frame_f64(const double* oPrev, const double* oCur, const
                                   int flow_ch, const double* last, float alpha, float beta, float gamma, int levels,
                                   int iters, float step, float mom, double* out, int W, int H)
 {
-    FrameParams p{alpha, beta, gamma, levels, iters, step, mom};
+    FrameParams p{alpha, beta, gamma, levels, iters, step, mom, false};
     stab_frame<double>(oPrev, oCur, oNext, pPrev, pCur, pNext, fwd, bwd, flow_ch, last, p, out, W, H);
 }
 

@@ -31,7 +31,7 @@ def test_pipeline_matches_manual_composition(vsc, cuda, form):
     pipe = vsc.Pipeline(W, H, form=form, seed=5)
     pipe.set_last_rgba(p[2])
     net = vsc.FlowNet(W, H, kw["downscale"], kw["flow_dtype"], seed=5, slots=3)
-    stab = vsc.VideoStabilizer(W, H)
+    stab = vsc.VideoStabilizer(W, H, mode=vsc.VSC_FRAME_FMA if kw["solver"] == "fma" else vsc.VSC_FRAME_FUSED)
     stab.set_last_rgba(p[2])
     ups = vsc.Upsampler(params=vsc.load_upsampler_weights()) if kw["upsample"] == "learned" else None
     prev_fwd = None

@@ -119,15 +119,26 @@ def _oracle_params(params):
     return {m.get(k, k): v for k, v in (params or {}).items()}
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+def _solver(mode):
+    """The oracle solver a frame mode follows: FMA form for VSC_FRAME_FMA."""
+    return "fma" if mode == 2 else "exact"
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_sequence_640x360_bit_exact(vsc, orc, cuda, mode):
     print(f"sequence 640x360 mode={mode}", flush=True)
     orig, proc, fwd, bwd = vsc.synth_video(nframes=6)
     outs, floats = _run_sequence(vsc, orig, proc, fwd, bwd, cuda, mode=mode)
-    r_outs, r_floats = orc.stabilize_sequence(orig, proc, fwd, bwd)
+    r_outs, r_floats = orc.stabilize_sequence(orig, proc, fwd, bwd, solver=_solver(mode))
     for i in range(1, 5):
         assert np.array_equal(floats[i], r_floats[i]), i
         assert np.array_equal(outs[i], r_outs[i]), i
+    if mode == 2:
+        # the FMA form stays within 1 LSB of the reference op order
+        e_outs, _ = orc.stabilize_sequence(orig, proc, fwd, bwd)
+        for i in range(1, 5):
+            d = (outs[i][..., :3].astype(int) - e_outs[i][..., :3].astype(int)) % 256
+            assert np.minimum(d, 256 - d).max() <= 1, i
 
 
 @pytest.mark.parametrize("params", [
@@ -137,14 +148,16 @@ def test_sequence_640x360_bit_exact(vsc, orc, cuda, mode):
     {"alpha": 100.0, "beta": 9000.0, "gamma": 7.5, "numIter": 17, "stepSize": 0.2, "momFac": 0.5},
     {"numIter": 0},
 ])
-def test_frame_params(vsc, orc, cuda, params):
+@pytest.mark.parametrize("mode", [0, 2])
+def test_frame_params(vsc, orc, cuda, params, mode):
     orig, proc, fwd, bwd = vsc.synth_video(nframes=4, width=160, height=96, nblobs=40, seed=11)
-    outs, floats = _run_sequence(vsc, orig, proc, fwd, bwd, cuda, params=params)
-    r_outs, r_floats = orc.stabilize_sequence(orig, proc, fwd, bwd, params=_oracle_params(params))
+    outs, floats = _run_sequence(vsc, orig, proc, fwd, bwd, cuda, params=params, mode=mode)
+    r_outs, r_floats = orc.stabilize_sequence(orig, proc, fwd, bwd, params=_oracle_params(params),
+                                              solver=_solver(mode))
     for i in (1, 2):
         assert np.array_equal(floats[i], r_floats[i])
         assert np.array_equal(outs[i], r_outs[i])
-    if params.get("gamma") == 0.0:
+    if params.get("gamma") == 0.0 and mode == 0:
         # gamma = 0, one level: output = processed frame, exactly
         assert np.array_equal(outs[1][..., :3], proc[1][..., :3])
 
@@ -152,9 +165,10 @@ def test_frame_params(vsc, orc, cuda, params):
 @pytest.mark.parametrize("W,H", [(37, 29), (130, 4), (4, 70), (255, 129)])
 def test_frame_odd_sizes(vsc, orc, cuda, W, H):
     orig, proc, fwd, bwd = vsc.synth_video(nframes=4, width=W, height=H, nblobs=20, seed=W, patch_size=8)
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         outs, floats = _run_sequence(vsc, orig, proc, fwd, bwd, cuda, params={"numIter": 21}, mode=mode)
-        r_outs, r_floats = orc.stabilize_sequence(orig, proc, fwd, bwd, params={"num_iter": 21})
+        r_outs, r_floats = orc.stabilize_sequence(orig, proc, fwd, bwd, params={"num_iter": 21},
+                                                  solver=_solver(mode))
         for i in (1, 2):
             assert np.array_equal(floats[i], r_floats[i]), (mode, i)
 
@@ -173,11 +187,12 @@ def test_flowdownscale_upsampled_flows(vsc, orc, cuda):
         assert np.array_equal(floats[i], r_floats[i])
 
 
-def test_frame_1080p_single(vsc, orc, cuda):
-    print("frame 1080p", flush=True)
+@pytest.mark.parametrize("mode", [0, 2])
+def test_frame_1080p_single(vsc, orc, cuda, mode):
+    print(f"frame 1080p mode={mode}", flush=True)
     orig, proc, fwd, bwd = vsc.synth_video(**dict(vsc.SYNTH_1080P, nframes=3))
-    outs, floats = _run_sequence(vsc, orig, proc, fwd, bwd, cuda)
-    r_outs, r_floats = orc.stabilize_sequence(orig, proc, fwd, bwd)
+    outs, floats = _run_sequence(vsc, orig, proc, fwd, bwd, cuda, mode=mode)
+    r_outs, r_floats = orc.stabilize_sequence(orig, proc, fwd, bwd, solver=_solver(mode))
     assert np.array_equal(floats[1], r_floats[1])
     assert np.array_equal(outs[1], r_outs[1])
 

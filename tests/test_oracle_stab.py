@@ -185,6 +185,74 @@ def test_solver_one_sweep_order(orc, rng):
     assert np.array_equal(got, ref)
 
 
+# ---- the FMA form of the solver ---------------------------------------------------------
+
+def test_fma_solver_1x1_against_fp64_recurrence(orc):
+    """No neighbours: the FMA form follows the heavy-ball recurrence
+    O' = O - s*w*(O-A) + m*U evaluated in float64 to float rounding, and
+    converges to the anchor."""
+    P = np.array([[[0.3, 0.6, 0.9]]], np.float32)
+    A = np.array([[[0.7, 0.1, 0.5]]], np.float32)
+    Wt = np.array([[[1.5, 0.2, 2.0]]], np.float32)
+    O = orc.stab_consist_out(P, A, Wt, P, 37, solver="fma")
+    o = P.astype(np.float64)
+    u = np.zeros_like(o)
+    for k in range(37):
+        sg = -0.15 * Wt * (o - A)
+        o = o + sg + (0.15 * u if k > 0 else 0.0)
+        u = sg
+    np.testing.assert_allclose(O, o, rtol=0, atol=2e-7)
+    O = orc.stab_consist_out(P, A, Wt, P, 1500, solver="fma")
+    np.testing.assert_allclose(O, A, atol=1e-5)
+
+
+def test_fma_solver_converges_to_sparse_direct_solution(orc, rng):
+    """Same fixed point as the reference order: neighbour sets, cnt + w and
+    lapP - w*A enter with the right signs and positions."""
+    H, W = 9, 12
+    P = rng.random((H, W, 3)).astype(np.float32)
+    A = rng.random((H, W, 3)).astype(np.float32)
+    Wt = rng.uniform(0.5, 2.0, (H, W, 3)).astype(np.float32)
+    ref = _sparse_fixed_point(P, A, Wt)
+    O32 = orc.stab_consist_out(P, A, Wt, P, 3000, solver="fma")
+    np.testing.assert_allclose(O32, ref, atol=2e-5)
+
+
+def test_fma_solver_sweeps_track_reference_order(orc, rng):
+    """Sweep by sweep the two forms differ by float rounding only (both within
+    a few ulp of the FP64 sweeps), including momentum, edges and w = 0."""
+    H, W = 11, 14
+    P = rng.random((H, W, 3)).astype(np.float32)
+    A = rng.random((H, W, 3)).astype(np.float32)
+    Wt = rng.uniform(0, 2, (H, W, 3)).astype(np.float32)
+    Wt[2:5, 3:9] = 0
+    for iters in (1, 2, 7, 40):
+        o64 = orc.stab_consist_out(P, A, Wt, P, iters, dtype=np.float64)
+        ex = orc.stab_consist_out(P, A, Wt, P, iters)
+        fm = orc.stab_consist_out(P, A, Wt, P, iters, solver="fma")
+        assert np.abs(fm - o64).max() <= 2e-6, iters
+        assert np.abs(ex - o64).max() <= 2e-6, iters
+    O = orc.stab_consist_out(P, A, np.zeros_like(P), P, 150, solver="fma")
+    np.testing.assert_allclose(O, P, rtol=0, atol=2e-6)   # gamma = 0: the processed frame stays
+
+
+def test_fma_frames_within_one_lsb_of_reference_order(orc, vsc):
+    """Whole frames (2-level pyramid, 150 sweeps) of a synthetic sequence: the
+    FMA form is within 1 LSB of the reference order everywhere and as close to
+    the FP64 twin."""
+    orig, proc, fwd, bwd = vsc.synth_video(width=96, height=64, nframes=5, nblobs=40, motion=3.0, npatches=1,
+                                           patch_size=16, seed=4)
+    ex, exf = orc.stabilize_sequence(orig, proc, fwd, bwd)
+    fm, fmf = orc.stabilize_sequence(orig, proc, fwd, bwd, solver="fma")
+    _, f64 = orc.stabilize_sequence(orig, proc, fwd, bwd, dtype=np.float64)
+    for i in range(1, 4):
+        d = (fm[i][..., :3].astype(int) - ex[i][..., :3].astype(int)) % 256
+        assert np.minimum(d, 256 - d).max() <= 1
+        e_fma = np.abs(fmf[i] - f64[i]).max()
+        e_ex = np.abs(exf[i] - f64[i]).max()
+        assert e_fma <= max(2 * e_ex, 1e-6), (i, e_fma, e_ex)
+
+
 # ---- bilinear resample --------------------------------------------------------------
 
 def test_bilinear_decimation_and_midpoints(orc, rng):

@@ -32,7 +32,8 @@ SYNTH_PATH = os.path.join(_HERE, "libvsc_synth.so")
 VSC_OK = 0
 VSC_E_AGAIN = -6
 VSC_F32, VSC_F64, VSC_BF16, VSC_F16, VSC_FP8E4M3 = 0, 1, 2, 3, 4
-VSC_FRAME_FUSED, VSC_FRAME_STAGED = 0, 1
+VSC_FRAME_FUSED, VSC_FRAME_STAGED, VSC_FRAME_FMA = 0, 1, 2
+SOLVERS = {"exact": 0, "fma": 1}    # stabilization solver: reference op order | FMA form
 VSC_MAX_LEVELS = 4
 
 #: every function the C header declares (checked against the built library)
@@ -79,7 +80,7 @@ class VscFrameIn(ctypes.Structure):
 class PipelineCfg(ctypes.Structure):
     _fields_ = [("flow_dtype", ctypes.c_int), ("downscale", ctypes.c_int), ("upsample", ctypes.c_int),
                 ("scale_vectors", ctypes.c_int), ("fb_max_inconsistent", ctypes.c_float),
-                ("ref_bwd_index", ctypes.c_int), ("seed", ctypes.c_ulonglong)]
+                ("ref_bwd_index", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("solver", ctypes.c_int)]
 
 
 class Window(ctypes.Structure):
@@ -726,17 +727,20 @@ class Pipeline:
     proc_next) and `frame` is the index of orig_cur (features are cached by
     index)."""
 
+    # both forms stabilize with the FMA-form solver (FP32, <= 1 LSB from the
+    # reference op order); solver="exact" selects the reference order
     FORMS = {
-        "plain": dict(flow_dtype="f32", downscale=1, upsample="bilinear"),
-        "chosen": dict(flow_dtype="bf16", downscale=2, upsample="learned"),
+        "plain": dict(flow_dtype="f32", downscale=1, upsample="bilinear", solver="fma"),
+        "chosen": dict(flow_dtype="bf16", downscale=2, upsample="learned", solver="fma"),
     }
 
     def __init__(self, width, height, form=None, flow_dtype="f32", downscale=1, upsample="bilinear",
                  scale_vectors=False, fb_max_inconsistent=0.25, ref_bwd_index=False, seed=0, upsampler_params=None,
-                 flow_head_scale=None):
+                 flow_head_scale=None, solver=None):
         if form is not None:
             kw = self.FORMS[form]
             flow_dtype, downscale, upsample = kw["flow_dtype"], kw["downscale"], kw["upsample"]
+            solver = solver or kw["solver"]
         c = PipelineCfg()
         lib().vsc_pipeline_default_cfg(ctypes.byref(c))
         c.flow_dtype = {"f32": VSC_F32, "bf16": VSC_BF16}[flow_dtype]
@@ -746,6 +750,7 @@ class Pipeline:
         c.fb_max_inconsistent = fb_max_inconsistent
         c.ref_bwd_index = int(bool(ref_bwd_index))
         c.seed = seed
+        c.solver = SOLVERS[solver or "exact"]
         self.cfg = c
         self.width, self.height = width, height
         h = ctypes.c_void_p()
@@ -771,6 +776,7 @@ class Pipeline:
         c.flow_dtype = {"f32": VSC_F32, "bf16": VSC_BF16}[fk["flow_dtype"]]
         c.downscale = fk["downscale"]
         c.upsample = {"bilinear": 0, "learned": 1}[fk["upsample"]]
+        c.solver = SOLVERS[fk["solver"]]
         return int(lib().vsc_pipeline_footprint(width, height, ctypes.byref(c)))
 
     def _stream_handle(self):
@@ -826,7 +832,7 @@ class Server:
     budget, continuous batching of ready windows over a pool of HIP streams."""
 
     def __init__(self, hbm_budget=64 << 30, max_streams=64, hip_streams=4, form="chosen", upsampler_params=None,
-                 use_graphs=True):
+                 use_graphs=True, solver=None):
         c = ServerCfg()
         lib().vsc_server_default_cfg(ctypes.byref(c))
         c.max_streams, c.hbm_budget, c.hip_streams = max_streams, hbm_budget, hip_streams
@@ -835,6 +841,7 @@ class Server:
         c.form.flow_dtype = {"f32": VSC_F32, "bf16": VSC_BF16}[fk["flow_dtype"]]
         c.form.downscale = fk["downscale"]
         c.form.upsample = {"bilinear": 0, "learned": 1}[fk["upsample"]]
+        c.form.solver = SOLVERS[solver or fk["solver"]]
         self._params = None
         if c.form.upsample == 1:
             self._params = np.ascontiguousarray(

@@ -73,6 +73,7 @@ extern "C" void vsc_pipeline_default_cfg(vsc_pipeline_cfg* c)
     c->fb_max_inconsistent = 0.25f;
     c->ref_bwd_index = 0;
     c->seed = 0;
+    c->solver = 0;
 }
 
 static int cfg_ok(int W, int H, const vsc_pipeline_cfg* c)
@@ -83,6 +84,7 @@ static int cfg_ok(int W, int H, const vsc_pipeline_cfg* c)
     if (c->downscale != 1 && c->downscale != 2 && c->downscale != 4)
         return set_error(VSC_E_INVALID, "vsc_pipeline: downscale must be 1, 2 or 4");
     if (c->upsample != 0 && c->upsample != 1) return set_error(VSC_E_INVALID, "vsc_pipeline: upsample must be 0 or 1");
+    if (c->solver != 0 && c->solver != 1) return set_error(VSC_E_INVALID, "vsc_pipeline: solver must be 0 or 1");
     return VSC_OK;
 }
 
@@ -248,7 +250,8 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
     }
     p->prev_fwd_frame = frame;
     vsc_frame_in in{w->orig_prev, w->orig_cur, w->orig_next, w->proc_prev, w->proc_cur, w->proc_next, ff, fb, 2};
-    return vsc_stab_frame(p->stab, &in, params, out_rgba, out_float, VSC_FRAME_FUSED, stream);
+    return vsc_stab_frame(p->stab, &in, params, out_rgba, out_float, p->cfg.solver ? VSC_FRAME_FMA : VSC_FRAME_FUSED,
+                          stream);
 }
 
 extern "C" int vsc_pipeline_gate(const vsc_pipeline* p, float* gate_dev, uint32_t* count_dev, vsc_hip_stream stream)

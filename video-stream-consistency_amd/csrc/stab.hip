@@ -195,12 +195,12 @@ int launch_consist_sweep(View P, View A, View Wt, View Oin, View Oout, View U, f
 // consistency solver, temporally blocked (planar state, one channel per block)
 //
 // A block of 128*STRIPS threads owns a TW x TH tile of one channel plane
-// (TW = 128 columns, TH = STRIPS*R rows) that overlaps its neighbours by a halo of HALO
+// (TW = 128 columns, TH = STRIPS*R rows) that overlaps its neighbours by a halo of `halo`
 // pixels.  Thread (tx, strip) keeps the R pixels of column tx in rows
 // strip*R .. strip*R+R-1 in registers: O, U, lapP, A, w.  O and U are read
 // with halo from the *in* planes and written to the *out* planes (neighbouring
-// blocks read the pixels a block writes).  After `sweeps` <= HALO sweeps the
-// inner (TW-2*HALO) x (TH-2*HALO) pixels are exact and written back; halo
+// blocks read the pixels a block writes).  After `sweeps` <= halo sweeps the
+// inner (TW-2*halo) x (TH-2*halo) pixels are exact and written back; halo
 // pixels whose out-of-tile neighbours are missing never reach the output.
 //
 // Neighbour exchange goes through LDS, stored column-major: the R values of
@@ -270,16 +270,67 @@ __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const 
     }
 }
 
-template <int R, int HALO, int STRIPS>
+// The same sweep in FMA form: q = cnt + w and kk = step*fma(-w, A, lapP) are
+// per-pixel constants, and per pixel
+//   X = fma(-q, O, sum of neighbours); U' = fma(step, X, -kk); O' = O + U' (+ mom*U)
+// (7 VALU operations in the interior instead of 13).
+template <int R, bool MOM, bool INTERIOR>
+__device__ __forceinline__ void sweep_strip_fma(float (&o)[R], float (&u)[R], const float (&q)[R],
+                                                const float (&kk)[R], const float* colL, const float* colR,
+                                                float up_edge, float dn_edge, float step, float mom, bool hr, bool hl,
+                                                int rU, int rD)
+{
+    float4 l4 = *reinterpret_cast<const float4*>(colL);
+    float4 r4 = *reinterpret_cast<const float4*>(colR);
+    float up_old = up_edge;
+#pragma unroll
+    for (int k = 0; k < R / 4; ++k) {
+        float4 ln = l4, rn = r4;
+        if (k + 1 < R / 4) {
+            ln = *reinterpret_cast<const float4*>(colL + 4 * (k + 1));
+            rn = *reinterpret_cast<const float4*>(colR + 4 * (k + 1));
+        }
+        const float lv[4] = {l4.x, l4.y, l4.z, l4.w};
+        const float rv[4] = {r4.x, r4.y, r4.z, r4.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = 4 * k + j;
+            const float cur = o[r];
+            const float dn = (r < R - 1) ? o[r + 1] : dn_edge;
+            float sum;
+            if (INTERIOR) {
+                sum = rv[j] + lv[j];
+                sum += dn;
+                sum += up_old;
+            } else {
+                sum = 0.0f;
+                if (hr) sum += rv[j];
+                if (hl) sum += lv[j];
+                if (r <= rD) sum += dn;
+                if (r >= rU) sum += up_old;
+            }
+            const float X = __builtin_fmaf(-q[r], cur, sum);
+            const float sg = __builtin_fmaf(step, X, -kk[r]);
+            const float on = cur + sg;
+            o[r] = MOM ? __builtin_fmaf(mom, u[r], on) : on;
+            u[r] = sg;
+            up_old = cur;
+        }
+        l4 = ln;
+        r4 = rn;
+    }
+}
+
+template <int R, int STRIPS, bool FMA>
 __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     const float* __restrict__ LapP, const float* __restrict__ A, const float* __restrict__ Wt, const float* Oin,
     float* Oout, const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step, float mom,
-    int tiles_x, int tiles_y, int per_xcd)
+    int halo, int tiles_x, int tiles_y, int per_xcd)
 {
     constexpr int TW = SB_TW;
     constexpr int TH = STRIPS * R;
     constexpr int RS = ((R / 4) % 2 == 0) ? R + 4 : R + 8;
-    constexpr int OW = TW - 2 * HALO, OH = TH - 2 * HALO;
+    const int OW = TW - 2 * halo, OH = TH - 2 * halo;
     static_assert(R % 4 == 0 && (RS / 4) % 2 == 1, "strip stride must be an odd number of 16-byte groups");
     __shared__ __attribute__((aligned(16))) float sO[2][STRIPS][TW][RS];
 
@@ -307,8 +358,8 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     const int strip = __builtin_amdgcn_readfirstlane(threadIdx.x / TW);
     const int wave_tx0 = __builtin_amdgcn_readfirstlane((threadIdx.x % TW) & ~63);
     const int ty0 = strip * R;
-    const int gx0 = btx * OW - HALO;
-    const int gy0 = bty * OH - HALO;
+    const int gx0 = btx * OW - halo;
+    const int gy0 = bty * OH - halo;
     const int gx = gx0 + tx;
     const bool colin = gx >= 0 && gx < W;
     const bool hr = (gx + 1) < (W - 1), hl = (gx - 1) >= 0;
@@ -317,7 +368,8 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     const bool interior = (gx0 + wave_tx0 >= 1) && (gx0 + wave_tx0 + 63 <= W - 3) && (gys >= 1) &&
                           (gys + R - 1 <= H - 3);
 
-    float o[R], u[R], a[R], w[R], lp[R];
+    // FMA: w holds q = cnt + w, lp holds kk; a is unused (and not loaded)
+    float o[R], u[R], a[FMA ? 1 : R], w[R], lp[R];
     // Loads are unconditional, from clamped positions, so that all of them
     // issue back to back: a tile pixel outside the image holds some finite
     // in-image value that no in-image pixel ever reads (the neighbour sets
@@ -330,7 +382,7 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
         const int gyc = min(max(gys + r, 0), H - 1);
         const int i = gyc * W + gxc;
         o[r] = Oi[i];
-        a[r] = Ac[i];
+        if (!FMA) a[r] = Ac[i];
         w[r] = Wc[i];
         lp[r] = Lc[i];
     }
@@ -366,7 +418,15 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
         // predicates out of the sweep loop into scalar registers
         asm volatile("" : "+s"(rU), "+s"(rD));
         const bool mom_on = (first_sweep + s) > 0;
-        if (interior) {
+        if constexpr (FMA) {
+            if (interior) {
+                if (mom_on) sweep_strip_fma<R, true, true>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+                else sweep_strip_fma<R, false, true>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+            } else {
+                if (mom_on) sweep_strip_fma<R, true, false>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+                else sweep_strip_fma<R, false, false>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+            }
+        } else if (interior) {
             if (mom_on) sweep_strip<R, true, true>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
             else sweep_strip<R, false, true>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
         } else {
@@ -375,13 +435,13 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
         }
     }
 
-    if (tx >= HALO && tx < TW - HALO && colin) {
+    if (tx >= halo && tx < TW - halo && colin) {
         const int wbase = gys * W + gx;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int ty = ty0 + r;
             const int gy = gy0 + ty;
-            if (ty >= HALO && ty < TH - HALO && gy >= 0 && gy < H) {
+            if (ty >= halo && ty < TH - halo && gy >= 0 && gy < H) {
                 Oo[wbase + r * W] = o[r];
                 Uo[wbase + r * W] = u[r];
             }
@@ -415,49 +475,121 @@ int launch_lap(const float* P, float* L, int W, int H, hipStream_t s)
     return check_launch("lap");
 }
 
-#ifndef VSC_SB_R
-#define VSC_SB_R 28
-#endif
-#ifndef VSC_SB_HALO
-#define VSC_SB_HALO 12
-#endif
-#ifndef VSC_SB_STRIPS
-#define VSC_SB_STRIPS 4
-#endif
-static constexpr int SB_R = VSC_SB_R;
-static constexpr int SB_HALO = VSC_SB_HALO;
-static constexpr int SB_STRIPS = VSC_SB_STRIPS;
+// FMA-form constants of one planar level, in place: q = cnt + w into Wt and
+// kk = step*fma(-w, A, lapP) into L (cnt: the neighbour count of
+// kernel_consist_out, flowconsistency.cu:215-236).
+__global__ __launch_bounds__(256) void k_fma_consts(float* __restrict__ L, const float* __restrict__ A,
+                                                     float* __restrict__ Wt, int W, int H, float step)
+{
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const long long i = (long long)blockIdx.z * W * H + (long long)y * W + x;
+    const int cnt = ((x + 1) < (W - 1) ? 1 : 0) + ((x - 1) >= 0 ? 1 : 0) + ((y + 1) < (H - 1) ? 1 : 0) +
+                    ((y - 1) >= 0 ? 1 : 0);
+    const float w = Wt[i];
+    L[i] = step * __builtin_fmaf(-w, A[i], L[i]);
+    Wt[i] = (float)cnt + w;
+}
 
-int consist_blocked_halo() { return SB_HALO; }
+int launch_fma_consts(float* L, const float* A, float* Wt, int W, int H, float step, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fma_consts, dim3((W + BX - 1) / BX, (H + BY - 1) / BY, 3), dim3(BX, BY), 0, s, L, A, Wt, W,
+                       H, step);
+    return check_launch("fma_consts");
+}
+
+static constexpr int SB_R = 28;      // rows per thread, reference op order (5 values per pixel)
+static constexpr int SB_R_FMA = 32;  // FMA form: 4 values per pixel in registers, taller strips
+static constexpr int SB_STRIPS = 4;
+
+// Launch plan of one pyramid level: the halo (= the most sweeps per launch)
+// and the tile height, from a cost model of the kernel fitted on MI355X.  A
+// launch costs A + r * sweeps, in units of one sweep over one tile per CU:
+// A ~ 24 (the tiles' load and store and the launch itself: 1080p launches of
+// 72.4 us for 12 sweeps and 50.4 us for 6 sweeps over 2.45 tiles per CU) and
+// r = the tiles per CU, half as the mean tile count and half as its ceiling
+// (blocks of the last partial round start late but still lengthen it).  A
+// wider halo means fewer launches per level but more, mostly redundant, tile
+// area.  Half-height tiles (two blocks per CU) when full-height ones would
+// leave CUs idle.
+namespace {
+
+BlockedPlan best_plan(int W, int H, int iters, int R, int strips, long long* tiles_out)
+{
+    constexpr double A = 24.0;
+    BlockedPlan best{12, strips};
+    double best_cost = 1e300;
+    for (int h = 4; h <= 24; ++h) {
+        const int OW = SB_TW - 2 * h, OH = strips * R - 2 * h;
+        if (OH < 8) break;
+        const long long tiles = 3LL * ((W + OW - 1) / OW) * ((H + OH - 1) / OH);
+        const double per_cu = 0.5 * ((double)tiles / 256.0 + (double)((tiles + 255) / 256)) * strips / SB_STRIPS;
+        const int full = iters / h, rem = iters % h;
+        const double cost = full * (A + per_cu * h) + (rem ? A + per_cu * rem : 0.0);
+        if (cost < best_cost - 1e-9) {
+            best_cost = cost;
+            best = BlockedPlan{h, strips};
+            *tiles_out = tiles;
+        }
+    }
+    return best;
+}
+
+}  // namespace
+
+BlockedPlan plan_consist_blocked(int W, int H, int iters, int fma)
+{
+    const int R = fma ? SB_R_FMA : SB_R;
+    long long tiles = 0;
+    const BlockedPlan full = best_plan(W, H, iters, R, SB_STRIPS, &tiles);
+    if (tiles >= 256) return full;
+    return best_plan(W, H, iters, R, 2, &tiles);
+}
+
+namespace {
+
+template <int R, bool FMA>
+int launch_blocked(const float* lapP, const float* A, const float* Wt, const float* Oin, float* Oout, const float* Uin,
+                   float* Uout, int W, int H, int sweeps, int first_sweep, float step, float mom,
+                   const BlockedPlan& plan, hipStream_t s)
+{
+    const int OW = SB_TW - 2 * plan.halo;
+    const int OH = plan.strips * R - 2 * plan.halo;
+    const int tiles_x = (W + OW - 1) / OW;
+    const int tiles_y = (H + OH - 1) / OH;
+    const int per_xcd = (tiles_x * tiles_y * 3 + 7) / 8;
+    // algorithmic bytes: read O, U and the per-pixel constants (lapP, A, w; FMA
+    // form: q, kk) and write O, U once per pixel and channel
+    const int slot = prof_start(PROF_CONSIST_BLOCKED, (FMA ? 6.0 : 7.0) * 3 * 4 * (double)W * H, s);
+    if (plan.strips == SB_STRIPS)
+        hipLaunchKernelGGL((k_consist_blocked<R, SB_STRIPS, FMA>), dim3(8 * per_xcd), dim3(SB_TW * SB_STRIPS), 0, s,
+                           lapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, plan.halo, tiles_x,
+                           tiles_y, per_xcd);
+    else
+        hipLaunchKernelGGL((k_consist_blocked<R, 2, FMA>), dim3(8 * per_xcd), dim3(SB_TW * 2), 0, s, lapP, A, Wt, Oin,
+                           Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, plan.halo, tiles_x, tiles_y, per_xcd);
+    prof_stop(slot, s);
+    return check_launch("consist_blocked");
+}
+
+}  // namespace
 
 int launch_consist_blocked(const float* lapP, const float* A, const float* Wt, const float* Oin, float* Oout,
                            const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step,
-                           float mom, hipStream_t s)
+                           float mom, int fma, const BlockedPlan& plan, hipStream_t s)
 {
-    if (sweeps < 1 || sweeps > SB_HALO) return set_error(VSC_E_INVALID, "consist_blocked: sweeps out of range");
-    constexpr int OW = SB_TW - 2 * SB_HALO;
-    const int tiles_x = (W + OW - 1) / OW;
-    // full-height tiles (SB_STRIPS strips) unless they leave CUs idle: then
-    // half-height tiles, whose 256-thread blocks run two per CU
-    int strips = SB_STRIPS;
-    if (SB_STRIPS > 2) {
-        const int oh = SB_STRIPS * SB_R - 2 * SB_HALO;
-        if (tiles_x * ((H + oh - 1) / oh) * 3 < 256) strips = 2;
-    }
-    const int OH = strips * SB_R - 2 * SB_HALO;
-    const int tiles_y = (H + OH - 1) / OH;
-    const int per_xcd = (tiles_x * tiles_y * 3 + 7) / 8;
-    // algorithmic bytes: read O, U, lapP, A, w and write O, U once per pixel and channel
-    const int slot = prof_start(PROF_CONSIST_BLOCKED, 7.0 * 3 * 4 * (double)W * H, s);
-    if (strips == SB_STRIPS)
-        hipLaunchKernelGGL((k_consist_blocked<SB_R, SB_HALO, SB_STRIPS>), dim3(8 * per_xcd), dim3(SB_TW * SB_STRIPS), 0,
-                           s, lapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, tiles_x, tiles_y,
-                           per_xcd);
-    else
-        hipLaunchKernelGGL((k_consist_blocked<SB_R, SB_HALO, 2>), dim3(8 * per_xcd), dim3(SB_TW * 2), 0, s, lapP, A, Wt,
-                           Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, tiles_x, tiles_y, per_xcd);
-    prof_stop(slot, s);
-    return check_launch("consist_blocked");
+    const int R = fma ? SB_R_FMA : SB_R;
+    if (plan.strips != SB_STRIPS && plan.strips != 2)
+        return set_error(VSC_E_INVALID, "consist_blocked: strips must be 2 or 4");
+    if (plan.halo < 1 || SB_TW - 2 * plan.halo < 1 || plan.strips * R - 2 * plan.halo < 1)
+        return set_error(VSC_E_INVALID, "consist_blocked: halo leaves no output tile");
+    if (sweeps < 1 || sweeps > plan.halo) return set_error(VSC_E_INVALID, "consist_blocked: sweeps out of range");
+    if (fma)
+        return launch_blocked<SB_R_FMA, true>(lapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom,
+                                              plan, s);
+    return launch_blocked<SB_R, false>(lapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, plan,
+                                       s);
 }
 
 // ---------------------------------------------------------------------------
@@ -492,7 +624,7 @@ __device__ inline void warp_rgba(const Rgba32& im, const WarpTap& t, float out[3
 // lapP of the processed frame at (x, y) of a level that samples the frame
 // every `st` pixels (st = 1: level 0; st = 2: the decimated level 1 of size
 // Wl x Hl), for all 3 channels.
-__device__ inline void lap_rgba(const Rgba32& im, int x, int y, int st, int xl, int yl, int Wl, int Hl, float out[3])
+__device__ inline int lap_rgba(const Rgba32& im, int x, int y, int st, int xl, int yl, int Wl, int Hl, float out[3])
 {
     const uint32_t v = im.px(x, y);
     const bool hr = (xl + 1) < (Wl - 1), hl = (xl - 1) >= 0, hd = (yl + 1) < (Hl - 1), hu = (yl - 1) >= 0;
@@ -509,6 +641,7 @@ __device__ inline void lap_rgba(const Rgba32& im, int x, int y, int st, int xl, 
         l -= (float)cnt * im.ch(v, c);
         out[c] = l;
     }
+    return cnt;
 }
 
 __global__ __launch_bounds__(256) void k_front(FrontArgs a)
@@ -530,12 +663,13 @@ __global__ __launch_bounds__(256) void k_front(FrontArgs a)
     warp_rgba(pPrev, tb, pwp);
     warp_rgba(oNext, tf, nwi);
     warp_rgba(pNext, tf, nwp);
-    lap_rgba(pCur, x, y, 1, x, y, W, H, lap0);
+    const int cnt0 = lap_rgba(pCur, x, y, 1, x, y, W, H, lap0);
     const uint32_t vci = oCur.px(x, y), vcp = pCur.px(x, y);
     const bool dec = a.decimate && ((x | y) & 1) == 0 && (x >> 1) < a.A1.w && (y >> 1) < a.A1.h;
     const int x1 = x >> 1, y1 = y >> 1;
     float lap1[3];
-    if (dec) lap_rgba(pCur, x, y, 2, x1, y1, a.A1.w, a.A1.h, lap1);
+    int cnt1 = 0;
+    if (dec) cnt1 = lap_rgba(pCur, x, y, 2, x1, y1, a.A1.w, a.A1.h, lap1);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float lw = warp_sample(a.last, tb, c);
@@ -544,14 +678,24 @@ __global__ __launch_bounds__(256) void k_front(FrontArgs a)
         float adIn, adPr;
         adap_comb(ci, cp, pwi[c], pwp[c], nwi[c], nwp[c], lw, a.alpha, adIn, adPr);
         const float wt = consist_wt(ci, adIn, a.beta, a.gamma);
-        a.L0.wr(x, y, c, lap0[c]);
-        a.A0.wr(x, y, c, adPr);
-        a.W0.wr(x, y, c, wt);
+        if (a.fma) {
+            a.L0.wr(x, y, c, a.step * __builtin_fmaf(-wt, adPr, lap0[c]));
+            a.W0.wr(x, y, c, (float)cnt0 + wt);
+        } else {
+            a.L0.wr(x, y, c, lap0[c]);
+            a.A0.wr(x, y, c, adPr);
+            a.W0.wr(x, y, c, wt);
+        }
         if (a.write_p) a.P0.wr(x, y, c, cp);
         if (dec) {
-            a.L1.wr(x1, y1, c, lap1[c]);
-            a.A1.wr(x1, y1, c, adPr);
-            a.W1.wr(x1, y1, c, wt);
+            if (a.fma) {
+                a.L1.wr(x1, y1, c, a.step * __builtin_fmaf(-wt, adPr, lap1[c]));
+                a.W1.wr(x1, y1, c, (float)cnt1 + wt);
+            } else {
+                a.L1.wr(x1, y1, c, lap1[c]);
+                a.A1.wr(x1, y1, c, adPr);
+                a.W1.wr(x1, y1, c, wt);
+            }
             a.O1.wr(x1, y1, c, cp);
             if (a.write_p) a.P1.wr(x1, y1, c, cp);
         }
@@ -564,7 +708,10 @@ int launch_front_pass(const FrontArgs& a, hipStream_t s)
     // 2 flows (2 * 4 * channels), level-0 lapP, A, w (36) and, when decimating,
     // a quarter pixel of level-1 lapP, A, w, O (48 / 4)
     const double px = (double)a.A0.w * a.A0.h;
-    const double bytes = px * (24.0 + 12.0 + 4.0 * (a.fwd.ch + a.bwd.ch) + 36.0 + (a.decimate ? 12.0 : 0.0) +
+    // (FMA form: 2 constants per pixel and channel instead of 3)
+    const double per_level = a.fma ? 24.0 : 36.0;
+    const double bytes = px * (24.0 + 12.0 + 4.0 * (a.fwd.ch + a.bwd.ch) + per_level +
+                               (a.decimate ? (per_level + 12.0) / 4.0 : 0.0) +
                                (a.write_p ? (a.decimate ? 15.0 : 12.0) : 0.0));
     const int slot = prof_start(PROF_FRONT_PASS, bytes, s);
     hipLaunchKernelGGL(k_front, grid2d(a.A0.w, a.A0.h), dim3(BX, BY), 0, s, a);
@@ -717,12 +864,12 @@ extern "C" int vsc_stab_consist_out_planar(const float* crnt_pr, const float* pr
     float* U[2] = {ws + n, ws + 2 * n};
     float* lapP = ws + 3 * n;
     rc = launch_lap(crnt_pr, lapP, width, height, s);
-    const int halo = consist_blocked_halo();
+    const BlockedPlan plan = plan_consist_blocked(width, height, num_iter, 0);
     int cur = 0;
     for (int k = 0; k < num_iter && !rc;) {
-        const int m = std::min(halo, num_iter - k);
+        const int m = std::min(plan.halo, num_iter - k);
         rc = launch_consist_blocked(lapP, prev_stab_warp, consist_wt, O[cur], O[cur ^ 1], U[cur], U[cur ^ 1], width,
-                                    height, m, k, step_size, mom_fac, s);
+                                    height, m, k, step_size, mom_fac, 0, plan, s);
         cur ^= 1;
         k += m;
     }

@@ -207,9 +207,8 @@ int prepare_flows(vsc_stream* s, const vsc_frame_in* in, View* fwd, View* bwd, h
 // temporally blocked solver.  Level j initial solution: O[j][0] (level L-1)
 // or bilinear(up) of the level above.  Returns the index of O[0] holding the
 // result.
-int solve_pyramid(vsc_stream* s, int L, const vsc_params& p, hipStream_t st, int* final_idx)
+int solve_pyramid(vsc_stream* s, int L, const vsc_params& p, int fma, hipStream_t st, int* final_idx)
 {
-    const int halo = consist_blocked_halo();
     int cur = 0;
     for (int j = L - 1; j >= 0; --j) {
         const int w = s->lw[j], h = s->lh[j];
@@ -220,11 +219,12 @@ int solve_pyramid(vsc_stream* s, int L, const vsc_params& p, hipStream_t st, int
             if (rc) return rc;
         }
         const int iters = p.num_iter / (j + 1);
+        const BlockedPlan plan = plan_consist_blocked(w, h, iters, fma);
         int ucur = 0;
         for (int k = 0; k < iters;) {
-            const int n = std::min(halo, iters - k);
+            const int n = std::min(plan.halo, iters - k);
             rc = launch_consist_blocked(s->L[j], s->A[j], s->Wt[j], s->O[j][cur], s->O[j][cur ^ 1], s->U[j][ucur],
-                                        s->U[j][ucur ^ 1], w, h, n, k, p.step_size, p.mom_fac, st);
+                                        s->U[j][ucur ^ 1], w, h, n, k, p.step_size, p.mom_fac, fma, plan, st);
             if (rc) return rc;
             cur ^= 1;
             ucur ^= 1;
@@ -235,8 +235,12 @@ int solve_pyramid(vsc_stream* s, int L, const vsc_params& p, hipStream_t st, int
     return VSC_OK;
 }
 
+// fma: solver in FMA form (VSC_FRAME_FMA).  Its per-pixel constants are
+// written by the front pass when it produces every level (one level, or two
+// with exact decimation); otherwise the levels are built as for the reference
+// order and converted in place afterwards.
 int frame_fused(vsc_stream* s, const vsc_frame_in* in, const vsc_params& p, uint8_t* out_rgba, float* out_float,
-                hipStream_t st)
+                int fma, hipStream_t st)
 {
     View fwd, bwd;
     int rc = prepare_flows(s, in, &fwd, &bwd, st);
@@ -275,6 +279,9 @@ int frame_fused(vsc_stream* s, const vsc_frame_in* in, const vsc_params& p, uint
     a.alpha = p.alpha;
     a.beta = p.beta;
     a.gamma = p.gamma;
+    const bool direct = fma && (L == 1 || (L == 2 && a.decimate));
+    a.fma = direct ? 1 : 0;
+    a.step = p.step_size;
     if ((rc = launch_front_pass(a, st))) return rc;
     // remaining pyramid levels (videostabilizer.cpp:203-217): level j from j-1,
     // the initial solution O_j = bilinear(O_{j-1}) = bilinear(P_{j-1}) = P_j
@@ -287,12 +294,15 @@ int frame_fused(vsc_stream* s, const vsc_frame_in* in, const vsc_params& p, uint
         if ((rc = launch_bilinear(pl(s->P[j - 1], w0, h0), pl(s->O[j][0], w1, h1), st))) return rc;
         if ((rc = launch_lap(s->P[j], s->L[j], w1, h1, st))) return rc;
     }
+    if (fma && !direct)
+        for (int j = 0; j < L; ++j)
+            if ((rc = launch_fma_consts(s->L[j], s->A[j], s->Wt[j], s->lw[j], s->lh[j], p.step_size, st))) return rc;
     if (L == 1) {
         // single level: the solution starts at the processed frame (videostabilizer.cpp:209)
         if ((rc = launch_copy_view(pl(s->P[0], W, H), pl(s->O[0][0], W, H), st))) return rc;
     }
     int fin = 0;
-    if ((rc = solve_pyramid(s, L, p, st, &fin))) return rc;
+    if ((rc = solve_pyramid(s, L, p, fma, st, &fin))) return rc;
     const View res = pl(s->O[0][fin], W, H);
     if (out_rgba && (rc = launch_to_rgba(res, out_rgba, st))) return rc;
     if (out_float && (rc = launch_copy_view(res, View{out_float, W, H, 3, 1, 3}, st))) return rc;
@@ -400,7 +410,8 @@ extern "C" int vsc_stab_frame(vsc_stream* s, const vsc_frame_in* in, const vsc_p
     else vsc_default_params(&p);
     if ((rc = check_params(s, p))) return rc;
     hipStream_t st = as_stream(stream);
-    if (mode == VSC_FRAME_FUSED) return frame_fused(s, in, p, out_rgba, out_float, st);
+    if (mode == VSC_FRAME_FUSED) return frame_fused(s, in, p, out_rgba, out_float, 0, st);
+    if (mode == VSC_FRAME_FMA) return frame_fused(s, in, p, out_rgba, out_float, 1, st);
     if (mode == VSC_FRAME_STAGED) return frame_staged(s, in, p, out_rgba, out_float, st);
     return set_error(VSC_E_INVALID, "vsc_stab_frame: unknown mode");
 }

@@ -206,10 +206,20 @@ int launch_bilinear(View in, View out, hipStream_t s);
 int launch_consist_sweep(View P, View A, View Wt, View Oin, View Oout, View U, float step, float mom, int use_mom,
                          hipStream_t s);
 int launch_lap(const float* P, float* L, int W, int H, hipStream_t s);
+// Temporally blocked solver: `halo` = the most sweeps one launch may do,
+// `strips` = 4 (full-height tiles) or 2 (half-height tiles, two blocks per CU).
+struct BlockedPlan {
+    int halo;
+    int strips;
+};
+BlockedPlan plan_consist_blocked(int W, int H, int iters, int fma);
+// fma == 0: lapP, A, Wt are the level's lapP, A, w (reference op order);
+// fma == 1: lapP holds k = step*fma(-w, A, lapP), Wt holds q = cnt + w, A unused.
 int launch_consist_blocked(const float* lapP, const float* A, const float* Wt, const float* Oin, float* Oout,
                            const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step,
-                           float mom, hipStream_t s);
-int consist_blocked_halo();
+                           float mom, int fma, const BlockedPlan& plan, hipStream_t s);
+// in place: Wt <- cnt + Wt, L <- step*fma(-Wt, A, L) (the FMA-form constants of a planar level)
+int launch_fma_consts(float* L, const float* A, float* Wt, int W, int H, float step, hipStream_t s);
 int launch_to_rgba(View in, uint8_t* out, hipStream_t s);
 int launch_rgba_to_view(const uint8_t* in, View out, hipStream_t s);
 int launch_copy_view(View in, View out, hipStream_t s);
@@ -223,6 +233,8 @@ struct FrontArgs {
     float alpha, beta, gamma;
     int decimate;
     int write_p;              // also write P0 (and P1): the pyramid needs them
+    int fma;                  // write the FMA-form constants: L <- step*fma(-w, A, lapP), W <- cnt + w (A unused)
+    float step;
 };
 int launch_front_pass(const FrontArgs& a, hipStream_t s);
 
