@@ -383,6 +383,11 @@ typedef struct {
     unsigned long long seed;    /* flow network (and untrained upsampler) initialisation */
     int solver;                 /* stabilization solver: 0 reference op order (VSC_FRAME_FUSED),
                                    1 FMA form (VSC_FRAME_FMA, <= 1 LSB from it) */
+    int overlap;                /* 1: frame-pipelined execution on two internal HIP streams: the flows
+                                   of a frame run while the previous frame is stabilized.  Frame t's
+                                   outputs are then ordered on the caller's stream by the call for
+                                   frame t+1 (or vsc_pipeline_sync), and the window frames of call t
+                                   must stay unchanged until then.  Same results as 0. */
 } vsc_pipeline_cfg;
 
 /* The k = 1 frame window (device RGBA8 frames, 4-byte aligned). */
@@ -412,6 +417,8 @@ int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long long frame, co
                        uint8_t* out_rgba, float* out_float, vsc_hip_stream stream);
 /* The last frame's fallback gate (1 learned, 0 bilinear) and inconsistent count. */
 int vsc_pipeline_gate(const vsc_pipeline* p, float* gate, uint32_t* count, vsc_hip_stream stream);
+/* Order `stream` after every frame submitted so far (needed only with overlap = 1). */
+int vsc_pipeline_sync(vsc_pipeline* p, vsc_hip_stream stream);
 
 /* ---------------------------------------------------------------------------
  * Multi-stream frame server: many concurrent video streams of different

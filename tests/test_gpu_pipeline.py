@@ -101,3 +101,34 @@ def test_pipeline_errors_and_footprint(vsc, cuda):
     with pytest.raises(vsc.VscError):
         vsc.Pipeline(W, H, flow_dtype="f32", downscale=3)
     assert vsc.Pipeline.footprint(1920, 1080, "plain") > vsc.Pipeline.footprint(1920, 1080, "chosen") > 0
+
+
+@pytest.mark.parametrize("form", ["plain", "chosen"])
+def test_pipeline_overlap_matches_serial(vsc, cuda, form):
+    """Frame-pipelined execution (flows of frame t+1 on one internal stream
+    while frame t is stabilized on another) gives the serial pipeline's bits
+    on every frame, float outputs and the fallback gate included."""
+    import torch
+    n = 9
+    orig, proc = _video(vsc, n=n + 2, seed=13)
+    o = torch.from_numpy(orig).to(cuda)
+    p = torch.from_numpy(proc).to(cuda)
+    res = {}
+    for ov in (False, True):
+        pipe = vsc.Pipeline(W, H, form=form, seed=4, overlap=ov)
+        pipe.set_last_rgba(p[2])
+        outs = [torch.empty((H, W, 4), dtype=torch.uint8, device=cuda) for _ in range(n)]
+        flts = [torch.empty((H, W, 3), dtype=torch.float32, device=cuda) for _ in range(n)]
+        for t in range(1, n):
+            pipe.frame(_window(o, p, t), t, out_rgba=outs[t], out_float=flts[t])
+        pipe.sync()
+        gate = pipe.gate()
+        torch.cuda.synchronize()
+        res[ov] = ([x.cpu() for x in outs[1:]], [x.cpu() for x in flts[1:]], gate)
+        pipe.close()
+    for t in range(n - 1):
+        assert torch.equal(res[False][0][t], res[True][0][t]), f"frame {t + 1}"
+        assert torch.equal(res[False][1][t], res[True][1][t]), f"frame {t + 1} float"
+    assert res[False][2] == res[True][2]
+    if form == "plain":
+        assert res[True][2] == (0.0, 0)     # no learned upsampler: bilinear, nothing counted

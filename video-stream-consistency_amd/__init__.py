@@ -53,7 +53,7 @@ EXPORTED = (
     "vsc_upsampler_set_params", "vsc_upsampler_workspace_size", "vsc_upsampler_run", "vsc_upsampler_train_step",
     "vsc_upsampler_loss_grad", "vsc_flow_fb_gate", "vsc_pipeline_default_cfg", "vsc_pipeline_create",
     "vsc_pipeline_destroy", "vsc_pipeline_footprint", "vsc_pipeline_set_upsampler_params", "vsc_pipeline_flownet",
-    "vsc_pipeline_stream", "vsc_pipeline_flows", "vsc_pipeline_frame", "vsc_pipeline_gate",
+    "vsc_pipeline_stream", "vsc_pipeline_flows", "vsc_pipeline_frame", "vsc_pipeline_gate", "vsc_pipeline_sync",
     "vsc_server_default_cfg", "vsc_server_create", "vsc_server_destroy", "vsc_server_stream_footprint",
     "vsc_server_used_bytes", "vsc_server_open", "vsc_server_close", "vsc_server_set_params", "vsc_server_submit",
     "vsc_server_tick", "vsc_server_poll", "vsc_server_drain", "vsc_server_read", "vsc_flo_read", "vsc_flo_write",
@@ -80,7 +80,8 @@ class VscFrameIn(ctypes.Structure):
 class PipelineCfg(ctypes.Structure):
     _fields_ = [("flow_dtype", ctypes.c_int), ("downscale", ctypes.c_int), ("upsample", ctypes.c_int),
                 ("scale_vectors", ctypes.c_int), ("fb_max_inconsistent", ctypes.c_float),
-                ("ref_bwd_index", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("solver", ctypes.c_int)]
+                ("ref_bwd_index", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("solver", ctypes.c_int),
+                ("overlap", ctypes.c_int)]
 
 
 class Window(ctypes.Structure):
@@ -197,6 +198,7 @@ def lib():
         L.vsc_pipeline_frame.argtypes = [vp, ctypes.POINTER(Window), ctypes.c_longlong, ctypes.POINTER(VscParams),
                                          vp, vp, vp]
         L.vsc_pipeline_gate.argtypes = [vp, vp, vp, vp]
+        L.vsc_pipeline_sync.argtypes = [vp, vp]
         L.vsc_server_default_cfg.argtypes = [ctypes.POINTER(ServerCfg)]
         L.vsc_server_default_cfg.restype = None
         L.vsc_server_create.argtypes = [ctypes.POINTER(ServerCfg), vp, ctypes.POINTER(vp)]
@@ -736,7 +738,7 @@ class Pipeline:
 
     def __init__(self, width, height, form=None, flow_dtype="f32", downscale=1, upsample="bilinear",
                  scale_vectors=False, fb_max_inconsistent=0.25, ref_bwd_index=False, seed=0, upsampler_params=None,
-                 flow_head_scale=None, solver=None):
+                 flow_head_scale=None, solver=None, overlap=False):
         if form is not None:
             kw = self.FORMS[form]
             flow_dtype, downscale, upsample = kw["flow_dtype"], kw["downscale"], kw["upsample"]
@@ -751,6 +753,7 @@ class Pipeline:
         c.ref_bwd_index = int(bool(ref_bwd_index))
         c.seed = seed
         c.solver = SOLVERS[solver or "exact"]
+        c.overlap = int(bool(overlap))
         self.cfg = c
         self.width, self.height = width, height
         h = ctypes.c_void_p()
@@ -807,6 +810,11 @@ class Pipeline:
         check(lib().vsc_pipeline_frame(self.h, ctypes.byref(self._window(window)), frame, ctypes.byref(prm),
                                        _ptr(out_rgba), None if out_float is None else _ptr(out_float), _stream()))
         return out_rgba
+
+    def sync(self):
+        """Order torch's current stream after every submitted frame (overlap mode:
+        frame t's outputs are otherwise ordered only by the call for frame t+1)."""
+        check(lib().vsc_pipeline_sync(self.h, _stream()))
 
     def gate(self):
         torch = _torch()

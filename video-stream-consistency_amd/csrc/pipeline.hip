@@ -9,7 +9,10 @@
 //      unscaled, as flowmodel.cpp:158-161, or scaled by the factor), or the
 //      learned upsampler gated per frame by the forward-backward check,
 //   4. the fused stabilization of vsc_stab_frame.
-// All buffers are allocated at creation; a frame allocates nothing.
+// All buffers are allocated at creation; a frame allocates nothing.  The
+// flows of frame t live in buffer set t & 1, so that with overlap the flows
+// of frame t+1 (internal stream sf) are computed while frame t is stabilized
+// (internal stream ss).
 #include "vsc_internal.h"
 
 #include <cmath>
@@ -21,13 +24,18 @@ struct vsc_pipeline {
     vsc_flownet* net = nullptr;
     vsc_upsampler* ups = nullptr;
     vsc_stream* stab = nullptr;
-    float* lo[3] = {nullptr, nullptr, nullptr};   // low-res flows: fwd, bwd, previous fwd (HWC 2)
-    float* hi[2] = {nullptr, nullptr};            // full-res fwd, bwd (HWC 2) when upsampling here
+    float* lo[2][2] = {};                          // [frame & 1][fwd, bwd] flows at the flow resolution (HWC 2)
+    float* hi[2][2] = {};                          // [frame & 1][fwd, bwd] at frame size when upsampling here
     void* ws = nullptr;                            // upsampler workspace
     uint32_t* count = nullptr;
     float* gate = nullptr;
     long long slot_frame[3] = {-1, -1, -1};        // frame index held by each feature slot
-    long long prev_fwd_frame = -2;                 // frame whose fwd flow lo[2] holds
+    long long prev_fwd_frame = -2;                 // last frame whose flows were computed
+    // overlap: flows on sf, stabilization on ss; ev_s[k] = the stabilization
+    // that last read buffer set k, ev_f = the last flow work
+    hipStream_t sf = nullptr, ss = nullptr;
+    hipEvent_t ev_in = nullptr, ev_f = nullptr, ev_s[2] = {nullptr, nullptr};
+    bool f_rec = false, s_rec[2] = {false, false};
 };
 
 namespace vsc {
@@ -58,6 +66,24 @@ int dalloc(void** p, size_t bytes)
 bool learned(const vsc_pipeline_cfg& c) { return c.downscale > 1 && c.upsample == 1; }
 bool upsample_here(const vsc_pipeline_cfg& c) { return c.downscale > 1 && (c.upsample == 1 || c.scale_vectors); }
 
+int parity(long long frame) { return (int)(((frame % 2) + 2) % 2); }
+
+bool capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+// overlap mode: make `s` wait for all outstanding work of the pipeline
+int join_all(vsc_pipeline* p, hipStream_t s)
+{
+    if (!p->cfg.overlap) return VSC_OK;
+    if (p->f_rec && check_hip(hipStreamWaitEvent(s, p->ev_f, 0), "vsc_pipeline join")) return VSC_E_HIP;
+    for (int k = 0; k < 2; ++k)
+        if (p->s_rec[k] && check_hip(hipStreamWaitEvent(s, p->ev_s[k], 0), "vsc_pipeline join")) return VSC_E_HIP;
+    return VSC_OK;
+}
+
 }  // namespace
 }  // namespace vsc
 
@@ -85,6 +111,7 @@ static int cfg_ok(int W, int H, const vsc_pipeline_cfg* c)
         return set_error(VSC_E_INVALID, "vsc_pipeline: downscale must be 1, 2 or 4");
     if (c->upsample != 0 && c->upsample != 1) return set_error(VSC_E_INVALID, "vsc_pipeline: upsample must be 0 or 1");
     if (c->solver != 0 && c->solver != 1) return set_error(VSC_E_INVALID, "vsc_pipeline: solver must be 0 or 1");
+    if (c->overlap != 0 && c->overlap != 1) return set_error(VSC_E_INVALID, "vsc_pipeline: overlap must be 0 or 1");
     return VSC_OK;
 }
 
@@ -95,8 +122,8 @@ extern "C" size_t vsc_pipeline_footprint(int width, int height, const vsc_pipeli
     size_t t = vsc_flownet_footprint(width, height, cfg->downscale, cfg->flow_dtype, 3);
     const bool here = upsample_here(*cfg);
     t += vsc_stream_footprint(width, height, here ? width : Wd, here ? height : Hd);
-    t += 3 * (size_t)Wd * Hd * 2 * 4;
-    if (here) t += 2 * (size_t)width * height * 2 * 4;
+    t += 4 * (size_t)Wd * Hd * 2 * 4;
+    if (here) t += 4 * (size_t)width * height * 2 * 4;
     if (learned(*cfg)) t += vsc_upsampler_workspace_size(width, height) + 4 * 3202 * 4 + 512;
     return t + 64;
 }
@@ -121,11 +148,25 @@ extern "C" int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg
         return rc;
     }
     const size_t lo_b = (size_t)p->Wd * p->Hd * 2 * 4, hi_b = (size_t)width * height * 2 * 4;
-    for (int i = 0; i < 3 && !rc; ++i) rc = dalloc(reinterpret_cast<void**>(&p->lo[i]), lo_b);
+    for (int i = 0; i < 4 && !rc; ++i) rc = dalloc(reinterpret_cast<void**>(&p->lo[i / 2][i % 2]), lo_b);
     if (here)
-        for (int i = 0; i < 2 && !rc; ++i) rc = dalloc(reinterpret_cast<void**>(&p->hi[i]), hi_b);
+        for (int i = 0; i < 4 && !rc; ++i) rc = dalloc(reinterpret_cast<void**>(&p->hi[i / 2][i % 2]), hi_b);
+    if (!rc && cfg->overlap) {
+        // default-priority streams; events without timing
+        if (check_hip(hipStreamCreateWithFlags(&p->sf, hipStreamNonBlocking), "vsc_pipeline_create") ||
+            check_hip(hipStreamCreateWithFlags(&p->ss, hipStreamNonBlocking), "vsc_pipeline_create") ||
+            check_hip(hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming), "vsc_pipeline_create") ||
+            check_hip(hipEventCreateWithFlags(&p->ev_f, hipEventDisableTiming), "vsc_pipeline_create") ||
+            check_hip(hipEventCreateWithFlags(&p->ev_s[0], hipEventDisableTiming), "vsc_pipeline_create") ||
+            check_hip(hipEventCreateWithFlags(&p->ev_s[1], hipEventDisableTiming), "vsc_pipeline_create"))
+            rc = VSC_E_HIP;
+    }
     if (!rc) rc = dalloc(reinterpret_cast<void**>(&p->count), 256);
     if (!rc) rc = dalloc(reinterpret_cast<void**>(&p->gate), 256);
+    // forms without the learned upsampler report gate 0 (bilinear) and count 0
+    if (!rc && (check_hip(hipMemset(p->count, 0, 256), "vsc_pipeline_create") ||
+                check_hip(hipMemset(p->gate, 0, 256), "vsc_pipeline_create")))
+        rc = VSC_E_HIP;
     if (!rc && learned(*cfg)) {
         rc = vsc_upsampler_create(cfg->seed, &p->ups);
         if (!rc) rc = dalloc(&p->ws, vsc_upsampler_workspace_size(width, height));
@@ -141,11 +182,18 @@ extern "C" int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg
 extern "C" int vsc_pipeline_destroy(vsc_pipeline* p)
 {
     if (!p) return VSC_OK;
+    for (hipStream_t h : {p->sf, p->ss})
+        if (h) (void)hipStreamSynchronize(h);
     if (p->net) vsc_flownet_destroy(p->net);
     if (p->ups) vsc_upsampler_destroy(p->ups);
     if (p->stab) vsc_stream_destroy(p->stab);
-    void* bufs[] = {p->lo[0], p->lo[1], p->lo[2], p->hi[0], p->hi[1], p->ws, p->count, p->gate};
+    void* bufs[] = {p->lo[0][0], p->lo[0][1], p->lo[1][0], p->lo[1][1], p->hi[0][0], p->hi[0][1],
+                    p->hi[1][0], p->hi[1][1], p->ws, p->count, p->gate};
     for (void* b : bufs) (void)hipFree(b);
+    for (hipEvent_t e : {p->ev_in, p->ev_f, p->ev_s[0], p->ev_s[1]})
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t h : {p->sf, p->ss})
+        if (h) (void)hipStreamDestroy(h);
     delete p;
     return VSC_OK;
 }
@@ -181,12 +229,9 @@ static int ensure_features(vsc_pipeline* p, const uint8_t* rgba, long long frame
     return VSC_OK;
 }
 
-extern "C" int vsc_pipeline_flows(vsc_pipeline* p, const vsc_window* w, long long frame, float* fwd_out,
-                                  float* bwd_out, vsc_hip_stream stream)
+static int flows_on(vsc_pipeline* p, const vsc_window* w, long long frame, float* fwd_out, float* bwd_out,
+                    hipStream_t s)
 {
-    if (!p || !w || !w->orig_prev || !w->orig_cur || !w->orig_next)
-        return set_error(VSC_E_INVALID, "vsc_pipeline_flows: NULL argument");
-    hipStream_t s = as_stream(stream);
     int rc;
     if ((rc = ensure_features(p, w->orig_prev, frame - 1, s)) || (rc = ensure_features(p, w->orig_cur, frame, s)) ||
         (rc = ensure_features(p, w->orig_next, frame + 1, s)))
@@ -195,69 +240,112 @@ extern "C" int vsc_pipeline_flows(vsc_pipeline* p, const vsc_window* w, long lon
     // forward cur -> next; backward cur -> prev (or next -> cur as videostabilizer.cpp:271-272)
     const int first[2] = {sc, p->cfg.ref_bwd_index ? sn : sc};
     const int second[2] = {sn, p->cfg.ref_bwd_index ? sc : sp};
-    float* outs[2] = {fwd_out ? fwd_out : p->lo[0], bwd_out ? bwd_out : p->lo[1]};
-    return vsc_flownet_flow(p->net, 2, first, second, outs, 2, stream);
+    float* outs[2] = {fwd_out, bwd_out};
+    return vsc_flownet_flow(p->net, 2, first, second, outs, 2, reinterpret_cast<vsc_hip_stream>(s));
+}
+
+extern "C" int vsc_pipeline_flows(vsc_pipeline* p, const vsc_window* w, long long frame, float* fwd_out,
+                                  float* bwd_out, vsc_hip_stream stream)
+{
+    if (!p || !w || !w->orig_prev || !w->orig_cur || !w->orig_next)
+        return set_error(VSC_E_INVALID, "vsc_pipeline_flows: NULL argument");
+    hipStream_t s = as_stream(stream);
+    int rc = join_all(p, s);   // the flow network's buffers may still be in use by frame work
+    if (rc) return rc;
+    const int k = parity(frame);
+    return flows_on(p, w, frame, fwd_out ? fwd_out : p->lo[k][0], bwd_out ? bwd_out : p->lo[k][1], s);
 }
 
 extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long long frame, const vsc_params* params,
                                   uint8_t* out_rgba, float* out_float, vsc_hip_stream stream)
 {
-    if (!p || !w) return set_error(VSC_E_INVALID, "vsc_pipeline_frame: NULL argument");
-    hipStream_t s = as_stream(stream);
+    if (!p || !w || !w->orig_prev || !w->orig_cur || !w->orig_next)
+        return set_error(VSC_E_INVALID, "vsc_pipeline_frame: NULL argument");
+    hipStream_t cs = as_stream(stream);
+    const bool ov = p->cfg.overlap && !capturing(cs);
+    hipStream_t fs = ov ? p->sf : cs, ss = ov ? p->ss : cs;
     const int f = p->cfg.downscale;
+    const int k = parity(frame);
+    int rc;
+    if (ov) {
+        // flows wait for the caller's inputs and for the stabilization that
+        // last read buffer set k (frame - 2)
+        if (check_hip(hipEventRecord(p->ev_in, cs), "vsc_pipeline_frame") ||
+            check_hip(hipStreamWaitEvent(fs, p->ev_in, 0), "vsc_pipeline_frame") ||
+            (p->s_rec[k] && check_hip(hipStreamWaitEvent(fs, p->ev_s[k], 0), "vsc_pipeline_frame")))
+            return VSC_E_HIP;
+    }
+    if ((rc = flows_on(p, w, frame, p->lo[k][0], p->lo[k][1], fs))) return rc;
+    const vsc_hip_stream fsv = reinterpret_cast<vsc_hip_stream>(fs);
     // previous forward flow for the consistency check of this frame's backward flow
     const bool have_prev = p->prev_fwd_frame == frame - 1;
-    if (have_prev) {
-        float* t = p->lo[2];
-        p->lo[2] = p->lo[0];
-        p->lo[0] = t;
-    }
-    int rc = vsc_pipeline_flows(p, w, frame, p->lo[0], p->lo[1], stream);
-    if (rc) return rc;
-    const float* ff = p->lo[0];
-    const float* fb = p->lo[1];
+    const float* ff = p->lo[k][0];
+    const float* fb = p->lo[k][1];
     if (learned(p->cfg)) {
         if (p->cfg.fb_max_inconsistent < 0.0f || (!have_prev && !p->cfg.ref_bwd_index)) {
-            hipLaunchKernelGGL(k_set_gate, dim3(1), dim3(64), 0, s, p->gate, 1.0f);
+            hipLaunchKernelGGL(k_set_gate, dim3(1), dim3(64), 0, fs, p->gate, 1.0f);
             if ((rc = check_launch("pipeline gate"))) return rc;
         } else if (p->cfg.ref_bwd_index) {
             // fwd: cur -> next, bwd: next -> cur are each other's inverse
-            if ((rc = vsc_flow_fb_gate(p->lo[0], p->lo[1], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent, p->count,
-                                       p->gate, stream)))
+            if ((rc = vsc_flow_fb_gate(p->lo[k][0], p->lo[k][1], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent, p->count,
+                                       p->gate, fsv)))
                 return rc;
         } else {
             // bwd: cur -> prev against the previous frame's fwd: prev -> cur
-            if ((rc = vsc_flow_fb_gate(p->lo[1], p->lo[2], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent, p->count,
-                                       p->gate, stream)))
+            if ((rc = vsc_flow_fb_gate(p->lo[k][1], p->lo[1 - k][0], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent,
+                                       p->count, p->gate, fsv)))
                 return rc;
         }
         // the learned upsampler is trained on full-resolution vectors: always scaled
         const float sc = (float)f;
-        if ((rc = vsc_upsampler_run(p->ups, p->lo[0], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[0], 2,
-                                    p->gate, p->ws, stream)) ||
-            (rc = vsc_upsampler_run(p->ups, p->lo[1], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[1], 2,
-                                    p->gate, p->ws, stream)))
+        if ((rc = vsc_upsampler_run(p->ups, p->lo[k][0], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[k][0], 2,
+                                    p->gate, p->ws, fsv)) ||
+            (rc = vsc_upsampler_run(p->ups, p->lo[k][1], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[k][1], 2,
+                                    p->gate, p->ws, fsv)))
             return rc;
-        ff = p->hi[0];
-        fb = p->hi[1];
+        ff = p->hi[k][0];
+        fb = p->hi[k][1];
     } else if (upsample_here(p->cfg)) {
         dim3 grid((p->W + 255) / 256, p->H);
-        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, s, p->lo[0], p->Wd, p->Hd, p->hi[0], p->W, p->H, (float)f);
-        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, s, p->lo[1], p->Wd, p->Hd, p->hi[1], p->W, p->H, (float)f);
+        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, fs, p->lo[k][0], p->Wd, p->Hd, p->hi[k][0], p->W, p->H,
+                           (float)f);
+        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, fs, p->lo[k][1], p->Wd, p->Hd, p->hi[k][1], p->W, p->H,
+                           (float)f);
         if ((rc = check_launch("pipeline flow upscale"))) return rc;
-        ff = p->hi[0];
-        fb = p->hi[1];
+        ff = p->hi[k][0];
+        fb = p->hi[k][1];
     }
     p->prev_fwd_frame = frame;
+    if (ov) {
+        if (check_hip(hipEventRecord(p->ev_f, fs), "vsc_pipeline_frame") ||
+            check_hip(hipStreamWaitEvent(ss, p->ev_f, 0), "vsc_pipeline_frame"))
+            return VSC_E_HIP;
+        p->f_rec = true;
+    }
     vsc_frame_in in{w->orig_prev, w->orig_cur, w->orig_next, w->proc_prev, w->proc_cur, w->proc_next, ff, fb, 2};
-    return vsc_stab_frame(p->stab, &in, params, out_rgba, out_float, p->cfg.solver ? VSC_FRAME_FMA : VSC_FRAME_FUSED,
-                          stream);
+    rc = vsc_stab_frame(p->stab, &in, params, out_rgba, out_float, p->cfg.solver ? VSC_FRAME_FMA : VSC_FRAME_FUSED,
+                        reinterpret_cast<vsc_hip_stream>(ss));
+    if (rc || !ov) return rc;
+    if (check_hip(hipEventRecord(p->ev_s[k], ss), "vsc_pipeline_frame")) return VSC_E_HIP;
+    p->s_rec[k] = true;
+    // the caller's stream is ordered after the previous frame (frame - 1)
+    if (p->s_rec[1 - k] && check_hip(hipStreamWaitEvent(cs, p->ev_s[1 - k], 0), "vsc_pipeline_frame"))
+        return VSC_E_HIP;
+    return VSC_OK;
+}
+
+extern "C" int vsc_pipeline_sync(vsc_pipeline* p, vsc_hip_stream stream)
+{
+    if (!p) return set_error(VSC_E_INVALID, "vsc_pipeline_sync: NULL argument");
+    return join_all(p, as_stream(stream));
 }
 
 extern "C" int vsc_pipeline_gate(const vsc_pipeline* p, float* gate_dev, uint32_t* count_dev, vsc_hip_stream stream)
 {
     if (!p) return set_error(VSC_E_INVALID, "vsc_pipeline_gate: NULL argument");
     hipStream_t s = as_stream(stream);
+    if (p->cfg.overlap && p->f_rec && check_hip(hipStreamWaitEvent(s, p->ev_f, 0), "vsc_pipeline_gate"))
+        return VSC_E_HIP;
     if (gate_dev && check_hip(hipMemcpyAsync(gate_dev, p->gate, 4, hipMemcpyDeviceToDevice, s), "vsc_pipeline_gate"))
         return VSC_E_HIP;
     if (count_dev && check_hip(hipMemcpyAsync(count_dev, p->count, 4, hipMemcpyDeviceToDevice, s), "vsc_pipeline_gate"))
@@ -276,9 +364,6 @@ int pipeline_advance(vsc_pipeline* p, long long frame)
     const int sp = (int)(((frame - 1) % 3 + 3) % 3), sc = (int)((frame % 3 + 3) % 3), sn = (int)(((frame + 1) % 3 + 3) % 3);
     if (p->prev_fwd_frame != frame - 1 || p->slot_frame[sp] != frame - 1 || p->slot_frame[sc] != frame)
         return set_error(VSC_E_INVALID, "pipeline_advance: not in steady state");
-    float* t = p->lo[2];
-    p->lo[2] = p->lo[0];
-    p->lo[0] = t;
     p->slot_frame[sn] = frame + 1;
     p->prev_fwd_frame = frame;
     return VSC_OK;
