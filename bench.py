@@ -288,10 +288,14 @@ def bench_pipeline(args):
         r = [(t - 1) % RING, t % RING, (t + 1) % RING]
         return (O[r[0]], O[r[1]], O[r[2]], P[r[0]], P[r[1]], P[r[2]])
 
-    def run(pipe, steps, warmup, t_start):
+    def run(pipe, steps, warmup, t_start, prof=False):
         for i in range(warmup):
             pipe.frame(window(t_start + i), t_start + i, out_rgba=out)
         torch.cuda.synchronize()
+        if prof:
+            # the library brackets every solver and front-pass launch with HIP
+            # events on the stream it launches on
+            vsc.profile_begin(64 * steps)
         t0 = time.perf_counter()
         for i in range(steps):
             t = t_start + warmup + i
@@ -306,17 +310,19 @@ def bench_pipeline(args):
     # while frame t is stabilized; the timed region ends with a device sync)
     chosen = vsc.Pipeline(W, H, form="chosen", seed=1, flow_head_scale=FLOW_HEAD_SCALE, overlap=True)
     chosen.set_last_rgba(P[2])
-    el_c, _ = run(chosen, args.steps, args.warmup, 1)
+    el_c, _ = run(chosen, args.steps, args.warmup, 1, prof=True)
+    n_solver, ms_solver, by_solver = vsc.profile_end("consist_blocked")
+    n_front, ms_front, by_front = vsc.profile_end("front_pass")
+    gate, count = chosen.gate()
     chosen.close()
     fps = args.steps / el_c
-    # serial pipeline (one stream): per-frame latency, and the solver launches
-    # timed by the library with HIP events on their stream
+    # serial pipeline (one stream): throughput without frame pipelining and the
+    # latency of one frame (HIP events on its stream)
     chosen = vsc.Pipeline(W, H, form="chosen", seed=1, flow_head_scale=FLOW_HEAD_SCALE)
     chosen.set_last_rgba(P[2])
     el_s, t_next = run(chosen, max(10, args.steps // 3), max(3, args.warmup // 4), 1)
     fps_serial = max(10, args.steps // 3) / el_s
     nprof = min(args.steps, 100)
-    vsc.profile_begin(64 * nprof)
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(nprof)]
     for i in range(nprof):
         t = t_next + i
@@ -324,10 +330,7 @@ def bench_pipeline(args):
         chosen.frame(window(t), t, out_rgba=out)
         evs[i][1].record()
     torch.cuda.synchronize()
-    n_solver, ms_solver, by_solver = vsc.profile_end("consist_blocked")
-    n_front, ms_front, by_front = vsc.profile_end("front_pass")
     lat = np.array([a.elapsed_time(b) for a, b in evs])
-    gate, count = chosen.gate()
     chosen.close()
 
     plain_steps = args.plain_steps or max(10, args.steps // 3)
@@ -392,14 +395,14 @@ def bench_pipeline(args):
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile("k_consist_blocked"),
             "launches": n_solver, "avg_launch_us": round(1000.0 * ms_solver / n_solver, 3),
             "algorithmic_bytes_per_launch": round(by_solver / n_solver),
-            "share_of_frame": round(ms_solver / float(lat.sum()), 4),
+            "share_of_timed_region": round(ms_solver / (1000.0 * el_c), 4),
             # the solver's algorithmic arithmetic (the reference formula's 14 FP32
             # operations per pixel, channel and sweep; the FMA form issues 7
             # instructions for them): its rate against the FP32 vector peak,
             # beside the HBM figure
-            "valu": {"algorithmic_tflops": round(SOLVER_FLOPS_PER_FRAME * nprof / (ms_solver * 1e-3) / 1e12, 2),
+            "valu": {"algorithmic_tflops": round(SOLVER_FLOPS_PER_FRAME * args.steps / (ms_solver * 1e-3) / 1e12, 2),
                      "peak_tflops": 157.3,
-                     "frac": round(SOLVER_FLOPS_PER_FRAME * nprof / (ms_solver * 1e-3) / 157.3e12, 4)},
+                     "frac": round(SOLVER_FLOPS_PER_FRAME * args.steps / (ms_solver * 1e-3) / 157.3e12, 4)},
         }
     if n_front:
         ach = by_front / (ms_front * 1e-3) / 1e9
