@@ -6,9 +6,9 @@
 // LeakyReLU(0.1) / ReLU and a residual add fused into the epilogue.
 //
 // GEMM form: D[cout][pixel] = sum_{tap, cin} W[cout][tap][cin] * X[tap(pixel)][cin].
-// A block owns TH = 4*MR output rows x 16 output columns and NT output
-// channels; wave q computes rows q*MR .. q*MR+MR-1, i.e. MR pixel tiles of 16
-// against NT/16 channel tiles.  Input channels stream through LDS in chunks of
+// A block of NW waves owns TH = NW*MR output rows x 16 output columns and NT
+// output channels; wave q computes rows q*MR .. q*MR+MR-1, i.e. MR pixel tiles
+// of 16 against NT/16 channel tiles.  Input channels stream through LDS in chunks of
 // KC (32 for 16-bit, 16 for FP32): the input patch of the block (TH+2 x 18
 // pixels at stride 1, 2*TH+1 x 33 at stride 2) and the weights of the chunk
 // for all nine taps; the nine taps then read shifted windows of the patch.
@@ -97,17 +97,18 @@ __device__ inline void conv_load4(const void* src, long long off, float v[4], in
     for (int r = 0; r < 4; ++r) v[r] += (r < nvalid) ? E::ld(p[r]) : 0.0f;
 }
 
-template <int EK, int EKO, int S, int NT, int MR>
-__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
+template <int EK, int EKO, int S, int NT, int MR, int NW>
+__global__ __launch_bounds__(64 * NW) void k_conv3x3(ConvArgs a)
 {
     using E = Elem<EK>;
     using T = typename E::T;
+    constexpr int NTH = 64 * NW;
     constexpr int VEC = E::VEC, KC = ConvK<EK>::KC, KV = KC / VEC;
-    constexpr int TH = 4 * MR, TW = 16;
+    constexpr int TH = NW * MR, TW = 16;
     constexpr int PR = (TH - 1) * S + 3, PC = (TW - 1) * S + 3;
     constexpr int ROW = KC + VEC;                     // LDS pixel stride (elements): 80 bytes
     constexpr int NP = PR * PC * KV, NWU = NT * 9 * KV;
-    constexpr int RP = (NP + 255) / 256, RW = (NWU + 255) / 256;
+    constexpr int RP = (NP + NTH - 1) / NTH, RW = (NWU + NTH - 1) / NTH;
     __shared__ __attribute__((aligned(16))) T sP[PR * PC * ROW];
     __shared__ __attribute__((aligned(16))) T sW[NT * 9 * ROW];
 
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
     auto fetch = [&](int c0) {
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
-            const int u = tid + 256 * k;
+            const int u = tid + NTH * k;
             const int p = u / KV, cv = u % KV;
             const int r = p / PC, c = p % PC;
             const int iy = base_y + ps * r, ix = base_x + ps * c, ch = c0 + cv * VEC;
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
         }
 #pragma unroll
         for (int k = 0; k < RW; ++k) {
-            const int u = tid + 256 * k;
+            const int u = tid + NTH * k;
             const int row = u / KV, cv = u % KV;           // row = co_local*9 + tap
             const int co = nt0 + row / 9, ch = c0 + cv * VEC;
             cv_u32x4 v = cv_u32x4{0u, 0u, 0u, 0u};
@@ -159,12 +160,12 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
-            const int u = tid + 256 * k;
+            const int u = tid + NTH * k;
             if (u < NP) *reinterpret_cast<cv_u32x4*>(&sP[(u / KV) * ROW + (u % KV) * VEC]) = rp[k];
         }
 #pragma unroll
         for (int k = 0; k < RW; ++k) {
-            const int u = tid + 256 * k;
+            const int u = tid + NTH * k;
             if (u < NWU) *reinterpret_cast<cv_u32x4*>(&sW[(u / KV) * ROW + (u % KV) * VEC]) = rw[k];
         }
         __syncthreads();
@@ -240,14 +241,14 @@ __global__ void k_conv_finalize(ConvArgs a, int parts)
     static_cast<typename E::T*>(a.out)[pix * a.out_cs + a.out_c0 + co] = E::st(x);
 }
 
-template <int EK, int EKO, int S, int NT, int MR>
+template <int EK, int EKO, int S, int NT, int MR, int NW>
 static void conv_go(const ConvArgs& a, hipStream_t s)
 {
     constexpr int KC = ConvK<EK>::KC;
     const int ps = a.pstep;
     ConvArgs b = a;
     const int hs = (a.Ho + ps - 1) / ps, ws = (a.Wo + ps - 1) / ps;
-    b.tiles_y = (hs + 4 * MR - 1) / (4 * MR);
+    b.tiles_y = (hs + NW * MR - 1) / (NW * MR);
     b.tiles_x = (ws + 15) / 16;
     b.ctiles = (a.Cout + NT - 1) / NT;
     const int chunks = (a.Cin + KC - 1) / KC;
@@ -269,13 +270,16 @@ static void conv_go(const ConvArgs& a, hipStream_t s)
     b.partial = parts > 1 ? a.partial : nullptr;
     b.part_stride = stride;
     dim3 grid(b.tiles_x * b.tiles_y * ps * ps, b.ctiles * parts, a.N);
-    hipLaunchKernelGGL((k_conv3x3<EK, EKO, S, NT, MR>), grid, dim3(256), 0, s, b);
+    hipLaunchKernelGGL((k_conv3x3<EK, EKO, S, NT, MR, NW>), grid, dim3(64 * NW), 0, s, b);
     if (parts > 1)
         hipLaunchKernelGGL(k_conv_finalize<EKO>, dim3((unsigned)((stride + 255) / 256)), dim3(256), 0, s, b, parts);
 }
 
 // tile choice: the widest channel tile that the layer fills, and the tallest
 // pixel tile that still gives about two blocks per CU
+#ifndef VSC_CONV_MODE
+#define VSC_CONV_MODE 0
+#endif
 template <int EK, int EKO, int S>
 static void conv_pick(const ConvArgs& a, hipStream_t s)
 {
@@ -284,12 +288,17 @@ static void conv_pick(const ConvArgs& a, hipStream_t s)
     const int NTsel = a.Cout > 32 ? 64 : (a.Cout > 16 ? 32 : 16);
     const long long ntiles = (a.Cout + NTsel - 1) / NTsel;
     auto blocks = [&](int mr) { return (px / (64LL * mr)) * ntiles; };
-    int mr = (S == 1 && blocks(4) >= 512) ? 4 : (blocks(2) >= 512 ? 2 : 1);
+    const bool wide = VSC_CONV_MODE != 0 && S == 1 && EK != EK_F32;
+    int mr;
+    if (wide) mr = (VSC_CONV_MODE == 1 && blocks(4) >= 512) ? 2 : 1;   // 8 waves: 128 or 256 pixels
+    else mr = (S == 1 && blocks(4) >= 512) ? 4 : (blocks(2) >= 512 ? 2 : 1);
     if (S == 2 && mr > 2) mr = 2;
-#define VSC_CONV_NT(NTV)                                         \
-    if (mr == 4) conv_go<EK, EKO, S, NTV, (S == 1 ? 4 : 2)>(a, s); \
-    else if (mr == 2) conv_go<EK, EKO, S, NTV, 2>(a, s);         \
-    else conv_go<EK, EKO, S, NTV, 1>(a, s);
+#define VSC_CONV_NT(NTV)                                              \
+    if (wide && mr == 2) conv_go<EK, EKO, S, NTV, 2, 8>(a, s);        \
+    else if (wide) conv_go<EK, EKO, S, NTV, 1, 8>(a, s);              \
+    else if (mr == 4) conv_go<EK, EKO, S, NTV, (S == 1 ? 4 : 2), 4>(a, s); \
+    else if (mr == 2) conv_go<EK, EKO, S, NTV, 2, 4>(a, s);           \
+    else conv_go<EK, EKO, S, NTV, 1, 4>(a, s);
     if (NTsel == 64) { VSC_CONV_NT(64) }
     else if (NTsel == 32) { VSC_CONV_NT(32) }
     else { VSC_CONV_NT(16) }

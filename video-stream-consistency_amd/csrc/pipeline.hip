@@ -10,8 +10,9 @@
 //      learned upsampler gated per frame by the forward-backward check,
 //   4. the fused stabilization of vsc_stab_frame.
 // All buffers are allocated at creation; a frame allocates nothing.  The
-// flows of frame t live in buffer set t & 1, so that with overlap the flows
-// of frame t+1 (internal stream sf) are computed while frame t is stabilized
+// flows of frame t (and its fallback gate) live in buffer set t & 1, so that
+// with overlap the flows of frame t+1 (internal stream sf: features, decode,
+// consistency gate) are computed while frame t is upsampled and stabilized
 // (internal stream ss).
 #include "vsc_internal.h"
 
@@ -27,8 +28,8 @@ struct vsc_pipeline {
     float* lo[2][2] = {};                          // [frame & 1][fwd, bwd] flows at the flow resolution (HWC 2)
     float* hi[2][2] = {};                          // [frame & 1][fwd, bwd] at frame size when upsampling here
     void* ws = nullptr;                            // upsampler workspace
-    uint32_t* count = nullptr;
-    float* gate = nullptr;
+    uint32_t* count = nullptr;                     // [frame & 1] inconsistent-pixel counts (64 B apart)
+    float* gate = nullptr;                         // [frame & 1] fallback gates (64 B apart)
     long long slot_frame[3] = {-1, -1, -1};        // frame index held by each feature slot
     long long prev_fwd_frame = -2;                 // last frame whose flows were computed
     // overlap: flows on sf, stabilization on ss; ev_s[k] = the stabilization
@@ -36,6 +37,7 @@ struct vsc_pipeline {
     hipStream_t sf = nullptr, ss = nullptr;
     hipEvent_t ev_in = nullptr, ev_f = nullptr, ev_s[2] = {nullptr, nullptr};
     bool f_rec = false, s_rec[2] = {false, false};
+    int last_parity = 0;                           // buffer set of the last frame
 };
 
 namespace vsc {
@@ -279,48 +281,56 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
     const vsc_hip_stream fsv = reinterpret_cast<vsc_hip_stream>(fs);
     // previous forward flow for the consistency check of this frame's backward flow
     const bool have_prev = p->prev_fwd_frame == frame - 1;
-    const float* ff = p->lo[k][0];
-    const float* fb = p->lo[k][1];
+    float* gate = p->gate + 16 * k;
+    uint32_t* count = p->count + 16 * k;
     if (learned(p->cfg)) {
         if (p->cfg.fb_max_inconsistent < 0.0f || (!have_prev && !p->cfg.ref_bwd_index)) {
-            hipLaunchKernelGGL(k_set_gate, dim3(1), dim3(64), 0, fs, p->gate, 1.0f);
+            hipLaunchKernelGGL(k_set_gate, dim3(1), dim3(64), 0, fs, gate, 1.0f);
             if ((rc = check_launch("pipeline gate"))) return rc;
         } else if (p->cfg.ref_bwd_index) {
             // fwd: cur -> next, bwd: next -> cur are each other's inverse
-            if ((rc = vsc_flow_fb_gate(p->lo[k][0], p->lo[k][1], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent, p->count,
-                                       p->gate, fsv)))
+            if ((rc = vsc_flow_fb_gate(p->lo[k][0], p->lo[k][1], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent, count,
+                                       gate, fsv)))
                 return rc;
         } else {
             // bwd: cur -> prev against the previous frame's fwd: prev -> cur
             if ((rc = vsc_flow_fb_gate(p->lo[k][1], p->lo[1 - k][0], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent,
-                                       p->count, p->gate, fsv)))
+                                       count, gate, fsv)))
                 return rc;
         }
-        // the learned upsampler is trained on full-resolution vectors: always scaled
-        const float sc = (float)f;
-        if ((rc = vsc_upsampler_run(p->ups, p->lo[k][0], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[k][0], 2,
-                                    p->gate, p->ws, fsv)) ||
-            (rc = vsc_upsampler_run(p->ups, p->lo[k][1], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[k][1], 2,
-                                    p->gate, p->ws, fsv)))
-            return rc;
-        ff = p->hi[k][0];
-        fb = p->hi[k][1];
-    } else if (upsample_here(p->cfg)) {
-        dim3 grid((p->W + 255) / 256, p->H);
-        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, fs, p->lo[k][0], p->Wd, p->Hd, p->hi[k][0], p->W, p->H,
-                           (float)f);
-        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, fs, p->lo[k][1], p->Wd, p->Hd, p->hi[k][1], p->W, p->H,
-                           (float)f);
-        if ((rc = check_launch("pipeline flow upscale"))) return rc;
-        ff = p->hi[k][0];
-        fb = p->hi[k][1];
     }
     p->prev_fwd_frame = frame;
+    p->last_parity = k;
     if (ov) {
         if (check_hip(hipEventRecord(p->ev_f, fs), "vsc_pipeline_frame") ||
             check_hip(hipStreamWaitEvent(ss, p->ev_f, 0), "vsc_pipeline_frame"))
             return VSC_E_HIP;
         p->f_rec = true;
+    }
+    // upsampling to the frame, on the stabilization stream (it reads only this
+    // frame's buffer set)
+    const vsc_hip_stream ssv = reinterpret_cast<vsc_hip_stream>(ss);
+    const float* ff = p->lo[k][0];
+    const float* fb = p->lo[k][1];
+    if (learned(p->cfg)) {
+        // the learned upsampler is trained on full-resolution vectors: always scaled
+        const float sc = (float)f;
+        if ((rc = vsc_upsampler_run(p->ups, p->lo[k][0], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[k][0], 2,
+                                    gate, p->ws, ssv)) ||
+            (rc = vsc_upsampler_run(p->ups, p->lo[k][1], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[k][1], 2,
+                                    gate, p->ws, ssv)))
+            return rc;
+        ff = p->hi[k][0];
+        fb = p->hi[k][1];
+    } else if (upsample_here(p->cfg)) {
+        dim3 grid((p->W + 255) / 256, p->H);
+        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, ss, p->lo[k][0], p->Wd, p->Hd, p->hi[k][0], p->W, p->H,
+                           (float)f);
+        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, ss, p->lo[k][1], p->Wd, p->Hd, p->hi[k][1], p->W, p->H,
+                           (float)f);
+        if ((rc = check_launch("pipeline flow upscale"))) return rc;
+        ff = p->hi[k][0];
+        fb = p->hi[k][1];
     }
     vsc_frame_in in{w->orig_prev, w->orig_cur, w->orig_next, w->proc_prev, w->proc_cur, w->proc_next, ff, fb, 2};
     rc = vsc_stab_frame(p->stab, &in, params, out_rgba, out_float, p->cfg.solver ? VSC_FRAME_FMA : VSC_FRAME_FUSED,
@@ -346,9 +356,12 @@ extern "C" int vsc_pipeline_gate(const vsc_pipeline* p, float* gate_dev, uint32_
     hipStream_t s = as_stream(stream);
     if (p->cfg.overlap && p->f_rec && check_hip(hipStreamWaitEvent(s, p->ev_f, 0), "vsc_pipeline_gate"))
         return VSC_E_HIP;
-    if (gate_dev && check_hip(hipMemcpyAsync(gate_dev, p->gate, 4, hipMemcpyDeviceToDevice, s), "vsc_pipeline_gate"))
+    const int k = p->last_parity;
+    if (gate_dev &&
+        check_hip(hipMemcpyAsync(gate_dev, p->gate + 16 * k, 4, hipMemcpyDeviceToDevice, s), "vsc_pipeline_gate"))
         return VSC_E_HIP;
-    if (count_dev && check_hip(hipMemcpyAsync(count_dev, p->count, 4, hipMemcpyDeviceToDevice, s), "vsc_pipeline_gate"))
+    if (count_dev &&
+        check_hip(hipMemcpyAsync(count_dev, p->count + 16 * k, 4, hipMemcpyDeviceToDevice, s), "vsc_pipeline_gate"))
         return VSC_E_HIP;
     return VSC_OK;
 }
@@ -366,6 +379,7 @@ int pipeline_advance(vsc_pipeline* p, long long frame)
         return set_error(VSC_E_INVALID, "pipeline_advance: not in steady state");
     p->slot_frame[sn] = frame + 1;
     p->prev_fwd_frame = frame;
+    p->last_parity = parity(frame);
     return VSC_OK;
 }
 
