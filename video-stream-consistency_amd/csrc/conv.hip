@@ -97,8 +97,15 @@ __device__ inline void conv_load4(const void* src, long long off, float v[4], in
     for (int r = 0; r < 4; ++r) v[r] += (r < nvalid) ? E::ld(p[r]) : 0.0f;
 }
 
+#ifndef VSC_CONV_WPE
+#define VSC_CONV_WPE 0
+#endif
 template <int EK, int EKO, int S, int NT, int MR, int NW>
-__global__ __launch_bounds__(64 * NW) void k_conv3x3(ConvArgs a)
+__global__ __launch_bounds__(64 * NW)
+#if VSC_CONV_WPE
+__attribute__((amdgpu_waves_per_eu(1, VSC_CONV_WPE)))
+#endif
+void k_conv3x3(ConvArgs a)
 {
     using E = Elem<EK>;
     using T = typename E::T;
@@ -170,22 +177,31 @@ __global__ __launch_bounds__(64 * NW) void k_conv3x3(ConvArgs a)
         }
         __syncthreads();
         if (c0 + KC < cend) fetch(c0 + KC);
-#pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
+        // the nine taps with the next tap's fragments read from LDS while this
+        // tap's MFMAs issue (two register sets)
+        cv_u32x4 wf[2][NT / 16], xf[2][MR];
+        auto frag = [&](int tap, cv_u32x4 (&w)[NT / 16], cv_u32x4 (&x)[MR]) {
             const int ky = tap / 3, kx = tap % 3;
-            cv_u32x4 wf[NT / 16], xf[MR];
 #pragma unroll
             for (int i = 0; i < NT / 16; ++i)
-                wf[i] = *reinterpret_cast<const cv_u32x4*>(&sW[((i * 16 + l16) * 9 + tap) * ROW + lg * (KC / 4)]);
+                w[i] = *reinterpret_cast<const cv_u32x4*>(&sW[((i * 16 + l16) * 9 + tap) * ROW + lg * (KC / 4)]);
 #pragma unroll
             for (int m = 0; m < MR; ++m) {
                 const int pr = (q * MR + m) * S + ky, pc = l16 * S + kx;
-                xf[m] = *reinterpret_cast<const cv_u32x4*>(&sP[(pr * PC + pc) * ROW + lg * (KC / 4)]);
+                x[m] = *reinterpret_cast<const cv_u32x4*>(&sP[(pr * PC + pc) * ROW + lg * (KC / 4)]);
             }
+        };
+        frag(0, wf[0], xf[0]);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            if (tap + 1 < 9) frag(tap + 1, wf[(tap + 1) & 1], xf[(tap + 1) & 1]);
+            // keep the reads of the next tap ahead of this tap's MFMAs
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < NT / 16; ++i)
 #pragma unroll
-                for (int m = 0; m < MR; ++m) acc[i][m] = conv_mfma<EK>(wf[i], xf[m], acc[i][m]);
+                for (int m = 0; m < MR; ++m) acc[i][m] = conv_mfma<EK>(wf[tap & 1][i], xf[tap & 1][m], acc[i][m]);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 
