@@ -71,10 +71,12 @@ static inline float bits_to_float(uint32_t u) { float f; std::memcpy(&f, &u, 4);
 
 /* exp(x) in binary32 by Cody & Waite (1980) argument reduction
  *   x = k*ln2 + r,  k = rint(x*log2(e)),  r = (x - k*ln2_hi) - k*ln2_lo
- * and a degree-7 Taylor polynomial for e^r on |r| <= ~0.35 (truncation error
- * r^8/8! < 6e-9), evaluated by Horner; the result is scaled by 2^k through the
- * exponent field.  Every step is a single IEEE binary32 operation, so the value
- * is the same on any IEEE machine that does not contract mul+add. */
+ * (each step one fused multiply-add; k*ln2_hi is exact) and a degree-7 Taylor
+ * polynomial for e^r on |r| <= ~0.35 (truncation error r^8/8! < 6e-9),
+ * evaluated by Horner with fused multiply-adds; the result is scaled by 2^k
+ * through the exponent field.  Every step is a single correctly rounded IEEE
+ * binary32 operation (std::fma; v_fma_f32 on the GPU), so the value is the same
+ * on any IEEE machine. */
 VSCO_API float vsco_expf(float x)
 {
     if (x != x) return x;
@@ -85,16 +87,16 @@ VSCO_API float vsco_expf(float x)
     const float ln2_lo = 1.428606765330187045e-06f;   /* ln2 - ln2_hi */
     float kf = rintf(x * log2e);
     int k = (int)kf;
-    float r = x - kf * ln2_hi;
-    r = r - kf * ln2_lo;
+    float r = std::fma(-kf, ln2_hi, x);
+    r = std::fma(-kf, ln2_lo, r);
     float p = 1.0f / 5040.0f;
-    p = p * r + 1.0f / 720.0f;
-    p = p * r + 1.0f / 120.0f;
-    p = p * r + 1.0f / 24.0f;
-    p = p * r + 1.0f / 6.0f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
+    p = std::fma(p, r, 1.0f / 720.0f);
+    p = std::fma(p, r, 1.0f / 120.0f);
+    p = std::fma(p, r, 1.0f / 24.0f);
+    p = std::fma(p, r, 1.0f / 6.0f);
+    p = std::fma(p, r, 0.5f);
+    p = std::fma(p, r, 1.0f);
+    p = std::fma(p, r, 1.0f);
     if (k > 127) { p = p * 2.0f; k -= 1; }
     if (k < -126) { p = p * 5.42101086242752217003726400434970855712890625e-20f; k += 64; }
     return p * bits_to_float((uint32_t)(k + 127) << 23);

@@ -656,8 +656,11 @@ __global__ __launch_bounds__(256) void k_front(FrontArgs a)
     const Rgba32 pPrev{reinterpret_cast<const uint32_t*>(a.pPrev.p), W};
     const Rgba32 pCur{reinterpret_cast<const uint32_t*>(a.pCur.p), W};
     const Rgba32 pNext{reinterpret_cast<const uint32_t*>(a.pNext.p), W};
-    const WarpTap tb = warp_tap(x, y, a.bwd.rd(x, y, 0), a.bwd.rd(x, y, 1), W, H);
-    const WarpTap tf = warp_tap(x, y, a.fwd.rd(x, y, 0), a.fwd.rd(x, y, 1), W, H);
+    const View32 fwd = v32(a.fwd), bwd = v32(a.bwd), last = v32(a.last);
+    const View32 L0 = v32(a.L0), A0 = v32(a.A0), W0 = v32(a.W0), P0 = v32(a.P0);
+    const View32 L1 = v32(a.L1), A1 = v32(a.A1), W1 = v32(a.W1), O1 = v32(a.O1), P1 = v32(a.P1);
+    const WarpTap tb = warp_tap(x, y, bwd.rd(x, y, 0), bwd.rd(x, y, 1), W, H);
+    const WarpTap tf = warp_tap(x, y, fwd.rd(x, y, 0), fwd.rd(x, y, 1), W, H);
     float pwi[3], pwp[3], nwi[3], nwp[3], lap0[3];
     warp_rgba(oPrev, tb, pwi);
     warp_rgba(pPrev, tb, pwp);
@@ -672,32 +675,32 @@ __global__ __launch_bounds__(256) void k_front(FrontArgs a)
     if (dec) cnt1 = lap_rgba(pCur, x, y, 2, x1, y1, a.A1.w, a.A1.h, lap1);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const float lw = warp_sample(a.last, tb, c);
+        const float lw = warp_sample(last, tb, c);
         const float ci = oCur.ch(vci, c);
         const float cp = pCur.ch(vcp, c);
         float adIn, adPr;
         adap_comb(ci, cp, pwi[c], pwp[c], nwi[c], nwp[c], lw, a.alpha, adIn, adPr);
         const float wt = consist_wt(ci, adIn, a.beta, a.gamma);
         if (a.fma) {
-            a.L0.wr(x, y, c, a.step * __builtin_fmaf(-wt, adPr, lap0[c]));
-            a.W0.wr(x, y, c, (float)cnt0 + wt);
+            L0.wr(x, y, c, a.step * __builtin_fmaf(-wt, adPr, lap0[c]));
+            W0.wr(x, y, c, (float)cnt0 + wt);
         } else {
-            a.L0.wr(x, y, c, lap0[c]);
-            a.A0.wr(x, y, c, adPr);
-            a.W0.wr(x, y, c, wt);
+            L0.wr(x, y, c, lap0[c]);
+            A0.wr(x, y, c, adPr);
+            W0.wr(x, y, c, wt);
         }
-        if (a.write_p) a.P0.wr(x, y, c, cp);
+        if (a.write_p) P0.wr(x, y, c, cp);
         if (dec) {
             if (a.fma) {
-                a.L1.wr(x1, y1, c, a.step * __builtin_fmaf(-wt, adPr, lap1[c]));
-                a.W1.wr(x1, y1, c, (float)cnt1 + wt);
+                L1.wr(x1, y1, c, a.step * __builtin_fmaf(-wt, adPr, lap1[c]));
+                W1.wr(x1, y1, c, (float)cnt1 + wt);
             } else {
-                a.L1.wr(x1, y1, c, lap1[c]);
-                a.A1.wr(x1, y1, c, adPr);
-                a.W1.wr(x1, y1, c, wt);
+                L1.wr(x1, y1, c, lap1[c]);
+                A1.wr(x1, y1, c, adPr);
+                W1.wr(x1, y1, c, wt);
             }
-            a.O1.wr(x1, y1, c, cp);
-            if (a.write_p) a.P1.wr(x1, y1, c, cp);
+            O1.wr(x1, y1, c, cp);
+            if (a.write_p) P1.wr(x1, y1, c, cp);
         }
     }
 }
@@ -713,6 +716,8 @@ int launch_front_pass(const FrontArgs& a, hipStream_t s)
     const double bytes = px * (24.0 + 12.0 + 4.0 * (a.fwd.ch + a.bwd.ch) + per_level +
                                (a.decimate ? (per_level + 12.0) / 4.0 : 0.0) +
                                (a.write_p ? (a.decimate ? 15.0 : 12.0) : 0.0));
+    for (const View* v : {&a.fwd, &a.bwd, &a.last, &a.L0, &a.A0, &a.W0, &a.P0, &a.L1, &a.A1, &a.W1, &a.O1, &a.P1})
+        if (!fits32(*v)) return set_error(VSC_E_INVALID, "front_pass: image too large for 32-bit offsets");
     const int slot = prof_start(PROF_FRONT_PASS, bytes, s);
     hipLaunchKernelGGL(k_front, grid2d(a.A0.w, a.A0.h), dim3(BX, BY), 0, s, a);
     prof_stop(slot, s);

@@ -44,6 +44,18 @@ struct View {
     __device__ inline void wr(int x, int y, int c, float v) const { d[idx(x, y, c)] = v; }
 };
 
+// The same view with 32-bit offsets (every element offset < 2^31, checked by
+// the launcher): cheaper addressing in the per-pixel kernels.
+struct View32 {
+    float* d;
+    int w, cs, ps;
+    __device__ inline int idx(int x, int y, int c) const { return c * cs + (y * w + x) * ps; }
+    __device__ inline float rd(int x, int y, int c) const { return d[idx(x, y, c)]; }
+    __device__ inline void wr(int x, int y, int c, float v) const { d[idx(x, y, c)] = v; }
+};
+__host__ __device__ inline View32 v32(const View& v) { return View32{v.d, v.w, (int)v.cs, (int)v.ps}; }
+inline bool fits32(const View& v) { return (long long)(v.ch - 1) * v.cs + ((long long)v.h * v.w - 1) * v.ps < (1LL << 31); }
+
 inline View hwc(const vsc_image& im) {
     return View{im.data, im.width, im.height, im.channels, 1, (long long)im.channels};
 }
@@ -52,17 +64,15 @@ inline View planar(float* d, int w, int h, int ch) {
 }
 
 // Byte to float as kernel_to_float_image (gpuimage.cu:49): (float)((double)v/255.0).
-// q = v * (1/255) is off by one ulp for 126 of the 256 values; one FMA
-// residual step (Markstein) rounds the quotient correctly, which for these
-// inputs equals the double quotient rounded to float (checked for all 256
-// values on the host with fmaf and on the device by the conversion tests).
+// 1/255 = 2^-8 + 2^-16 + 2^-24 + ...: q0 = v * (65793 * 2^-24) is exact
+// (v * 65793 < 2^24) and v/255 = q0 * (1 + 2^-24 + ...), whose tail lies
+// between half an ulp and an ulp of q0, so the correctly rounded quotient is
+// fma(q0, 2^-24, q0) (q0 = 0 stays 0).  Equal to the double quotient rounded
+// to float for all 256 values (exhaustive device test).
 __host__ __device__ inline float u8_to_f32(uint32_t v)
 {
-    const float r = 1.0f / 255.0f;
-    const float fv = (float)v;
-    float q = fv * r;
-    const float e = __builtin_fmaf(-q, 255.0f, fv);
-    return __builtin_fmaf(e, r, q);
+    const float q0 = (float)v * (65793.0f / 16777216.0f);
+    return __builtin_fmaf(q0, 1.0f / 16777216.0f, q0);
 }
 
 // RGBA8 source image: channel c of pixel (x, y) at p[(y*w + x)*4 + c].
@@ -74,9 +84,10 @@ struct Rgba {
 
 // ---------------------------------------------------------------------------
 // exp: Cody & Waite (1980) reduction x = k*ln2 + r with a two-part ln2, then a
-// degree-7 Taylor polynomial for e^r (|r| <= ~0.35), Horner form, and the
-// scale 2^k through the exponent field.  Only binary32 +, -, *, rint: identical
-// results on every IEEE device that does not contract (see the pragma above).
+// degree-7 Taylor polynomial for e^r (|r| <= ~0.35), Horner form with fused
+// multiply-adds, and the scale 2^k through the exponent field.  Only
+// correctly rounded binary32 operations: identical results on every IEEE
+// device (the oracle's std::fma on the host).
 // ---------------------------------------------------------------------------
 __host__ __device__ inline float vexp(float x)
 {
@@ -88,16 +99,16 @@ __host__ __device__ inline float vexp(float x)
     const float ln2_lo = 1.428606765330187045e-06f;
     float kf = __builtin_rintf(x * log2e);
     int k = (int)kf;
-    float r = x - kf * ln2_hi;
-    r = r - kf * ln2_lo;
+    float r = __builtin_fmaf(-kf, ln2_hi, x);
+    r = __builtin_fmaf(-kf, ln2_lo, r);
     float p = 1.0f / 5040.0f;
-    p = p * r + 1.0f / 720.0f;
-    p = p * r + 1.0f / 120.0f;
-    p = p * r + 1.0f / 24.0f;
-    p = p * r + 1.0f / 6.0f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
+    p = __builtin_fmaf(p, r, 1.0f / 720.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 120.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 24.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 6.0f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
     if (k > 127) { p = p * 2.0f; k -= 1; }
     if (k < -126) { p = p * 5.42101086242752217003726400434970855712890625e-20f; k += 64; }
     return p * __builtin_bit_cast(float, (uint32_t)(k + 127) << 23);
