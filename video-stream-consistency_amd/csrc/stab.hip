@@ -325,7 +325,7 @@ template <int R, int STRIPS, bool FMA>
 __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     const float* __restrict__ LapP, const float* __restrict__ A, const float* __restrict__ Wt, const float* Oin,
     float* Oout, const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step, float mom,
-    int halo, int tiles_x, int tiles_y, int per_xcd)
+    int halo, int tiles_x, int tiles_y, int per_xcd, int store_u)
 {
     constexpr int TW = SB_TW;
     constexpr int TH = STRIPS * R;
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
             const int gy = gy0 + ty;
             if (ty >= halo && ty < TH - halo && gy >= 0 && gy < H) {
                 Oo[wbase + r * W] = o[r];
-                Uo[wbase + r * W] = u[r];
+                if (store_u) Uo[wbase + r * W] = u[r];
             }
         }
     }
@@ -552,7 +552,7 @@ namespace {
 template <int R, bool FMA>
 int launch_blocked(const float* lapP, const float* A, const float* Wt, const float* Oin, float* Oout, const float* Uin,
                    float* Uout, int W, int H, int sweeps, int first_sweep, float step, float mom,
-                   const BlockedPlan& plan, hipStream_t s)
+                   const BlockedPlan& plan, int store_u, hipStream_t s)
 {
     const int OW = SB_TW - 2 * plan.halo;
     const int OH = plan.strips * R - 2 * plan.halo;
@@ -560,15 +560,18 @@ int launch_blocked(const float* lapP, const float* A, const float* Wt, const flo
     const int tiles_y = (H + OH - 1) / OH;
     const int per_xcd = (tiles_x * tiles_y * 3 + 7) / 8;
     // algorithmic bytes: read O, U and the per-pixel constants (lapP, A, w; FMA
-    // form: q, kk) and write O, U once per pixel and channel
-    const int slot = prof_start(PROF_CONSIST_BLOCKED, (FMA ? 6.0 : 7.0) * 3 * 4 * (double)W * H, s);
+    // form: q, kk) and write O (and U unless it is the level's last launch)
+    // once per pixel and channel
+    const int slot =
+        prof_start(PROF_CONSIST_BLOCKED, ((FMA ? 6.0 : 7.0) - (store_u ? 0.0 : 1.0)) * 3 * 4 * (double)W * H, s);
     if (plan.strips == SB_STRIPS)
         hipLaunchKernelGGL((k_consist_blocked<R, SB_STRIPS, FMA>), dim3(8 * per_xcd), dim3(SB_TW * SB_STRIPS), 0, s,
                            lapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, plan.halo, tiles_x,
-                           tiles_y, per_xcd);
+                           tiles_y, per_xcd, store_u);
     else
         hipLaunchKernelGGL((k_consist_blocked<R, 2, FMA>), dim3(8 * per_xcd), dim3(SB_TW * 2), 0, s, lapP, A, Wt, Oin,
-                           Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, plan.halo, tiles_x, tiles_y, per_xcd);
+                           Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, plan.halo, tiles_x, tiles_y, per_xcd,
+                           store_u);
     prof_stop(slot, s);
     return check_launch("consist_blocked");
 }
@@ -577,7 +580,7 @@ int launch_blocked(const float* lapP, const float* A, const float* Wt, const flo
 
 int launch_consist_blocked(const float* lapP, const float* A, const float* Wt, const float* Oin, float* Oout,
                            const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step,
-                           float mom, int fma, const BlockedPlan& plan, hipStream_t s)
+                           float mom, int fma, const BlockedPlan& plan, hipStream_t s, int store_u)
 {
     const int R = fma ? SB_R_FMA : SB_R;
     if (plan.strips != SB_STRIPS && plan.strips != 2)
@@ -587,9 +590,9 @@ int launch_consist_blocked(const float* lapP, const float* A, const float* Wt, c
     if (sweeps < 1 || sweeps > plan.halo) return set_error(VSC_E_INVALID, "consist_blocked: sweeps out of range");
     if (fma)
         return launch_blocked<SB_R_FMA, true>(lapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom,
-                                              plan, s);
+                                              plan, store_u, s);
     return launch_blocked<SB_R, false>(lapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, plan,
-                                       s);
+                                       store_u, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -874,7 +877,7 @@ extern "C" int vsc_stab_consist_out_planar(const float* crnt_pr, const float* pr
     for (int k = 0; k < num_iter && !rc;) {
         const int m = std::min(plan.halo, num_iter - k);
         rc = launch_consist_blocked(lapP, prev_stab_warp, consist_wt, O[cur], O[cur ^ 1], U[cur], U[cur ^ 1], width,
-                                    height, m, k, step_size, mom_fac, 0, plan, s);
+                                    height, m, k, step_size, mom_fac, 0, plan, s, k + m < num_iter);
         cur ^= 1;
         k += m;
     }

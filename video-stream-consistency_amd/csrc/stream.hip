@@ -212,19 +212,20 @@ int solve_pyramid(vsc_stream* s, int L, const vsc_params& p, int fma, hipStream_
     int cur = 0;
     for (int j = L - 1; j >= 0; --j) {
         const int w = s->lw[j], h = s->lh[j];
+        const int iters = p.num_iter / (j + 1);
         int rc;
         if (j != L - 1) {
             cur = 0;
             rc = launch_bilinear(pl(s->O[j + 1][*final_idx], s->lw[j + 1], s->lh[j + 1]), pl(s->O[j][0], w, h), st);
             if (rc) return rc;
         }
-        const int iters = p.num_iter / (j + 1);
         const BlockedPlan plan = plan_consist_blocked(w, h, iters, fma);
         int ucur = 0;
         for (int k = 0; k < iters;) {
             const int n = std::min(plan.halo, iters - k);
             rc = launch_consist_blocked(s->L[j], s->A[j], s->Wt[j], s->O[j][cur], s->O[j][cur ^ 1], s->U[j][ucur],
-                                        s->U[j][ucur ^ 1], w, h, n, k, p.step_size, p.mom_fac, fma, plan, st);
+                                        s->U[j][ucur ^ 1], w, h, n, k, p.step_size, p.mom_fac, fma, plan, st,
+                                        k + n < iters);
             if (rc) return rc;
             cur ^= 1;
             ucur ^= 1;

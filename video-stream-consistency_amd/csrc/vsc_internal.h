@@ -226,9 +226,11 @@ struct BlockedPlan {
 BlockedPlan plan_consist_blocked(int W, int H, int iters, int fma);
 // fma == 0: lapP, A, Wt are the level's lapP, A, w (reference op order);
 // fma == 1: lapP holds k = step*fma(-w, A, lapP), Wt holds q = cnt + w, A unused.
+// store_u = 0 skips writing the momentum (last launch of a level: the next
+// level starts with U = 0).
 int launch_consist_blocked(const float* lapP, const float* A, const float* Wt, const float* Oin, float* Oout,
                            const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step,
-                           float mom, int fma, const BlockedPlan& plan, hipStream_t s);
+                           float mom, int fma, const BlockedPlan& plan, hipStream_t s, int store_u = 1);
 // in place: Wt <- cnt + Wt, L <- step*fma(-Wt, A, L) (the FMA-form constants of a planar level)
 int launch_fma_consts(float* L, const float* A, float* Wt, int W, int H, float step, hipStream_t s);
 int launch_to_rgba(View in, uint8_t* out, hipStream_t s);
