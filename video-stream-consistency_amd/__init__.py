@@ -727,7 +727,12 @@ class Pipeline:
     at FLOWDOWNSCALE 2 with the learned upsampler.  Frames are RGBA8 device
     tensors; a window is (orig_prev, orig_cur, orig_next, proc_prev, proc_cur,
     proc_next) and `frame` is the index of orig_cur (features are cached by
-    index)."""
+    index).
+
+    overlap=True pipelines frames on two internal HIP streams (the flows of a
+    frame while the previous one is stabilized): frame t's outputs are then
+    ordered on torch's current stream by the call for frame t+1, or by sync(),
+    and the window tensors of a call must stay unchanged until then."""
 
     # both forms stabilize with the FMA-form solver (FP32, <= 1 LSB from the
     # reference op order); solver="exact" selects the reference order
@@ -803,6 +808,8 @@ class Pipeline:
         return f, b
 
     def frame(self, window, frame, out_rgba=None, out_float=None, params=None):
+        """Stabilize window `frame` into out_rgba (and out_float, HWC float);
+        stream-ordered on torch's current stream (one frame later with overlap)."""
         torch = _torch()
         if out_rgba is None:
             out_rgba = torch.empty((self.height, self.width, 4), dtype=torch.uint8, device="cuda")
