@@ -97,15 +97,8 @@ __device__ inline void conv_load4(const void* src, long long off, float v[4], in
     for (int r = 0; r < 4; ++r) v[r] += (r < nvalid) ? E::ld(p[r]) : 0.0f;
 }
 
-#ifndef VSC_CONV_WPE
-#define VSC_CONV_WPE 0
-#endif
 template <int EK, int EKO, int S, int NT, int MR, int NW>
-__global__ __launch_bounds__(64 * NW)
-#if VSC_CONV_WPE
-__attribute__((amdgpu_waves_per_eu(1, VSC_CONV_WPE)))
-#endif
-void k_conv3x3(ConvArgs a)
+__global__ __launch_bounds__(64 * NW) void k_conv3x3(ConvArgs a)
 {
     using E = Elem<EK>;
     using T = typename E::T;
@@ -137,8 +130,13 @@ void k_conv3x3(ConvArgs a)
 #pragma unroll
         for (int m = 0; m < MR; ++m) acc[i][m] = cv_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    cv_u32x4 rp[RP], rw[RW];
-    auto fetch = [&](int c0) {
+    // input channel chunks are staged through registers DEPTH chunks ahead of
+    // the chunk in LDS, so the global loads of a chunk overlap the MFMAs of the
+    // DEPTH chunks before it (deeper staging measured slower: 2 and 3 chunks
+    // ahead cost the 1080p BF16 network 3% and 18%)
+    constexpr int DEPTH = 1;
+    cv_u32x4 rp[DEPTH][RP], rw[DEPTH][RW];
+    auto fetch = [&](int c0, cv_u32x4 (&P)[RP], cv_u32x4 (&Wr)[RW]) {
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
             const int u = tid + NTH * k;
@@ -148,7 +146,7 @@ void k_conv3x3(ConvArgs a)
             cv_u32x4 v = cv_u32x4{0u, 0u, 0u, 0u};
             if (u < NP && iy >= 0 && iy < a.H && ix >= 0 && ix < a.W && ch < a.Cin)
                 v = *reinterpret_cast<const cv_u32x4*>(in + ((long long)iy * a.W + ix) * a.in_cs + ch);
-            rp[k] = v;
+            P[k] = v;
         }
 #pragma unroll
         for (int k = 0; k < RW; ++k) {
@@ -158,27 +156,26 @@ void k_conv3x3(ConvArgs a)
             cv_u32x4 v = cv_u32x4{0u, 0u, 0u, 0u};
             if (u < NWU && co < a.Cout && ch < a.Cin)
                 v = *reinterpret_cast<const cv_u32x4*>(wt + ((long long)co * 9 + row % 9) * a.Cin + ch);
-            rw[k] = v;
+            Wr[k] = v;
         }
     };
-
-    fetch(cbeg);
-    for (int c0 = cbeg; c0 < cend; c0 += KC) {
+    auto stage = [&](const cv_u32x4 (&P)[RP], const cv_u32x4 (&Wr)[RW]) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
             const int u = tid + NTH * k;
-            if (u < NP) *reinterpret_cast<cv_u32x4*>(&sP[(u / KV) * ROW + (u % KV) * VEC]) = rp[k];
+            if (u < NP) *reinterpret_cast<cv_u32x4*>(&sP[(u / KV) * ROW + (u % KV) * VEC]) = P[k];
         }
 #pragma unroll
         for (int k = 0; k < RW; ++k) {
             const int u = tid + NTH * k;
-            if (u < NWU) *reinterpret_cast<cv_u32x4*>(&sW[(u / KV) * ROW + (u % KV) * VEC]) = rw[k];
+            if (u < NWU) *reinterpret_cast<cv_u32x4*>(&sW[(u / KV) * ROW + (u % KV) * VEC]) = Wr[k];
         }
         __syncthreads();
-        if (c0 + KC < cend) fetch(c0 + KC);
-        // the nine taps with the next tap's fragments read from LDS while this
-        // tap's MFMAs issue (two register sets)
+    };
+    // the nine taps with the next tap's fragments read from LDS while this
+    // tap's MFMAs issue (two register sets)
+    auto compute = [&]() {
         cv_u32x4 wf[2][NT / 16], xf[2][MR];
         auto frag = [&](int tap, cv_u32x4 (&w)[NT / 16], cv_u32x4 (&x)[MR]) {
             const int ky = tap / 3, kx = tap % 3;
@@ -202,6 +199,20 @@ void k_conv3x3(ConvArgs a)
 #pragma unroll
                 for (int m = 0; m < MR; ++m) acc[i][m] = conv_mfma<EK>(wf[tap & 1][i], xf[tap & 1][m], acc[i][m]);
             __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d)
+        if (cbeg + d * KC < cend) fetch(cbeg + d * KC, rp[d], rw[d]);
+    for (int c0 = cbeg; c0 < cend; c0 += DEPTH * KC) {
+#pragma unroll
+        for (int d = 0; d < DEPTH; ++d) {
+            const int cc = c0 + d * KC;
+            if (cc >= cend) break;
+            stage(rp[d], rw[d]);
+            if (cc + DEPTH * KC < cend) fetch(cc + DEPTH * KC, rp[d], rw[d]);
+            compute();
         }
     }
 
@@ -293,9 +304,6 @@ static void conv_go(const ConvArgs& a, hipStream_t s)
 
 // tile choice: the widest channel tile that the layer fills, and the tallest
 // pixel tile that still gives about two blocks per CU
-#ifndef VSC_CONV_MODE
-#define VSC_CONV_MODE 0
-#endif
 template <int EK, int EKO, int S>
 static void conv_pick(const ConvArgs& a, hipStream_t s)
 {
@@ -304,16 +312,11 @@ static void conv_pick(const ConvArgs& a, hipStream_t s)
     const int NTsel = a.Cout > 32 ? 64 : (a.Cout > 16 ? 32 : 16);
     const long long ntiles = (a.Cout + NTsel - 1) / NTsel;
     auto blocks = [&](int mr) { return (px / (64LL * mr)) * ntiles; };
-    const bool wide = VSC_CONV_MODE != 0 && S == 1 && EK != EK_F32;
-    int mr;
-    if (wide) mr = (VSC_CONV_MODE == 1 && blocks(4) >= 512) ? 2 : 1;   // 8 waves: 128 or 256 pixels
-    else mr = (S == 1 && blocks(4) >= 512) ? 4 : (blocks(2) >= 512 ? 2 : 1);
+    int mr = (S == 1 && blocks(4) >= 512) ? 4 : (blocks(2) >= 512 ? 2 : 1);
     if (S == 2 && mr > 2) mr = 2;
-#define VSC_CONV_NT(NTV)                                              \
-    if (wide && mr == 2) conv_go<EK, EKO, S, NTV, 2, 8>(a, s);        \
-    else if (wide) conv_go<EK, EKO, S, NTV, 1, 8>(a, s);              \
-    else if (mr == 4) conv_go<EK, EKO, S, NTV, (S == 1 ? 4 : 2), 4>(a, s); \
-    else if (mr == 2) conv_go<EK, EKO, S, NTV, 2, 4>(a, s);           \
+#define VSC_CONV_NT(NTV)                                                   \
+    if (mr == 4) conv_go<EK, EKO, S, NTV, (S == 1 ? 4 : 2), 4>(a, s);      \
+    else if (mr == 2) conv_go<EK, EKO, S, NTV, 2, 4>(a, s);                \
     else conv_go<EK, EKO, S, NTV, 1, 4>(a, s);
     if (NTsel == 64) { VSC_CONV_NT(64) }
     else if (NTsel == 32) { VSC_CONV_NT(32) }
