@@ -4,6 +4,7 @@
 #include "vsc_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace vsc {
 
@@ -219,34 +220,45 @@ int launch_consist_sweep(View P, View A, View Wt, View Oin, View Oout, View U, f
 // ---------------------------------------------------------------------------
 static constexpr int SB_TW = 128;
 
-template <int R, bool MOM, bool INTERIOR>
+// Rows of a strip are visited in ascending (REV false) or descending (REV
+// true) order.  A row's old value is still needed by the next row visited
+// after the row itself is updated, so a sweep leaves its O and U shifted by
+// one register; the kernel alternates the two orders, and a descending sweep
+// shifts them back, so a pair of sweeps needs no register copies.  Each
+// pixel's arithmetic (and its operation order) is the same in both orders.
+template <int R, bool MOM, bool INTERIOR, bool REV>
 __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const float (&a)[R], const float (&w)[R],
                                             const float (&lp)[R], const float* colL, const float* colR,
                                             float up_edge, float dn_edge, float step, float mom, bool hr, bool hl,
                                             int rU, int rD)
 {
-    float4 l4 = *reinterpret_cast<const float4*>(colL);
-    float4 r4 = *reinterpret_cast<const float4*>(colR);
-    float up_old = up_edge;
+    constexpr int NK = R / 4;
+    float4 l4 = *reinterpret_cast<const float4*>(colL + (REV ? R - 4 : 0));
+    float4 r4 = *reinterpret_cast<const float4*>(colR + (REV ? R - 4 : 0));
+    float prev = REV ? dn_edge : up_edge;  // old value of the row visited before
 #pragma unroll
-    for (int k = 0; k < R / 4; ++k) {
+    for (int i = 0; i < NK; ++i) {
+        const int k = REV ? NK - 1 - i : i;
         float4 ln = l4, rn = r4;
-        if (k + 1 < R / 4) {
-            ln = *reinterpret_cast<const float4*>(colL + 4 * (k + 1));
-            rn = *reinterpret_cast<const float4*>(colR + 4 * (k + 1));
+        if (i + 1 < NK) {
+            const int kn = REV ? k - 1 : k + 1;
+            ln = *reinterpret_cast<const float4*>(colL + 4 * kn);
+            rn = *reinterpret_cast<const float4*>(colR + 4 * kn);
         }
         const float lv[4] = {l4.x, l4.y, l4.z, l4.w};
         const float rv[4] = {r4.x, r4.y, r4.z, r4.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int jj = 0; jj < 4; ++jj) {
+            const int j = REV ? 3 - jj : jj;
             const int r = 4 * k + j;
             const float cur = o[r];
-            const float dn = (r < R - 1) ? o[r + 1] : dn_edge;
+            const float up = REV ? (r > 0 ? o[r - 1] : up_edge) : prev;
+            const float dn = REV ? prev : (r < R - 1 ? o[r + 1] : dn_edge);
             float lo;
             if (INTERIOR) {
                 lo = rv[j] + lv[j];
                 lo += dn;
-                lo += up_old;
+                lo += up;
                 lo -= 4.0f * cur;
             } else {
                 int cnt = 0;
@@ -254,7 +266,7 @@ __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const 
                 if (hr) { lo += rv[j]; cnt += 1; }
                 if (hl) { lo += lv[j]; cnt += 1; }
                 if (r <= rD) { lo += dn; cnt += 1; }
-                if (r >= rU) { lo += up_old; cnt += 1; }
+                if (r >= rU) { lo += up; cnt += 1; }
                 lo -= (float)cnt * cur;
             }
             const float t1 = w[r] * (cur - a[r]);
@@ -263,7 +275,7 @@ __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const 
             const float sg = step * g;
             o[r] = MOM ? (cur + sg) + mom * u[r] : cur + sg;
             u[r] = sg;
-            up_old = cur;
+            prev = cur;
         }
         l4 = ln;
         r4 = rn;
@@ -274,47 +286,52 @@ __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const 
 // per-pixel constants, and per pixel
 //   X = fma(-q, O, sum of neighbours); U' = fma(step, X, -kk); O' = O + U' (+ mom*U)
 // (7 VALU operations in the interior instead of 13).
-template <int R, bool MOM, bool INTERIOR>
+template <int R, bool MOM, bool INTERIOR, bool REV>
 __device__ __forceinline__ void sweep_strip_fma(float (&o)[R], float (&u)[R], const float (&q)[R],
                                                 const float (&kk)[R], const float* colL, const float* colR,
                                                 float up_edge, float dn_edge, float step, float mom, bool hr, bool hl,
                                                 int rU, int rD)
 {
-    float4 l4 = *reinterpret_cast<const float4*>(colL);
-    float4 r4 = *reinterpret_cast<const float4*>(colR);
-    float up_old = up_edge;
+    constexpr int NK = R / 4;
+    float4 l4 = *reinterpret_cast<const float4*>(colL + (REV ? R - 4 : 0));
+    float4 r4 = *reinterpret_cast<const float4*>(colR + (REV ? R - 4 : 0));
+    float prev = REV ? dn_edge : up_edge;  // old value of the row visited before
 #pragma unroll
-    for (int k = 0; k < R / 4; ++k) {
+    for (int i = 0; i < NK; ++i) {
+        const int k = REV ? NK - 1 - i : i;
         float4 ln = l4, rn = r4;
-        if (k + 1 < R / 4) {
-            ln = *reinterpret_cast<const float4*>(colL + 4 * (k + 1));
-            rn = *reinterpret_cast<const float4*>(colR + 4 * (k + 1));
+        if (i + 1 < NK) {
+            const int kn = REV ? k - 1 : k + 1;
+            ln = *reinterpret_cast<const float4*>(colL + 4 * kn);
+            rn = *reinterpret_cast<const float4*>(colR + 4 * kn);
         }
         const float lv[4] = {l4.x, l4.y, l4.z, l4.w};
         const float rv[4] = {r4.x, r4.y, r4.z, r4.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int jj = 0; jj < 4; ++jj) {
+            const int j = REV ? 3 - jj : jj;
             const int r = 4 * k + j;
             const float cur = o[r];
-            const float dn = (r < R - 1) ? o[r + 1] : dn_edge;
+            const float up = REV ? (r > 0 ? o[r - 1] : up_edge) : prev;
+            const float dn = REV ? prev : (r < R - 1 ? o[r + 1] : dn_edge);
             float sum;
             if (INTERIOR) {
                 sum = rv[j] + lv[j];
                 sum += dn;
-                sum += up_old;
+                sum += up;
             } else {
                 sum = 0.0f;
                 if (hr) sum += rv[j];
                 if (hl) sum += lv[j];
                 if (r <= rD) sum += dn;
-                if (r >= rU) sum += up_old;
+                if (r >= rU) sum += up;
             }
             const float X = __builtin_fmaf(-q[r], cur, sum);
             const float sg = __builtin_fmaf(step, X, -kk[r]);
             const float on = cur + sg;
             o[r] = MOM ? __builtin_fmaf(mom, u[r], on) : on;
             u[r] = sg;
-            up_old = cur;
+            prev = cur;
         }
         l4 = ln;
         r4 = rn;
@@ -402,7 +419,10 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     const int rU0 = 1 - gys;
     const int rD0 = (H - 3) - gys;
 
-    for (int s = 0; s < sweeps; ++s) {
+    // One sweep: publish this strip's O, then update it from the neighbours.
+    auto sweep = [&](int s, auto mom_c, auto rev_c, auto interior_c) {
+        constexpr bool MOM = decltype(mom_c)::value, REV = decltype(rev_c)::value;
+        constexpr bool INTERIOR = decltype(interior_c)::value;
         float (*buf)[TW][RS] = sO[s & 1];
         float* own = buf[strip][tx];
 #pragma unroll
@@ -417,23 +437,29 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
         // opaque per sweep: keeps the compiler from hoisting 2*R row
         // predicates out of the sweep loop into scalar registers
         asm volatile("" : "+s"(rU), "+s"(rD));
-        const bool mom_on = (first_sweep + s) > 0;
-        if constexpr (FMA) {
-            if (interior) {
-                if (mom_on) sweep_strip_fma<R, true, true>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
-                else sweep_strip_fma<R, false, true>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
-            } else {
-                if (mom_on) sweep_strip_fma<R, true, false>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
-                else sweep_strip_fma<R, false, false>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
-            }
-        } else if (interior) {
-            if (mom_on) sweep_strip<R, true, true>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
-            else sweep_strip<R, false, true>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
-        } else {
-            if (mom_on) sweep_strip<R, true, false>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
-            else sweep_strip<R, false, false>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+        if constexpr (FMA)
+            sweep_strip_fma<R, MOM, INTERIOR, REV>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+        else
+            sweep_strip<R, MOM, INTERIOR, REV>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+    };
+    // The level's first sweep has no momentum term; after it, ascending and
+    // descending sweeps alternate in pairs.
+    auto run = [&](auto interior_c) {
+        const std::true_type yes{};
+        const std::false_type no{};
+        int s = 0;
+        if (first_sweep == 0) {
+            sweep(0, no, no, interior_c);
+            s = 1;
         }
-    }
+        for (; s + 1 < sweeps; s += 2) {
+            sweep(s, yes, no, interior_c);
+            sweep(s + 1, yes, yes, interior_c);
+        }
+        if (s < sweeps) sweep(s, yes, no, interior_c);
+    };
+    if (interior) run(std::true_type{});
+    else run(std::false_type{});
 
     if (tx >= halo && tx < TW - halo && colin) {
         const int wbase = gys * W + gx;
