@@ -221,10 +221,13 @@ def test_frame_errors(vsc, cuda):
     st.close()
 
 
-@pytest.mark.parametrize("H,W,iters", [(37, 53, 19), (300, 500, 8), (1080, 1920, 9), (3, 700, 5)])
+@pytest.mark.parametrize("H,W,iters", [(37, 53, 19), (300, 500, 8), (1080, 1920, 9), (3, 700, 5),
+                                        (40, 70, 1), (40, 70, 2), (130, 260, 4)])
 def test_consist_out_planar_blocked(vsc, orc, cuda, H, W, iters):
     """The temporally blocked solver on planar images, checked against the
-    oracle (HWC) over tile and launch boundaries."""
+    oracle (HWC) over tile and launch boundaries.  Sweep counts 1, 2 and 4
+    cover a launch with only the level's first sweep, the first sweep plus an
+    odd remainder, and the first sweep plus ascending/descending pairs."""
     import torch
     rng = np.random.default_rng(H + W)
     P, A, O = (rng.random((H, W, 3)).astype(np.float32) for _ in range(3))
