@@ -279,15 +279,18 @@ static void conv_go(const ConvArgs& a, hipStream_t s)
     b.tiles_x = (ws + 15) / 16;
     b.ctiles = (a.Cout + NT - 1) / NT;
     const int chunks = (a.Cin + KC - 1) / KC;
-    // split the channel loop when the grid leaves most CUs idle and the
-    // caller provided room for the partial sums.  The split depends on one
-    // batch item's geometry only (room reserved for two items), so an item's
-    // result has the same bits alone and in a batch of two.
+    // split the channel loop when the grid leaves most CUs idle (FP32: under
+    // 128 blocks per item; 16-bit: under 32, where the partial-sum traffic
+    // and the finalize launch cost more than idle CUs down to that size, as
+    // measured on 1080p frames) and the caller provided room for the partial
+    // sums.  The split depends on one batch item's geometry only (room
+    // reserved for two items), so an item's result has the same bits alone
+    // and in a batch of two.
     const long long item_blocks = (long long)b.tiles_x * b.tiles_y * ps * ps * b.ctiles;
     const long long item_stride = (long long)a.Ho * a.Wo * a.Cout;
     const long long stride = item_stride * a.N;
     int parts = 1;
-    if (a.partial && chunks > 1 && item_blocks < 128 && a.N <= 2) {
+    if (a.partial && chunks > 1 && item_blocks < (EK == EK_F32 ? 128 : 32) && a.N <= 2) {
         parts = (int)std::min<long long>(chunks, (256 + item_blocks - 1) / item_blocks);
         parts = (int)std::min<long long>(parts, a.part_stride / std::max<long long>(2 * item_stride, 1));
         if (parts < 2) parts = 1;
@@ -302,8 +305,11 @@ static void conv_go(const ConvArgs& a, hipStream_t s)
         hipLaunchKernelGGL(k_conv_finalize<EKO>, dim3((unsigned)((stride + 255) / 256)), dim3(256), 0, s, b, parts);
 }
 
-// tile choice: the widest channel tile that the layer fills, and the tallest
-// pixel tile that still gives about two blocks per CU
+// tile choice: the widest channel tile that the layer fills; 256-pixel tiles
+// from two blocks per CU; 128-pixel tiles from two blocks per CU (FP32) or
+// from half a block per CU (16-bit: its MFMAs are 16x faster, so the weights
+// every block re-reads dominate, and halving that traffic pays for the idle
+// CUs of a smaller grid); else 64-pixel tiles
 template <int EK, int EKO, int S>
 static void conv_pick(const ConvArgs& a, hipStream_t s)
 {
@@ -312,7 +318,7 @@ static void conv_pick(const ConvArgs& a, hipStream_t s)
     const int NTsel = a.Cout > 32 ? 64 : (a.Cout > 16 ? 32 : 16);
     const long long ntiles = (a.Cout + NTsel - 1) / NTsel;
     auto blocks = [&](int mr) { return (px / (64LL * mr)) * ntiles; };
-    int mr = (S == 1 && blocks(4) >= 512) ? 4 : (blocks(2) >= 512 ? 2 : 1);
+    int mr = (S == 1 && blocks(4) >= 512) ? 4 : (blocks(2) >= (EK == EK_F32 ? 512 : 128) ? 2 : 1);
     if (S == 2 && mr > 2) mr = 2;
 #define VSC_CONV_NT(NTV)                                                   \
     if (mr == 4) conv_go<EK, EKO, S, NTV, (S == 1 ? 4 : 2), 4>(a, s);      \
