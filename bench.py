@@ -47,9 +47,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 PSNR")
-    ap.add_argument("--workload", choices=["pipeline", "stabilize", "ops", "serve"], default="pipeline",
+    ap.add_argument("--workload", choices=["pipeline", "stabilize", "flownet", "ops", "serve"], default="pipeline",
                     help="pipeline: 1080p frames with computed flows, chosen vs plain form (headline); "
-                         "stabilize: 1080p frames with flows given; ops: Correlation/Warp on the 1920x1088 pyramid; "
+                         "stabilize: 1080p frames with flows given; flownet: the flow network alone per 1080p frame "
+                         "(chosen and plain forms); ops: Correlation/Warp on the 1920x1088 pyramid; "
                          "serve: 24 concurrent streams of mixed resolutions at 30 fps (multi-stream server)")
     ap.add_argument("--serve-frames", type=int, default=60, help="frames per stream (serve workload)")
     ap.add_argument("--serve-fps", type=float, default=30.0, help="arrival rate per stream (serve workload)")
@@ -185,6 +186,61 @@ def bench_ops(args):
                                 "hbm_frac": round(tot_bytes / tot_us / 1e3 / HBM_PEAK_GBS, 4),
                                 "max_abs_err": worst if orc is not None else None, "levels": levels}
     print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_flownet(args):
+    """The flow network alone, per 1080p frame as the pipeline runs it: the new
+    frame's features (one extraction) and the forward and backward flows of the
+    window decoded as one batch of two pairs.  Chosen form (BF16 at
+    FLOWDOWNSCALE=2) first, then the plain form (FP32 at full resolution);
+    HIP events on the current stream around the timed frames."""
+    import importlib
+
+    import torch
+
+    vsc = importlib.import_module("video-stream-consistency_amd")
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    orig, _, _, _ = vsc.synth_video(**dict(vsc.SYNTH_1080P, nframes=4))
+    fr = [torch.from_numpy(o).to(dev) for o in orig]
+    forms = {}
+    for name, down, dtype, steps in (("chosen", 2, "bf16", args.steps), ("plain", 1, "f32", max(1, args.steps // 3))):
+        net = vsc.FlowNet(W, H, down, dtype, seed=1, slots=3)
+        net.scale_flow_heads(FLOW_HEAD_SCALE)
+        outs = [torch.empty((net.flow_size[1], net.flow_size[0], 2), device=dev) for _ in range(2)]
+
+        def frame(i):
+            # slots rotate like the pipeline's: the new frame lands in slot i % 3
+            net.extract(fr[i % 4], i % 3)
+            cur, prev, nxt = (i - 1) % 3, (i - 2) % 3, i % 3
+            net.flow([(cur, nxt), (cur, prev)], 2, outs)
+
+        for i in range(3 + args.warmup):
+            frame(i)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for i in range(steps):
+            frame(3 + args.warmup + i)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        forms[name] = {"frames_per_s": round(1000.0 / ms, 2), "ms_per_frame": round(ms, 4), "steps": steps,
+                       "network": f"{dtype.upper()} PWCNet-light at FLOWDOWNSCALE={down}",
+                       "flow_size": list(net.flow_size),
+                       "footprint_mb": round(vsc.FlowNet.footprint(W, H, down, dtype, 3) / 1e6, 1)}
+        net.close()
+    result = {"metric": "1080p flow-network frames/s (new frame's features + fwd/bwd decode)",
+              "value": forms["chosen"]["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+              "warmup": args.warmup, "ms_per_step": forms["chosen"]["ms_per_frame"], "higher_is_better": True,
+              "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+              "data": "synthetic: seed-generated 1920x1080 RGBA8 frames resident in HBM; seeded network weights, "
+                      "flow heads scaled by 0.05",
+              "config": {"workload": "flow network of one 1920x1080 frame: one feature extraction and a batch of "
+                                     "two pair decodes", "width": W, "height": H},
+              "forms": forms}
+    print(json.dumps(result), flush=True)
     return 0
 
 
@@ -521,6 +577,8 @@ def main():
         return bench_pipeline(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "flownet":
+        return bench_flownet(args)
     import importlib
 
     import numpy as np
