@@ -329,7 +329,7 @@ int decode(vsc_flownet* n, int np, const int* first, const int* second, hipStrea
         void* cat = n->cat0[lvl];
         if (lvl == 6) {
             if ((rc = launch_zero_slice(cat, n->eb, cc, 88 + CV, 2, px * np, s))) return rc;
-            if ((rc = launch_corr_nhwc(ek, f1, f2, CV, CV, CV, np, h, w, cat, cc, s))) return rc;
+            if ((rc = launch_corr_nhwc(ek, f1, f2, CV, CV, CV, np, h, w, cat, cc, 88, s))) return rc;
         } else {
             const Layer& up = n->L[n->up0 + (5 - lvl)];
             if ((rc = launch_upflow(ek, n->flow[lvl + 1], np, h / 2, w / 2, static_cast<const float*>(up.dw), up.db,
@@ -337,9 +337,9 @@ int decode(vsc_flownet* n, int np, const int* first, const int* second, hipStrea
                 return rc;
             if ((rc = launch_warp_nhwc(ek, f2, CV, n->upflow[lvl], 2, kDbl[lvl], n->warped, CV, CV, np, h, w, s))) return rc;
             const void* wp[2] = {n->warped, static_cast<const char*>(n->warped) + px * CV * n->eb};
-            if ((rc = launch_corr_nhwc(ek, f1, wp, CV, CV, CV, np, h, w, cat, cc, s))) return rc;
+            if ((rc = launch_corr_nhwc(ek, f1, wp, CV, CV, CV, np, h, w, cat, cc, 88, s))) return rc;
         }
-        if ((rc = launch_copy_slice(ek, ek, f1, np, CV, 0, cat, cc, 88, CV, px, s))) return rc;
+        // cat0 = [leaky(corr) | pad | f1 | upflow | pad]: the cost volume kernel wrote f1 at channel 88
         const Layer* l = &n->L[n->dec0 + 6 * (6 - lvl)];
         void* d = n->dec[lvl];
         if ((rc = conv(n, l[0], T(cat, np, h, w, cc, cc, 0), T(d, np, h, w, 128, kDecC, 0), Nhwc{}, ek, s))) return rc;

@@ -6,8 +6,9 @@
 
 namespace vsc {
 
+// f1_c0 >= 0: also copies f1 (C channels) into `out` at channel f1_c0
 int launch_corr_nhwc(int ek, const void* const f1[2], const void* const f2[2], int cs1, int cs2, int C, int N, int H,
-                     int W, void* out, int ocs, hipStream_t s);
+                     int W, void* out, int ocs, int f1_c0, hipStream_t s);
 int launch_warp_nhwc(int ek, const void* const src[2], int scs, const float* flow, int fcs, float scale, void* out,
                      int ocs, int C, int N, int H, int W, hipStream_t s);
 int launch_upflow(int ek, const float* in, int N, int Hi, int Wi, const float* w, const float* b, float* out, void* cat,

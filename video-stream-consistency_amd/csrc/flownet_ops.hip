@@ -31,6 +31,7 @@ struct CorrArgs {
     const void* f1[2]; const void* f2[2];   // per batch item, channel 0 of pixel 0
     int cs1, cs2, C, H, W;
     void* out; int ocs;                      // out: channel 0 of pixel 0 of batch 0
+    int f1_c0;                               // >= 0: f1's C channels are also copied to out at this channel
 };
 
 template <int EK>
@@ -86,8 +87,14 @@ __global__ __launch_bounds__(512) void k_corr_nhwc(CorrArgs a)
             if (u < NA) {
                 const int p = u / KV, cv = u % KV, r = p / RP, x = p % RP;
                 const int gh = h0 + r, gw = w0 + x, ch = c0 + cv * VEC;
-                if (gh < a.H && gw < a.W && ch < a.C)
+                if (gh < a.H && gw < a.W && ch < a.C) {
                     v = *reinterpret_cast<const fo_u32x4*>(f1 + ((long long)gh * a.W + gw) * a.cs1 + ch);
+                    // the decoder's concatenation takes f1 beside the cost volume
+                    // (pwcnet.py:578): every block stages its own pixels' f1 once
+                    if (a.f1_c0 >= 0)
+                        *reinterpret_cast<fo_u32x4*>(static_cast<T*>(a.out) +
+                                                     (((long long)n * a.H + gh) * a.W + gw) * a.ocs + a.f1_c0 + ch) = v;
+                }
             } else if (u < NU) {
                 const int ub = u - NA, p = ub / KV, cv = ub % KV, r = p / BP, x = p % BP;
                 const int gh = h0 - HALO + r, gw = w0 - HALO + x, ch = c0 + cv * VEC;
@@ -159,11 +166,13 @@ __global__ __launch_bounds__(512) void k_corr_nhwc(CorrArgs a)
 }
 
 int launch_corr_nhwc(int ek, const void* const f1[2], const void* const f2[2], int cs1, int cs2, int C, int N,
-                     int H, int W, void* out, int ocs, hipStream_t s)
+                     int H, int W, void* out, int ocs, int f1_c0, hipStream_t s)
 {
     CorrArgs a{};
     for (int i = 0; i < N; ++i) { a.f1[i] = f1[i]; a.f2[i] = f2[i]; }
     a.cs1 = cs1; a.cs2 = cs2; a.C = C; a.H = H; a.W = W; a.out = out; a.ocs = ocs;
+    a.f1_c0 = f1_c0;
+    if (f1_c0 >= 0 && f1_c0 + C > ocs) return set_error(VSC_E_INVALID, "corr nhwc: f1 copy outside the pixel");
     dim3 grid((W + 31) / 32, (H + 3) / 4, N);
     if (ek == EK_BF16) hipLaunchKernelGGL(k_corr_nhwc<EK_BF16>, grid, dim3(512), 0, s, a);
     else hipLaunchKernelGGL(k_corr_nhwc<EK_F32>, grid, dim3(512), 0, s, a);
