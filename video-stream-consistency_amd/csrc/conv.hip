@@ -39,6 +39,7 @@ struct ConvArgs {
     int N, H, W, Cin;
     const void* wt; const float* bias;
     void* out; int out_cs, out_c0;
+    void* out2; int out2_cs, out2_c0;   // optional second copy of the result, in the input's element kind (Cout <= 16)
     const void* add; int add_cs, add_c0;
     const float* gate;
     int Ho, Wo, Cout;
@@ -244,12 +245,15 @@ __global__ __launch_bounds__(64 * NW) void k_conv3x3(ConvArgs a)
             }
             if (a.add) conv_load4<EKO>(a.add, pix * a.add_cs + a.add_c0 + co, v, nv);
             conv_store4<EKO>(a.out, pix * a.out_cs + a.out_c0 + co, v, nv);
+            // (compiled into the 16-channel tiles only: the flow heads use it)
+            if constexpr (NT == 16)
+                if (a.out2) conv_store4<EK>(a.out2, pix * a.out2_cs + a.out2_c0 + co, v, nv);
         }
     }
 }
 
 // sum of the split partials in part order, then the epilogue of k_conv3x3
-template <int EKO>
+template <int EK, int EKO>
 __global__ void k_conv_finalize(ConvArgs a, int parts)
 {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,6 +270,7 @@ __global__ void k_conv_finalize(ConvArgs a, int parts)
     using E = Elem<EKO>;
     if (a.add) x += E::ld(static_cast<const typename E::T*>(a.add)[pix * a.add_cs + a.add_c0 + co]);
     static_cast<typename E::T*>(a.out)[pix * a.out_cs + a.out_c0 + co] = E::st(x);
+    if (a.out2) static_cast<typename Elem<EK>::T*>(a.out2)[pix * a.out2_cs + a.out2_c0 + co] = Elem<EK>::st(x);
 }
 
 template <int EK, int EKO, int S, int NT, int MR, int NW>
@@ -302,7 +307,7 @@ static void conv_go(const ConvArgs& a, hipStream_t s)
     dim3 grid(b.tiles_x * b.tiles_y * ps * ps, b.ctiles * parts, a.N);
     hipLaunchKernelGGL((k_conv3x3<EK, EKO, S, NT, MR, NW>), grid, dim3(64 * NW), 0, s, b);
     if (parts > 1)
-        hipLaunchKernelGGL(k_conv_finalize<EKO>, dim3((unsigned)((stride + 255) / 256)), dim3(256), 0, s, b, parts);
+        hipLaunchKernelGGL((k_conv_finalize<EK, EKO>), dim3((unsigned)((stride + 255) / 256)), dim3(256), 0, s, b, parts);
 }
 
 // tile choice: the widest channel tile that the layer fills; 256-pixel tiles
@@ -357,6 +362,9 @@ int launch_conv3x3(const ConvDesc& d, hipStream_t s)
     const int Wo = (d.in.w + 2 * d.pad - 2 * d.dil - 1) / d.stride + 1;
     if (Ho != d.out.h || Wo != d.out.w || d.out.n != d.in.n || (d.add.p && (d.add.h != Ho || d.add.w != Wo)))
         return set_error(VSC_E_INVALID, "conv3x3: output size does not match");
+    if (d.out2.p && (Cout > 16 || d.out2.c != Cout || d.out2.c0 + Cout > d.out2.cs || d.out2.h != Ho ||
+                     d.out2.w != Wo || d.out2.n != d.in.n || ((d.out2.c0 % 4 || d.out2.cs % 4) && Cout >= 4)))
+        return set_error(VSC_E_INVALID, "conv3x3: bad second output slice (at most 16 channels)");
     if ((long long)d.in.n * d.in.h * d.in.w * d.in.cs >= (1LL << 31) ||
         (long long)d.out.n * Ho * Wo * d.out.cs >= (1LL << 31))
         return set_error(VSC_E_INVALID, "conv3x3: tensor too large");
@@ -366,6 +374,7 @@ int launch_conv3x3(const ConvDesc& d, hipStream_t s)
     a.N = d.in.n; a.H = d.in.h; a.W = d.in.w; a.Cin = d.in.c;
     a.wt = d.wt; a.bias = d.bias;
     a.out = d.out.p; a.out_cs = d.out.cs; a.out_c0 = d.out.c0;
+    a.out2 = d.out2.p; a.out2_cs = d.out2.cs; a.out2_c0 = d.out2.c0;
     a.add = d.add.p; a.add_cs = d.add.cs; a.add_c0 = d.add.c0;
     a.gate = d.gate;
     a.partial = d.partial;

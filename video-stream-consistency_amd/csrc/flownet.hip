@@ -275,13 +275,13 @@ int alloc_all(vsc_flownet* n)
 
 void* feat_ptr(vsc_flownet* n, int slot, int lvl) { return n->feat[(size_t)slot * 4 + (lvl - 3)]; }
 
-int conv(vsc_flownet* n, const Layer& l, Nhwc in, Nhwc out, Nhwc add, int ek_out, hipStream_t s)
+int conv(vsc_flownet* n, const Layer& l, Nhwc in, Nhwc out, Nhwc add, int ek_out, hipStream_t s, Nhwc out2 = Nhwc{})
 {
     // the device weights are [cout][3][3][cin_pad]: the slices must match them exactly
     if (in.c != l.cin_pad || out.c != l.cout || (add.p && add.c != l.cout))
         return set_error(VSC_E_INVALID, "flownet: layer slice does not match its weights");
     ConvDesc d{};
-    d.in = in; d.out = out; d.add = add;
+    d.in = in; d.out = out; d.add = add; d.out2 = out2;
     d.wt = l.dw; d.bias = l.db;
     d.stride = l.stride; d.dil = l.dil; d.pad = l.dil; d.act = l.act;
     d.ek_in = n->ek; d.ek_out = ek_out;
@@ -328,7 +328,7 @@ int decode(vsc_flownet* n, int np, const int* first, const int* second, hipStrea
         for (int i = 0; i < np; ++i) { f1[i] = feat_ptr(n, first[i], lvl); f2[i] = feat_ptr(n, second[i], lvl); }
         void* cat = n->cat0[lvl];
         if (lvl == 6) {
-            if ((rc = launch_zero_slice(cat, n->eb, cc, 88 + CV, 2, px * np, s))) return rc;
+            // level 6 has no upflow: its channels of cat0 stay as zeroed at create
             if ((rc = launch_corr_nhwc(ek, f1, f2, CV, CV, CV, np, h, w, cat, cc, 88, s))) return rc;
         } else {
             const Layer& up = n->L[n->up0 + (5 - lvl)];
@@ -347,17 +347,15 @@ int decode(vsc_flownet* n, int np, const int* first, const int* second, hipStrea
             if ((rc = conv(n, l[j], T(d, np, h, w, kDecInC[j], kDecC, kDecIn0[j]),
                            T(d, np, h, w, kDecOut[j], kDecC, kDecOut0[j]), Nhwc{}, ek, s)))
                 return rc;
+        // level 3: the flow also goes into dec (channel 448) as the refiner's input
         if ((rc = conv(n, l[5], T(d, np, h, w, kDecInC[5], kDecC, kDecIn0[5]), T(n->flow[lvl], np, h, w, 2, 2, 0),
-                       Nhwc{}, EK_F32, s)))
+                       Nhwc{}, EK_F32, s, lvl == 3 ? T(d, np, h, w, 2, kDecC, 448) : Nhwc{})))
             return rc;
         (void)C;
     }
     // refiner on cat([x5, flow]) of level 3; flow + refinement (pwcnet.py:981-986)
     const int h = n->Hp >> 3, w = n->Wp >> 3;
-    const long long px = (long long)h * w;
     void* d = n->dec[3];
-    const void* fl[2] = {n->flow[3], n->flow[3] + px * 2};
-    if ((rc = launch_copy_slice(EK_F32, ek, fl, np, 2, 0, d, kDecC, 448, 2, px, s))) return rc;
     const Layer* r = &n->L[n->ref0];
     Nhwc x = T(d, np, h, w, kRefInC, kDecC, kRefIn0);
     for (int j = 0; j < 6; ++j) {

@@ -13,9 +13,6 @@ int launch_warp_nhwc(int ek, const void* const src[2], int scs, const float* flo
                      int ocs, int C, int N, int H, int W, hipStream_t s);
 int launch_upflow(int ek, const float* in, int N, int Hi, int Wi, const float* w, const float* b, float* out, void* cat,
                   int ccs, int cc0, hipStream_t s);
-int launch_copy_slice(int eki, int eko, const void* const srcs[2], int N, int scs, int sc0, void* dst, int dcs, int dc0,
-                      int channels, long long npx, hipStream_t s);
-int launch_zero_slice(void* dst, int eb, int cs, int c0, int channels, long long npx, hipStream_t s);
 int launch_preprocess(int ek, const uint8_t* rgba, int W, int H, int Wd, int Hd, int Wp, int Hp, void* out, int ocs,
                       hipStream_t s);
 int launch_flow_final(const float* fl, int h, int w, int Hp, int Wp, float* out, int Wd, int Hd, int out_ch,

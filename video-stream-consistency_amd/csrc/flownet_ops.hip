@@ -322,39 +322,6 @@ int launch_upflow(int ek, const float* in, int N, int Hi, int Wi, const float* w
 }
 
 // ---------------------------------------------------------------------------
-// slice copies: `channels` channels from a source slice into a destination
-// slice (feature copy into the concatenation buffer; FP32 flow into a
-// 16-bit slot).  Scalar per element: these are the small tensors of the net.
-// ---------------------------------------------------------------------------
-struct PtrPair { const void* p[2]; };
-
-template <int EKI, int EKO>
-__global__ void k_copy_slice2(PtrPair srcs, int scs, int sc0, void* dst, int dcs, int dc0, int channels, long long npx)
-{
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = blockIdx.y;
-    if (e >= npx * channels) return;
-    const int c = (int)(e % channels);
-    const long long p = e / channels;
-    const auto* s = static_cast<const typename Elem<EKI>::T*>(srcs.p[n]);
-    auto* d = static_cast<typename Elem<EKO>::T*>(dst);
-    d[((long long)n * npx + p) * dcs + dc0 + c] = Elem<EKO>::st(Elem<EKI>::ld(s[p * scs + sc0 + c]));
-}
-
-int launch_copy_slice(int eki, int eko, const void* const srcs[2], int N, int scs, int sc0, void* dst, int dcs,
-                      int dc0, int channels, long long npx, hipStream_t s)
-{
-    PtrPair pp{};
-    for (int i = 0; i < N; ++i) pp.p[i] = srcs[i];
-    dim3 grid((unsigned)((npx * channels + 255) / 256), N);
-    if (eki == EK_F32 && eko == EK_F32) hipLaunchKernelGGL((k_copy_slice2<EK_F32, EK_F32>), grid, dim3(256), 0, s, pp, scs, sc0, dst, dcs, dc0, channels, npx);
-    else if (eki == EK_F32 && eko == EK_BF16) hipLaunchKernelGGL((k_copy_slice2<EK_F32, EK_BF16>), grid, dim3(256), 0, s, pp, scs, sc0, dst, dcs, dc0, channels, npx);
-    else if (eki == EK_BF16 && eko == EK_BF16) hipLaunchKernelGGL((k_copy_slice2<EK_BF16, EK_BF16>), grid, dim3(256), 0, s, pp, scs, sc0, dst, dcs, dc0, channels, npx);
-    else return set_error(VSC_E_DTYPE, "copy slice: unsupported kinds");
-    return check_launch("copy slice");
-}
-
-// ---------------------------------------------------------------------------
 // Input preprocessing (convert_onnx.py:63-71 with FLOWDOWNSCALE,
 // flowmodel.cpp:126-133): RGBA8 frame -> optional nearest downscale to
 // (Wd, Hd) (pixel centres: src = floor((dst + 0.5) * W / Wd)) -> /255 ->
@@ -443,22 +410,6 @@ __global__ void k_flow_final(const float* __restrict__ fl, int h, int w, int Hp,
         o[c] = v * (c == 0 ? su : sv);
     }
     if (out_ch == 3) o[2] = 0.0f;
-}
-
-__global__ void k_zero_slice(void* dst, int eb, int cs, int c0, int channels, long long npx)
-{
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= npx * channels) return;
-    const long long off = (e / channels) * cs + c0 + e % channels;
-    if (eb == 4) static_cast<float*>(dst)[off] = 0.0f;
-    else static_cast<uint16_t*>(dst)[off] = 0;
-}
-
-int launch_zero_slice(void* dst, int eb, int cs, int c0, int channels, long long npx, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_zero_slice, dim3((unsigned)((npx * channels + 255) / 256)), dim3(256), 0, s, dst, eb, cs, c0,
-                       channels, npx);
-    return check_launch("zero slice");
 }
 
 int launch_flow_final(const float* fl, int h, int w, int Hp, int Wp, float* out, int Wd, int Hd, int out_ch,

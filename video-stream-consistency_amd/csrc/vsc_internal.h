@@ -303,6 +303,7 @@ struct Nhwc {
 // add (optional, same element kind as out) is added after the activation.
 struct ConvDesc {
     Nhwc in, out, add;   // add.p == nullptr: no residual
+    Nhwc out2;           // out2.p == nullptr: none; else a second copy of the result in the input's element kind (Cout <= 16)
     const void* wt;      // [Cout][3][3][in.c], element kind of in
     const float* bias;   // [Cout] or nullptr
     const float* gate;   // nullptr, or a device scalar multiplying the activated result before `add`
