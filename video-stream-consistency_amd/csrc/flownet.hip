@@ -331,11 +331,10 @@ int decode(vsc_flownet* n, int np, const int* first, const int* second, hipStrea
             // level 6 has no upflow: its channels of cat0 stay as zeroed at create
             if ((rc = launch_corr_nhwc(ek, f1, f2, CV, CV, CV, np, h, w, cat, cc, 88, s))) return rc;
         } else {
+            // upflow of the coarser level's flow, computed inside the warp of f2 by it
             const Layer& up = n->L[n->up0 + (5 - lvl)];
-            if ((rc = launch_upflow(ek, n->flow[lvl + 1], np, h / 2, w / 2, static_cast<const float*>(up.dw), up.db,
-                                    n->upflow[lvl], cat, cc, 88 + CV, s)))
-                return rc;
-            if ((rc = launch_warp_nhwc(ek, f2, CV, n->upflow[lvl], 2, kDbl[lvl], n->warped, CV, CV, np, h, w, s))) return rc;
+            const UpflowArgs ua{n->flow[lvl + 1], static_cast<const float*>(up.dw), up.db, n->upflow[lvl], cat, cc, 88 + CV};
+            if ((rc = launch_warp_nhwc(ek, f2, CV, nullptr, 2, kDbl[lvl], n->warped, CV, CV, np, h, w, s, &ua))) return rc;
             const void* wp[2] = {n->warped, static_cast<const char*>(n->warped) + px * CV * n->eb};
             if ((rc = launch_corr_nhwc(ek, f1, wp, CV, CV, CV, np, h, w, cat, cc, 88, s))) return rc;
         }

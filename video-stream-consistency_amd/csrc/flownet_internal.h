@@ -9,10 +9,17 @@ namespace vsc {
 // f1_c0 >= 0: also copies f1 (C channels) into `out` at channel f1_c0
 int launch_corr_nhwc(int ek, const void* const f1[2], const void* const f2[2], int cs1, int cs2, int C, int N, int H,
                      int W, void* out, int ocs, int f1_c0, hipStream_t s);
+// upflow fused into the warp (moduleUpflow, pwcnet.py:472, then Backward)
+struct UpflowArgs {
+    const float* in;           // coarser flow, [N][H/2][W/2][2] FP32
+    const float* w;            // ConvTranspose2d weights [2][2][4][4]
+    const float* b;            // bias [2]
+    float* out;                // upsampled flow, [N][H][W][2] FP32
+    void* cat; int ccs, cc0;   // its copy in the concatenation buffer (element kind)
+};
+// up: nullptr, or the warp flow is computed from up->in (flow/fcs unused)
 int launch_warp_nhwc(int ek, const void* const src[2], int scs, const float* flow, int fcs, float scale, void* out,
-                     int ocs, int C, int N, int H, int W, hipStream_t s);
-int launch_upflow(int ek, const float* in, int N, int Hi, int Wi, const float* w, const float* b, float* out, void* cat,
-                  int ccs, int cc0, hipStream_t s);
+                     int ocs, int C, int N, int H, int W, hipStream_t s, const UpflowArgs* up = nullptr);
 int launch_preprocess(int ek, const uint8_t* rgba, int W, int H, int Wd, int Hd, int Wp, int Hp, void* out, int ocs,
                       hipStream_t s);
 int launch_flow_final(const float* fl, int h, int w, int Hp, int Wp, float* out, int Wd, int Hd, int out_ch,

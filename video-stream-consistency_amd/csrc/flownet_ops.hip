@@ -180,6 +180,34 @@ int launch_corr_nhwc(int ek, const void* const f1[2], const void* const f2[2], i
 }
 
 // ---------------------------------------------------------------------------
+// moduleUpflow: ConvTranspose2d(2, 2, kernel 4, stride 2, padding 1)
+// (pwcnet.py:472): out[co][oy][ox] = b[co] + sum in[ci][iy][ix] * w[ci][co][ky][kx]
+// over oy = 2*iy - 1 + ky, ox = 2*ix - 1 + kx, for one output pixel of item n
+// (FP32 flow in, 2 channels).
+// ---------------------------------------------------------------------------
+__device__ inline void upflow_px(const float* __restrict__ in, int Hi, int Wi, const float* __restrict__ w,
+                                 const float* __restrict__ b, int n, int oy, int ox, float acc[2])
+{
+    acc[0] = b[0];
+    acc[1] = b[1];
+    for (int ky = 0; ky < 4; ++ky) {
+        const int ty = oy + 1 - ky;
+        if (ty < 0 || (ty & 1)) continue;
+        const int iy = ty >> 1;
+        if (iy >= Hi) continue;
+        for (int kx = 0; kx < 4; ++kx) {
+            const int tx = ox + 1 - kx;
+            if (tx < 0 || (tx & 1)) continue;
+            const int ix = tx >> 1;
+            if (ix >= Wi) continue;
+            const float* s = in + (((long long)n * Hi + iy) * Wi + ix) * 2;
+            for (int co = 0; co < 2; ++co)
+                acc[co] = acc[co] + (s[0] * w[(0 * 2 + co) * 16 + ky * 4 + kx] + s[1] * w[(1 * 2 + co) * 16 + ky * 4 + kx]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Backward warp of f2 by flow * scale (Backward(): pwcnet.py:575-576 through
 // the Warp op, warp.cc:71-134): bilinear taps, per-corner bounds masks, the
 // sample kept only where the in-bounds weight mass passes 0.999 (the FP32
@@ -191,6 +219,11 @@ struct WarpArgs {
     float scale;
     void* out; int ocs;                     // warped features
     int C, H, W, groups;
+    // with up_in: the flow is the upflow of up_in (H/2 x W/2, computed here
+    // per pixel), and channel group 0 also writes it to up_out (FP32, 2
+    // channels) and, in the element kind, to cat at channel cc0
+    const float* up_in; const float* up_w; const float* up_b;
+    float* up_out; void* cat; int ccs, cc0;
 };
 
 template <int EK>
@@ -206,8 +239,23 @@ __global__ __launch_bounds__(256) void k_warp_nhwc(WarpArgs a)
     const int g = (int)(e % a.groups);
     const long long p = e / a.groups;
     const int x = (int)(p % a.W), y = (int)(p / a.W);
-    const float* fl = a.flow + ((long long)n * npx + p) * a.fcs;
-    const float fx = fl[0] * a.scale, fy = fl[1] * a.scale;
+    float fv[2];
+    if (a.up_in) {
+        upflow_px(a.up_in, a.H / 2, a.W / 2, a.up_w, a.up_b, n, y, x, fv);
+        if (g == 0) {
+            float* fo = a.up_out + ((long long)n * npx + p) * 2;
+            fo[0] = fv[0];
+            fo[1] = fv[1];
+            T* c = static_cast<T*>(a.cat) + ((long long)n * npx + p) * a.ccs + a.cc0;
+            c[0] = E::st(fv[0]);
+            c[1] = E::st(fv[1]);
+        }
+    } else {
+        const float* fl = a.flow + ((long long)n * npx + p) * a.fcs;
+        fv[0] = fl[0];
+        fv[1] = fl[1];
+    }
+    const float fx = fv[0] * a.scale, fy = fv[1] * a.scale;
     const float xf = (float)x + fx, yf = (float)y + fy;
     const float xL = floorf(xf), yT = floorf(yf);
     const float al = xf - xL, be = yf - yT;
@@ -258,12 +306,17 @@ __global__ __launch_bounds__(256) void k_warp_nhwc(WarpArgs a)
 }
 
 int launch_warp_nhwc(int ek, const void* const src[2], int scs, const float* flow, int fcs, float scale, void* out,
-                     int ocs, int C, int N, int H, int W, hipStream_t s)
+                     int ocs, int C, int N, int H, int W, hipStream_t s, const UpflowArgs* up)
 {
     WarpArgs a{};
     for (int i = 0; i < N; ++i) a.src[i] = src[i];
     a.scs = scs; a.flow = flow; a.fcs = fcs; a.scale = scale; a.out = out; a.ocs = ocs;
     a.C = C; a.H = H; a.W = W;
+    if (up) {
+        if ((H & 1) || (W & 1)) return set_error(VSC_E_INVALID, "warp nhwc: upflow needs even sizes");
+        a.up_in = up->in; a.up_w = up->w; a.up_b = up->b; a.up_out = up->out;
+        a.cat = up->cat; a.ccs = up->ccs; a.cc0 = up->cc0;
+    }
     const int vec = ek == EK_F32 ? 4 : 8;
     a.groups = (C + vec - 1) / vec;
     const long long units = (long long)H * W * a.groups;
@@ -271,54 +324,6 @@ int launch_warp_nhwc(int ek, const void* const src[2], int scs, const float* flo
     if (ek == EK_BF16) hipLaunchKernelGGL(k_warp_nhwc<EK_BF16>, grid, dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_warp_nhwc<EK_F32>, grid, dim3(256), 0, s, a);
     return check_launch("warp nhwc");
-}
-
-// ---------------------------------------------------------------------------
-// moduleUpflow: ConvTranspose2d(2, 2, kernel 4, stride 2, padding 1)
-// (pwcnet.py:472): out[co][oy][ox] = b[co] + sum in[ci][iy][ix] * w[ci][co][ky][kx]
-// over oy = 2*iy - 1 + ky, ox = 2*ix - 1 + kx.  FP32 flow in, FP32 flow out
-// (2 channels), plus a copy in the element kind of the concatenation buffer.
-// ---------------------------------------------------------------------------
-template <int EK>
-__global__ void k_upflow(const float* __restrict__ in, int Hi, int Wi, const float* __restrict__ w,
-                         const float* __restrict__ b, float* __restrict__ out, void* cat, int ccs, int cc0, int N)
-{
-    using E = Elem<EK>;
-    const int Ho = 2 * Hi, Wo = 2 * Wi;
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)N * Ho * Wo) return;
-    const int ox = (int)(e % Wo), oy = (int)((e / Wo) % Ho), n = (int)(e / ((long long)Wo * Ho));
-    float acc[2] = {b[0], b[1]};
-    for (int ky = 0; ky < 4; ++ky) {
-        const int ty = oy + 1 - ky;
-        if (ty < 0 || (ty & 1)) continue;
-        const int iy = ty >> 1;
-        if (iy >= Hi) continue;
-        for (int kx = 0; kx < 4; ++kx) {
-            const int tx = ox + 1 - kx;
-            if (tx < 0 || (tx & 1)) continue;
-            const int ix = tx >> 1;
-            if (ix >= Wi) continue;
-            const float* s = in + (((long long)n * Hi + iy) * Wi + ix) * 2;
-            for (int co = 0; co < 2; ++co)
-                acc[co] = acc[co] + (s[0] * w[(0 * 2 + co) * 16 + ky * 4 + kx] + s[1] * w[(1 * 2 + co) * 16 + ky * 4 + kx]);
-        }
-    }
-    out[e * 2] = acc[0];
-    out[e * 2 + 1] = acc[1];
-    typename E::T* c = static_cast<typename E::T*>(cat) + e * ccs + cc0;
-    c[0] = E::st(acc[0]);
-    c[1] = E::st(acc[1]);
-}
-
-int launch_upflow(int ek, const float* in, int N, int Hi, int Wi, const float* w, const float* b, float* out,
-                  void* cat, int ccs, int cc0, hipStream_t s)
-{
-    const long long n = (long long)N * 4 * Hi * Wi;
-    const unsigned blocks = (unsigned)((n + 255) / 256);
-    if (ek == EK_BF16) hipLaunchKernelGGL(k_upflow<EK_BF16>, dim3(blocks), dim3(256), 0, s, in, Hi, Wi, w, b, out, cat, ccs, cc0, N);
-    else hipLaunchKernelGGL(k_upflow<EK_F32>, dim3(blocks), dim3(256), 0, s, in, Hi, Wi, w, b, out, cat, ccs, cc0, N);
-    return check_launch("upflow");
 }
 
 // ---------------------------------------------------------------------------
