@@ -509,10 +509,9 @@ extern "C" int vsc_flownet_flow(vsc_flownet* n, int npairs, const int* first, co
     int rc = decode(n, npairs, first, second, s);
     if (rc) return rc;
     const int h = n->Hp >> 3, w = n->Wp >> 3;
-    for (int i = 0; i < npairs; ++i)
-        if ((rc = launch_flow_final(n->refined + (long long)i * h * w * 2, h, w, n->Hp, n->Wp, out[i], n->Wd, n->Hd,
-                                    out_channels, 0, s)))
-            return rc;
+    float* const outs[2] = {out[0], npairs > 1 ? out[1] : nullptr};
+    if ((rc = launch_flow_final(n->refined, npairs, h, w, n->Hp, n->Wp, outs, n->Wd, n->Hd, out_channels, 0, s)))
+        return rc;
     n->last_pairs = npairs;
     return VSC_OK;
 }

@@ -22,7 +22,8 @@ int launch_warp_nhwc(int ek, const void* const src[2], int scs, const float* flo
                      int ocs, int C, int N, int H, int W, hipStream_t s, const UpflowArgs* up = nullptr);
 int launch_preprocess(int ek, const uint8_t* rgba, int W, int H, int Wd, int Hd, int Wp, int Hp, void* out, int ocs,
                       hipStream_t s);
-int launch_flow_final(const float* fl, int h, int w, int Hp, int Wp, float* out, int Wd, int Hd, int out_ch,
-                      int swap_scale, hipStream_t s);
+// fl: N (1 or 2) level-3 flows [N][h][w][2]; out[i]: the i-th flow at Wd x Hd
+int launch_flow_final(const float* fl, int N, int h, int w, int Hp, int Wp, float* const out[2], int Wd, int Hd,
+                      int out_ch, int swap_scale, hipStream_t s);
 
 }  // namespace vsc

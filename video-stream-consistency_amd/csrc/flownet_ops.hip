@@ -385,11 +385,16 @@ int launch_preprocess(int ek, const uint8_t* rgba, int W, int H, int Wd, int Hd,
 // channel 0 scaled by Hd/Hp and channel 1 by Wd/Wp (the wrapper's order),
 // written HWC with out_ch (2 or 3; the third is 0) channels.
 // ---------------------------------------------------------------------------
-__global__ void k_flow_final(const float* __restrict__ fl, int h, int w, int Hp, int Wp, float* out, int Wd, int Hd,
-                             int out_ch, int swap_scale)
+struct OutPair { float* p[2]; };
+
+// blockIdx.z: the pair (flow item) of the batch
+__global__ void k_flow_final(const float* __restrict__ flows, int h, int w, int Hp, int Wp, OutPair outs, int Wd,
+                             int Hd, int out_ch, int swap_scale)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= Wd || y >= Hd) return;
+    const float* fl = flows + (long long)blockIdx.z * h * w * 2;
+    float* out = outs.p[blockIdx.z];
     int x0, x1, y0, y1;
     float lx, ly;
     resize_tap(x, Wp, Wd, x0, x1, lx);
@@ -417,11 +422,14 @@ __global__ void k_flow_final(const float* __restrict__ fl, int h, int w, int Hp,
     if (out_ch == 3) o[2] = 0.0f;
 }
 
-int launch_flow_final(const float* fl, int h, int w, int Hp, int Wp, float* out, int Wd, int Hd, int out_ch,
-                      int swap_scale, hipStream_t s)
+int launch_flow_final(const float* fl, int N, int h, int w, int Hp, int Wp, float* const out[2], int Wd, int Hd,
+                      int out_ch, int swap_scale, hipStream_t s)
 {
-    dim3 grid((Wd + 255) / 256, Hd);
-    hipLaunchKernelGGL(k_flow_final, grid, dim3(256), 0, s, fl, h, w, Hp, Wp, out, Wd, Hd, out_ch, swap_scale);
+    if (N < 1 || N > 2) return set_error(VSC_E_INVALID, "flow final: one or two flows");
+    OutPair o{};
+    for (int i = 0; i < N; ++i) o.p[i] = out[i];
+    dim3 grid((Wd + 255) / 256, Hd, N);
+    hipLaunchKernelGGL(k_flow_final, grid, dim3(256), 0, s, fl, h, w, Hp, Wp, o, Wd, Hd, out_ch, swap_scale);
     return check_launch("flow final");
 }
 
