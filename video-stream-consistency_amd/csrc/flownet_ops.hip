@@ -1,6 +1,7 @@
 // Channels-last (NHWC) kernels of the flow network besides the convolutions:
-// the cost volume, the backward warp, the flow upsampling (ConvTranspose2d),
-// input preprocessing and the final flow resize.  FP32 arithmetic throughout;
+// the cost volume, the backward warp (with the flow upsampling, the
+// ConvTranspose2d moduleUpflow, computed inside it), input preprocessing and
+// the final flow resize.  FP32 arithmetic throughout;
 // tensors are stored as FP32 or BF16 (the element kind of the network form).
 #include "vsc_internal.h"
 #include "flownet_internal.h"
@@ -15,7 +16,8 @@ typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t fo_u32x4
 // Cost volume (Correlation, md = 4, non-legacy: pwcnet.py:337-355 with the
 // op of correlation.cc:203-275) on MFMA, fused with LeakyReLU(0.1)
 // (pwcnet.py:568,577), written into the decoder's concatenation buffer as
-// channels [0, 81) of each pixel, with channels [81, 88) zeroed.
+// channels [0, 81) of each pixel, with channels [81, 88) zeroed; the f1
+// features it stages are copied beside them (channel f1_c0 on).
 //
 // For output row h and 16 pixels w0..w0+15, G[i][j] = sum_c f1[h][w0+i][c] *
 // f2[h+dy][w0-4+j][c] (j < 24, two 16x16 MFMA tiles) holds every horizontal
