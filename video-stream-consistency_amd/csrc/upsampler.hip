@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 namespace vsc {
@@ -303,108 +304,116 @@ __global__ __launch_bounds__(512) void k_ups_fused(const float* __restrict__ lo,
     const int offx0 = (t0 / 3) * UF_XW + t0 % 3, offx1 = (t1 / 3) * UF_XW + t1 % 3, offx8 = 2 * UF_XW + 2;
     __syncthreads();
 
-    // input tile: (u, v) = bilinear x scale, luma; zero outside the frame
-    for (int e = tid; e < UF_XH * UF_XW; e += 64 * NW) {
-        const int r = e / UF_XW, c = e % UF_XW;
-        const int iy = y0 - 3 + r, ix = x0 - 3 + c;
-        uint2 v = make_uint2(0u, 0u);
-        if (iy >= 0 && iy < H && ix >= 0 && ix < W) {
-            // bilinear_sample's operation order, both channels from 8-byte loads
-            const int a0 = tx0[c], a1 = tx1[c], r0 = ty0[r] * lw, r1 = ty1[r] * lw;
-            const float fx = tfx[c], fy = tfy[r];
-            float2 v00, v10, v01, v11;
-            if (lch == 2) {
-                const float2* f2 = reinterpret_cast<const float2*>(lo);
-                v00 = f2[r0 + a0]; v10 = f2[r0 + a1]; v01 = f2[r1 + a0]; v11 = f2[r1 + a1];
-            } else {
-                v00 = make_float2(lo[(r0 + a0) * lch], lo[(r0 + a0) * lch + 1]);
-                v10 = make_float2(lo[(r0 + a1) * lch], lo[(r0 + a1) * lch + 1]);
-                v01 = make_float2(lo[(r1 + a0) * lch], lo[(r1 + a0) * lch + 1]);
-                v11 = make_float2(lo[(r1 + a1) * lch], lo[(r1 + a1) * lch + 1]);
+    // Tiles whose input tile (with its 3-pixel halo) lies inside the frame
+    // skip every per-pixel frame test (block-uniform choice).
+    const bool interior = y0 >= 3 && y0 + UF_TH + 3 <= H && x0 >= 3 && x0 + UF_TW + 3 <= W;
+    auto run = [&](auto interior_c) {
+        constexpr bool INTERIOR = decltype(interior_c)::value;
+        // input tile: (u, v) = bilinear x scale, luma; zero outside the frame
+        for (int e = tid; e < UF_XH * UF_XW; e += 64 * NW) {
+            const int r = e / UF_XW, c = e % UF_XW;
+            const int iy = y0 - 3 + r, ix = x0 - 3 + c;
+            uint2 v = make_uint2(0u, 0u);
+            if (INTERIOR || (iy >= 0 && iy < H && ix >= 0 && ix < W)) {
+                // bilinear_sample's operation order, both channels from 8-byte loads
+                const int a0 = tx0[c], a1 = tx1[c], r0 = ty0[r] * lw, r1 = ty1[r] * lw;
+                const float fx = tfx[c], fy = tfy[r];
+                float2 v00, v10, v01, v11;
+                if (lch == 2) {
+                    const float2* f2 = reinterpret_cast<const float2*>(lo);
+                    v00 = f2[r0 + a0]; v10 = f2[r0 + a1]; v01 = f2[r1 + a0]; v11 = f2[r1 + a1];
+                } else {
+                    v00 = make_float2(lo[(r0 + a0) * lch], lo[(r0 + a0) * lch + 1]);
+                    v10 = make_float2(lo[(r0 + a1) * lch], lo[(r0 + a1) * lch + 1]);
+                    v01 = make_float2(lo[(r1 + a0) * lch], lo[(r1 + a0) * lch + 1]);
+                    v11 = make_float2(lo[(r1 + a1) * lch], lo[(r1 + a1) * lch + 1]);
+                }
+                const float gx = 1.0f - fx, gy = 1.0f - fy;
+                const float u = ((((v00.x * gx) * gy + (v10.x * fx) * gy) + (v01.x * gx) * fy) + (v11.x * fx) * fy) * scale;
+                const float w = ((((v00.y * gx) * gy + (v10.y * fx) * gy) + (v01.y * gx) * fy) + (v11.y * fx) * fy) * scale;
+                const uint8_t* px = rgba + ((long long)iy * W + ix) * 4;
+                const float luma = (0.299f * (float)px[0] + 0.587f * (float)px[1] + 0.114f * (float)px[2]) * (1.0f / 255.0f);
+                v = make_uint2(pack_h2(u, w), pack_h2(luma, 0.0f));
+                if (r >= 3 && r < 3 + UF_TH && c >= 3 && c < 3 + UF_TW) sbase[(r - 3) * UF_TW + c - 3] = make_float2(u, w);
             }
-            const float gx = 1.0f - fx, gy = 1.0f - fy;
-            const float u = ((((v00.x * gx) * gy + (v10.x * fx) * gy) + (v01.x * gx) * fy) + (v11.x * fx) * fy) * scale;
-            const float w = ((((v00.y * gx) * gy + (v10.y * fx) * gy) + (v01.y * gx) * fy) + (v11.y * fx) * fy) * scale;
-            const uint8_t* px = rgba + ((long long)iy * W + ix) * 4;
-            const float luma = (0.299f * (float)px[0] + 0.587f * (float)px[1] + 0.114f * (float)px[2]) * (1.0f / 255.0f);
-            v = make_uint2(pack_h2(u, w), pack_h2(luma, 0.0f));
-            if (r >= 3 && r < 3 + UF_TH && c >= 3 && c < 3 + UF_TW) sbase[(r - 3) * UF_TW + c - 3] = make_float2(u, w);
+            sx[e] = v;
         }
-        sx[e] = v;
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // layer 1 on the 20 x 36 tile, ReLU, zero outside the frame
-    constexpr int N1 = UF_1H * UF_1W, G1 = (N1 + 15) / 16;
-    auto l1 = [&](int p) {
-        const int bx = (p / UF_1W) * UF_XW + p % UF_1W;
-        const uint2 a = sx[bx + offx0], b = sx[bx + offx1];
-        const uint2 e = lg == 0 ? sx[bx + offx8] : make_uint2(0u, 0u);
-        up_f32x4 acc = up_mma(w1[0], up_u32x4{a.x, a.y, b.x, b.y}, up_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-        return up_mma(w1[1], up_u32x4{e.x, e.y, 0u, 0u}, acc);
-    };
-    auto put1 = [&](int g, const up_f32x4& acc) {
-        const int p = 16 * g + l16;
-        if (p >= N1) return;
-        const int r = p / UF_1W, c = p % UF_1W, iy = y0 - 2 + r, ix = x0 - 2 + c;
-        const bool in = iy >= 0 && iy < H && ix >= 0 && ix < W;
-        float o[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) o[k] = in ? fmaxf(acc[k] + b1[k], 0.0f) : 0.0f;
-        *reinterpret_cast<uint2*>(&s1[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
-    };
-    for (int g = q; g < G1; g += 2 * NW) {
-        const int g2 = g + NW;
-        const up_f32x4 a0 = l1(min(16 * g + l16, N1 - 1));
-        const up_f32x4 a1 = l1(min(16 * g2 + l16, N1 - 1));
-        put1(g, a0);
-        if (g2 < G1) put1(g2, a1);
-    }
-    __syncthreads();
+        // layer 1 on the 20 x 36 tile, ReLU, zero outside the frame
+        constexpr int N1 = UF_1H * UF_1W, G1 = (N1 + 15) / 16;
+        auto l1 = [&](int p) {
+            const int bx = (p / UF_1W) * UF_XW + p % UF_1W;
+            const uint2 a = sx[bx + offx0], b = sx[bx + offx1];
+            const uint2 e = lg == 0 ? sx[bx + offx8] : make_uint2(0u, 0u);
+            up_f32x4 acc = up_mma(w1[0], up_u32x4{a.x, a.y, b.x, b.y}, up_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            return up_mma(w1[1], up_u32x4{e.x, e.y, 0u, 0u}, acc);
+        };
+        auto put1 = [&](int g, const up_f32x4& acc) {
+            const int p = 16 * g + l16;
+            if (p >= N1) return;
+            const int r = p / UF_1W, c = p % UF_1W, iy = y0 - 2 + r, ix = x0 - 2 + c;
+            const bool in = INTERIOR || (iy >= 0 && iy < H && ix >= 0 && ix < W);
+            float o[4];
+    #pragma unroll
+            for (int k = 0; k < 4; ++k) o[k] = in ? fmaxf(acc[k] + b1[k], 0.0f) : 0.0f;
+            *reinterpret_cast<uint2*>(&s1[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
+        };
+        for (int g = q; g < G1; g += 2 * NW) {
+            const int g2 = g + NW;
+            const up_f32x4 a0 = l1(min(16 * g + l16, N1 - 1));
+            const up_f32x4 a1 = l1(min(16 * g2 + l16, N1 - 1));
+            put1(g, a0);
+            if (g2 < G1) put1(g2, a1);
+        }
+        __syncthreads();
 
-    // layer 2 on the 18 x 34 tile
-    constexpr int N2 = UF_2H * UF_2W, G2 = (N2 + 15) / 16;
-    auto put2 = [&](int g, const up_f32x4& acc) {
-        const int p = 16 * g + l16;
-        if (p >= N2) return;
-        const int r = p / UF_2W, c = p % UF_2W, iy = y0 - 1 + r, ix = x0 - 1 + c;
-        const bool in = iy >= 0 && iy < H && ix >= 0 && ix < W;
-        float o[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) o[k] = in ? fmaxf(acc[k] + b2[k], 0.0f) : 0.0f;
-        *reinterpret_cast<uint2*>(&s2[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
-    };
-    auto base2 = [&](int p) { return ((p / UF_2W) * UF_1W + p % UF_2W) * UF_AS; };
-    for (int g = q; g < G2; g += 2 * NW) {
-        const int g2 = g + NW;
-        const up_f32x4 a0 = ups_l16(s1, w2, base2(min(16 * g + l16, N2 - 1)), off1);
-        const up_f32x4 a1 = ups_l16(s1, w2, base2(min(16 * g2 + l16, N2 - 1)), off1);
-        put2(g, a0);
-        if (g2 < G2) put2(g2, a1);
-    }
-    __syncthreads();
+        // layer 2 on the 18 x 34 tile
+        constexpr int N2 = UF_2H * UF_2W, G2 = (N2 + 15) / 16;
+        auto put2 = [&](int g, const up_f32x4& acc) {
+            const int p = 16 * g + l16;
+            if (p >= N2) return;
+            const int r = p / UF_2W, c = p % UF_2W, iy = y0 - 1 + r, ix = x0 - 1 + c;
+            const bool in = INTERIOR || (iy >= 0 && iy < H && ix >= 0 && ix < W);
+            float o[4];
+    #pragma unroll
+            for (int k = 0; k < 4; ++k) o[k] = in ? fmaxf(acc[k] + b2[k], 0.0f) : 0.0f;
+            *reinterpret_cast<uint2*>(&s2[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
+        };
+        auto base2 = [&](int p) { return ((p / UF_2W) * UF_1W + p % UF_2W) * UF_AS; };
+        for (int g = q; g < G2; g += 2 * NW) {
+            const int g2 = g + NW;
+            const up_f32x4 a0 = ups_l16(s1, w2, base2(min(16 * g + l16, N2 - 1)), off1);
+            const up_f32x4 a1 = ups_l16(s1, w2, base2(min(16 * g2 + l16, N2 - 1)), off1);
+            put2(g, a0);
+            if (g2 < G2) put2(g2, a1);
+        }
+        __syncthreads();
 
-    // layer 3 on the 16 x 32 output tile, added to the FP32 bilinear flow
-    const float gv = gate ? *gate : 1.0f;
-    constexpr int G3 = UF_TH * UF_TW / 16;
-    auto put3 = [&](int g, const up_f32x4& acc) {
-        const int p = 16 * g + l16, r = p / UF_TW, c = p % UF_TW, iy = y0 + r, ix = x0 + c;
-        if (lg == 0 && iy < H && ix < W) {
-            const float2 bs = sbase[p];
-            float* o = out + ((long long)iy * W + ix) * out_ch;
-            o[0] = bs.x + gv * (acc[0] + b3[0]);
-            o[1] = bs.y + gv * (acc[1] + b3[1]);
-            if (out_ch == 3) o[2] = 0.0f;
+        // layer 3 on the 16 x 32 output tile, added to the FP32 bilinear flow
+        const float gv = gate ? *gate : 1.0f;
+        constexpr int G3 = UF_TH * UF_TW / 16;
+        auto put3 = [&](int g, const up_f32x4& acc) {
+            const int p = 16 * g + l16, r = p / UF_TW, c = p % UF_TW, iy = y0 + r, ix = x0 + c;
+            if (lg == 0 && (INTERIOR || (iy < H && ix < W))) {
+                const float2 bs = sbase[p];
+                float* o = out + ((long long)iy * W + ix) * out_ch;
+                o[0] = bs.x + gv * (acc[0] + b3[0]);
+                o[1] = bs.y + gv * (acc[1] + b3[1]);
+                if (out_ch == 3) o[2] = 0.0f;
+            }
+        };
+        auto base3 = [&](int p) { return ((p / UF_TW) * UF_2W + p % UF_TW) * UF_AS; };
+        for (int g = q; g < G3; g += 2 * NW) {
+            const int g2 = g + NW;
+            const up_f32x4 a0 = ups_l16(s2, w3, base3(16 * g + l16), off2);
+            const up_f32x4 a1 = ups_l16(s2, w3, base3(16 * min(g2, G3 - 1) + l16), off2);
+            put3(g, a0);
+            if (g2 < G3) put3(g2, a1);
         }
     };
-    auto base3 = [&](int p) { return ((p / UF_TW) * UF_2W + p % UF_TW) * UF_AS; };
-    for (int g = q; g < G3; g += 2 * NW) {
-        const int g2 = g + NW;
-        const up_f32x4 a0 = ups_l16(s2, w3, base3(16 * g + l16), off2);
-        const up_f32x4 a1 = ups_l16(s2, w3, base3(16 * min(g2, G3 - 1) + l16), off2);
-        put3(g, a0);
-        if (g2 < G3) put3(g2, a1);
-    }
+    if (interior) run(std::true_type{});
+    else run(std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
