@@ -27,7 +27,6 @@ struct vsc_pipeline {
     vsc_stream* stab = nullptr;
     float* lo[2][2] = {};                          // [frame & 1][fwd, bwd] flows at the flow resolution (HWC 2)
     float* hi[2][2] = {};                          // [frame & 1][fwd, bwd] at frame size when upsampling here
-    void* ws = nullptr;                            // upsampler workspace
     uint32_t* count = nullptr;                     // [frame & 1] inconsistent-pixel counts (64 B apart)
     float* gate = nullptr;                         // [frame & 1] fallback gates (64 B apart)
     long long slot_frame[3] = {-1, -1, -1};        // frame index held by each feature slot
@@ -126,7 +125,7 @@ extern "C" size_t vsc_pipeline_footprint(int width, int height, const vsc_pipeli
     t += vsc_stream_footprint(width, height, here ? width : Wd, here ? height : Hd);
     t += 4 * (size_t)Wd * Hd * 2 * 4;
     if (here) t += 4 * (size_t)width * height * 2 * 4;
-    if (learned(*cfg)) t += vsc_upsampler_workspace_size(width, height) + 4 * 3202 * 4 + 512;
+    if (learned(*cfg)) t += 4 * 3202 * 4 + 512;
     return t + 64;
 }
 
@@ -171,7 +170,6 @@ extern "C" int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg
         rc = VSC_E_HIP;
     if (!rc && learned(*cfg)) {
         rc = vsc_upsampler_create(cfg->seed, &p->ups);
-        if (!rc) rc = dalloc(&p->ws, vsc_upsampler_workspace_size(width, height));
     }
     if (rc) {
         vsc_pipeline_destroy(p);
@@ -190,7 +188,7 @@ extern "C" int vsc_pipeline_destroy(vsc_pipeline* p)
     if (p->ups) vsc_upsampler_destroy(p->ups);
     if (p->stab) vsc_stream_destroy(p->stab);
     void* bufs[] = {p->lo[0][0], p->lo[0][1], p->lo[1][0], p->lo[1][1], p->hi[0][0], p->hi[0][1],
-                    p->hi[1][0], p->hi[1][1], p->ws, p->count, p->gate};
+                    p->hi[1][0], p->hi[1][1], p->count, p->gate};
     for (void* b : bufs) (void)hipFree(b);
     for (hipEvent_t e : {p->ev_in, p->ev_f, p->ev_s[0], p->ev_s[1]})
         if (e) (void)hipEventDestroy(e);
@@ -309,16 +307,14 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
     }
     // upsampling to the frame, on the stabilization stream (it reads only this
     // frame's buffer set)
-    const vsc_hip_stream ssv = reinterpret_cast<vsc_hip_stream>(ss);
     const float* ff = p->lo[k][0];
     const float* fb = p->lo[k][1];
     if (learned(p->cfg)) {
         // the learned upsampler is trained on full-resolution vectors: always scaled
         const float sc = (float)f;
-        if ((rc = vsc_upsampler_run(p->ups, p->lo[k][0], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[k][0], 2,
-                                    gate, p->ws, ssv)) ||
-            (rc = vsc_upsampler_run(p->ups, p->lo[k][1], p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, p->hi[k][1], 2,
-                                    gate, p->ws, ssv)))
+        const float* const lo[2] = {p->lo[k][0], p->lo[k][1]};
+        float* const hi[2] = {p->hi[k][0], p->hi[k][1]};
+        if ((rc = upsampler_run_flows(p->ups, 2, lo, p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, hi, 2, gate, ss)))
             return rc;
         ff = p->hi[k][0];
         fb = p->hi[k][1];

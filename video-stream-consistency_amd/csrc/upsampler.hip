@@ -250,12 +250,16 @@ __device__ inline up_f32x4 ups_l16(const uint16_t* src, const up_u32x4 (&w)[5], 
     return acc;
 }
 
-__global__ __launch_bounds__(512) void k_ups_fused(const float* __restrict__ lo, int lw, int lh, int lch,
+// blockIdx.y selects the flow (up to two flows of one frame per launch)
+struct UpsFlows { const float* lo[2]; float* out[2]; };
+
+__global__ __launch_bounds__(512) void k_ups_fused(UpsFlows flows, int lw, int lh, int lch,
                                                    const uint8_t* __restrict__ rgba, int W, int H, float scale,
                                                    const uint16_t* __restrict__ wf, const float* __restrict__ prm,
-                                                   const float* gate, float* __restrict__ out, int out_ch,
-                                                   int tiles_x)
+                                                   const float* gate, int out_ch, int tiles_x)
 {
+    const float* __restrict__ lo = flows.lo[blockIdx.y];
+    float* __restrict__ out = flows.out[blockIdx.y];
     constexpr int NW = 8;   // waves per block; each keeps two 16-pixel groups in flight
     __shared__ __attribute__((aligned(16))) uint2 sx[UF_XH * UF_XW];
     __shared__ __attribute__((aligned(16))) uint16_t s1[UF_1H * UF_1W * UF_AS];
@@ -700,19 +704,37 @@ extern "C" size_t vsc_upsampler_workspace_size(int width, int height)
     return 0;
 }
 
+namespace vsc {
+
+int upsampler_run_flows(vsc_upsampler* u, int n, const float* const flow_lo[2], int lo_w, int lo_h, int lo_ch,
+                        const uint8_t* rgba, int width, int height, float scale, float* const flow_out[2], int out_ch,
+                        const float* gate, hipStream_t s)
+{
+    if (!u || !rgba || n < 1 || n > 2) return set_error(VSC_E_INVALID, "vsc_upsampler_run: NULL argument");
+    for (int i = 0; i < n; ++i)
+        if (!flow_lo[i] || !flow_out[i]) return set_error(VSC_E_INVALID, "vsc_upsampler_run: NULL argument");
+    if (lo_w <= 0 || lo_h <= 0 || width <= 0 || height <= 0 || (lo_ch != 2 && lo_ch != 3) ||
+        (out_ch != 2 && out_ch != 3))
+        return set_error(VSC_E_INVALID, "vsc_upsampler_run: bad shape");
+    UpsFlows f{};
+    for (int i = 0; i < n; ++i) { f.lo[i] = flow_lo[i]; f.out[i] = flow_out[i]; }
+    const int tx = (width + UF_TW - 1) / UF_TW, ty = (height + UF_TH - 1) / UF_TH;
+    hipLaunchKernelGGL(k_ups_fused, dim3(tx * ty, n), dim3(512), 0, s, f, lo_w, lo_h, lo_ch, rgba, width, height,
+                       scale, u->w16, u->p, gate, out_ch, tx);
+    return check_launch("upsampler fused");
+}
+
+}  // namespace vsc
+
 extern "C" int vsc_upsampler_run(vsc_upsampler* u, const float* flow_lo, int lo_w, int lo_h, int lo_ch,
                                  const uint8_t* rgba, int width, int height, float scale, float* flow_out, int out_ch,
                                  const float* gate, void* workspace, vsc_hip_stream stream)
 {
     (void)workspace;
-    if (!u || !flow_lo || !rgba || !flow_out) return set_error(VSC_E_INVALID, "vsc_upsampler_run: NULL argument");
-    if (lo_w <= 0 || lo_h <= 0 || width <= 0 || height <= 0 || (lo_ch != 2 && lo_ch != 3) ||
-        (out_ch != 2 && out_ch != 3))
-        return set_error(VSC_E_INVALID, "vsc_upsampler_run: bad shape");
-    const int tx = (width + UF_TW - 1) / UF_TW, ty = (height + UF_TH - 1) / UF_TH;
-    hipLaunchKernelGGL(k_ups_fused, dim3(tx * ty), dim3(512), 0, as_stream(stream), flow_lo, lo_w, lo_h, lo_ch, rgba,
-                       width, height, scale, u->w16, u->p, gate, flow_out, out_ch, tx);
-    return check_launch("upsampler fused");
+    const float* const lo[2] = {flow_lo, nullptr};
+    float* const out[2] = {flow_out, nullptr};
+    return upsampler_run_flows(u, 1, lo, lo_w, lo_h, lo_ch, rgba, width, height, scale, out, out_ch, gate,
+                               as_stream(stream));
 }
 
 extern "C" int vsc_upsampler_loss_grad(vsc_upsampler* u, int batch, const float* flow_lo, int lo_w, int lo_h,

@@ -314,6 +314,12 @@ struct ConvDesc {
 };
 int launch_conv3x3(const ConvDesc& d, hipStream_t s);
 
+// the fused learned upsampler on n (1 or 2) flows of one frame in one launch
+// (upsampler.hip; vsc_upsampler_run is the one-flow case)
+int upsampler_run_flows(vsc_upsampler* u, int n, const float* const flow_lo[2], int lo_w, int lo_h, int lo_ch,
+                        const uint8_t* rgba, int width, int height, float scale, float* const flow_out[2], int out_ch,
+                        const float* gate, hipStream_t s);
+
 // host-state step of a pipeline frame replayed from a graph (pipeline.hip)
 int pipeline_advance(vsc_pipeline* p, long long frame);
 
