@@ -45,6 +45,11 @@ CASES = [
     (1, 50, 66, 32, 16, 1, 16, 1),
     (1, 30, 50, 160, 2, 1, 1, 0),
     (1, 136, 240, 256, 96, 1, 1, 1),
+    # 16-bit layers with 8 / 16 input channels stage narrow chunks and pack
+    # 4 / 2 taps into one MFMA K-step (stride 2 and dilation included)
+    (1, 37, 41, 8, 32, 2, 1, 1),
+    (1, 30, 50, 16, 16, 1, 2, 1),
+    (2, 19, 45, 8, 16, 1, 1, 2),
 ]
 
 

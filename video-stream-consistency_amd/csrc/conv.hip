@@ -9,7 +9,8 @@
 // A block of NW waves owns TH = NW*MR output rows x 16 output columns and NT
 // output channels; wave q computes rows q*MR .. q*MR+MR-1, i.e. MR pixel tiles
 // of 16 against NT/16 channel tiles.  Input channels stream through LDS in chunks of
-// KC (32 for 16-bit, 16 for FP32): the input patch of the block (TH+2 x 18
+// KC (32 for 16-bit, 16 for FP32; 8 or 16 for 16-bit layers that narrow, whose
+// MFMA K-steps then hold 4 or 2 taps): the input patch of the block (TH+2 x 18
 // pixels at stride 1, 2*TH+1 x 33 at stride 2) and the weights of the chunk
 // for all nine taps; the nine taps then read shifted windows of the patch.
 // Dilated layers (dilation d > 1) are tiled on the d x d sub-grids of equal
@@ -98,13 +99,13 @@ __device__ inline void conv_load4(const void* src, long long off, float v[4], in
     for (int r = 0; r < 4; ++r) v[r] += (r < nvalid) ? E::ld(p[r]) : 0.0f;
 }
 
-template <int EK, int EKO, int S, int NT, int MR, int NW>
+template <int EK, int EKO, int S, int NT, int MR, int NW, int KCX>
 __global__ __launch_bounds__(64 * NW) void k_conv3x3(ConvArgs a)
 {
     using E = Elem<EK>;
     using T = typename E::T;
     constexpr int NTH = 64 * NW;
-    constexpr int VEC = E::VEC, KC = ConvK<EK>::KC, KV = KC / VEC;
+    constexpr int VEC = E::VEC, KC = KCX, KV = KC / VEC;
     constexpr int TH = NW * MR, TW = 16;
     constexpr int PR = (TH - 1) * S + 3, PC = (TW - 1) * S + 3;
     constexpr int ROW = KC + VEC;                     // LDS pixel stride (elements): 80 bytes
@@ -174,31 +175,47 @@ __global__ __launch_bounds__(64 * NW) void k_conv3x3(ConvArgs a)
         }
         __syncthreads();
     };
-    // the nine taps with the next tap's fragments read from LDS while this
-    // tap's MFMAs issue (two register sets)
+    // The MFMA's K (32 for 16-bit, 16 for FP32: four lane groups of CPL
+    // channels) covers TPK taps of a chunk narrower than K: lane group lg takes
+    // tap step*TPK + lg/LPT, channels (lg % LPT)*CPL.. of it (a 16-bit layer
+    // with 8 or 16 input channels runs 3 or 5 K-steps instead of 9 mostly
+    // zero ones).  Each step's fragments are read from LDS while the previous
+    // step's MFMAs issue (two register sets).
+    constexpr int KK = EK == EK_F32 ? 16 : 32;
+    constexpr int TPK = KK / KC, LPT = 4 / TPK, CPL = KC / LPT;
+    constexpr int NSTEP = (9 + TPK - 1) / TPK;
+    static_assert(KK % KC == 0 && CPL * (int)sizeof(T) == 16, "a lane group reads 16 bytes of one tap");
     auto compute = [&]() {
         cv_u32x4 wf[2][NT / 16], xf[2][MR];
-        auto frag = [&](int tap, cv_u32x4 (&w)[NT / 16], cv_u32x4 (&x)[MR]) {
+        auto frag = [&](int step, cv_u32x4 (&w)[NT / 16], cv_u32x4 (&x)[MR]) {
+            const int tap = step * TPK + lg / LPT, coff = (lg % LPT) * CPL;
+            if (TPK > 1 && tap >= 9) {   // padding of the last K-step
+#pragma unroll
+                for (int i = 0; i < NT / 16; ++i) w[i] = cv_u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int m = 0; m < MR; ++m) x[m] = cv_u32x4{0u, 0u, 0u, 0u};
+                return;
+            }
             const int ky = tap / 3, kx = tap % 3;
 #pragma unroll
             for (int i = 0; i < NT / 16; ++i)
-                w[i] = *reinterpret_cast<const cv_u32x4*>(&sW[((i * 16 + l16) * 9 + tap) * ROW + lg * (KC / 4)]);
+                w[i] = *reinterpret_cast<const cv_u32x4*>(&sW[((i * 16 + l16) * 9 + tap) * ROW + coff]);
 #pragma unroll
             for (int m = 0; m < MR; ++m) {
                 const int pr = (q * MR + m) * S + ky, pc = l16 * S + kx;
-                x[m] = *reinterpret_cast<const cv_u32x4*>(&sP[(pr * PC + pc) * ROW + lg * (KC / 4)]);
+                x[m] = *reinterpret_cast<const cv_u32x4*>(&sP[(pr * PC + pc) * ROW + coff]);
             }
         };
         frag(0, wf[0], xf[0]);
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            if (tap + 1 < 9) frag(tap + 1, wf[(tap + 1) & 1], xf[(tap + 1) & 1]);
-            // keep the reads of the next tap ahead of this tap's MFMAs
+        for (int step = 0; step < NSTEP; ++step) {
+            if (step + 1 < NSTEP) frag(step + 1, wf[(step + 1) & 1], xf[(step + 1) & 1]);
+            // keep the reads of the next step ahead of this step's MFMAs
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < NT / 16; ++i)
 #pragma unroll
-                for (int m = 0; m < MR; ++m) acc[i][m] = conv_mfma<EK>(wf[tap & 1][i], xf[tap & 1][m], acc[i][m]);
+                for (int m = 0; m < MR; ++m) acc[i][m] = conv_mfma<EK>(wf[step & 1][i], xf[step & 1][m], acc[i][m]);
             __builtin_amdgcn_sched_barrier(0);
         }
     };
@@ -273,10 +290,10 @@ __global__ void k_conv_finalize(ConvArgs a, int parts)
     if (a.out2) static_cast<typename Elem<EK>::T*>(a.out2)[pix * a.out2_cs + a.out2_c0 + co] = Elem<EK>::st(x);
 }
 
-template <int EK, int EKO, int S, int NT, int MR, int NW>
+template <int EK, int EKO, int S, int NT, int MR, int NW, int KCX>
 static void conv_go(const ConvArgs& a, hipStream_t s)
 {
-    constexpr int KC = ConvK<EK>::KC;
+    constexpr int KC = KCX;
     const int ps = a.pstep;
     ConvArgs b = a;
     const int hs = (a.Ho + ps - 1) / ps, ws = (a.Wo + ps - 1) / ps;
@@ -305,7 +322,7 @@ static void conv_go(const ConvArgs& a, hipStream_t s)
     b.partial = parts > 1 ? a.partial : nullptr;
     b.part_stride = stride;
     dim3 grid(b.tiles_x * b.tiles_y * ps * ps, b.ctiles * parts, a.N);
-    hipLaunchKernelGGL((k_conv3x3<EK, EKO, S, NT, MR, NW>), grid, dim3(64 * NW), 0, s, b);
+    hipLaunchKernelGGL((k_conv3x3<EK, EKO, S, NT, MR, NW, KCX>), grid, dim3(64 * NW), 0, s, b);
     if (parts > 1)
         hipLaunchKernelGGL((k_conv_finalize<EK, EKO>), dim3((unsigned)((stride + 255) / 256)), dim3(256), 0, s, b, parts);
 }
@@ -315,6 +332,14 @@ static void conv_go(const ConvArgs& a, hipStream_t s)
 // from half a block per CU (16-bit: its MFMAs are 16x faster, so the weights
 // every block re-reads dominate, and halving that traffic pays for the idle
 // CUs of a smaller grid); else 64-pixel tiles
+template <int EK, int EKO, int S, int NT, int KCX>
+static void conv_mr(const ConvArgs& a, int mr, hipStream_t s)
+{
+    if (mr == 4) conv_go<EK, EKO, S, NT, (S == 1 ? 4 : 2), 4, KCX>(a, s);
+    else if (mr == 2) conv_go<EK, EKO, S, NT, 2, 4, KCX>(a, s);
+    else conv_go<EK, EKO, S, NT, 1, 4, KCX>(a, s);
+}
+
 template <int EK, int EKO, int S>
 static void conv_pick(const ConvArgs& a, hipStream_t s)
 {
@@ -325,14 +350,26 @@ static void conv_pick(const ConvArgs& a, hipStream_t s)
     auto blocks = [&](int mr) { return (px / (64LL * mr)) * ntiles; };
     int mr = (S == 1 && blocks(4) >= 512) ? 4 : (blocks(2) >= (EK == EK_F32 ? 512 : 128) ? 2 : 1);
     if (S == 2 && mr > 2) mr = 2;
-#define VSC_CONV_NT(NTV)                                                   \
-    if (mr == 4) conv_go<EK, EKO, S, NTV, (S == 1 ? 4 : 2), 4>(a, s);      \
-    else if (mr == 2) conv_go<EK, EKO, S, NTV, 2, 4>(a, s);                \
-    else conv_go<EK, EKO, S, NTV, 1, 4>(a, s);
-    if (NTsel == 64) { VSC_CONV_NT(64) }
-    else if (NTsel == 32) { VSC_CONV_NT(32) }
-    else { VSC_CONV_NT(16) }
-#undef VSC_CONV_NT
+    constexpr int KC = ConvK<EK>::KC;
+    if (NTsel == 64) {
+        conv_mr<EK, EKO, S, 64, KC>(a, mr, s);
+    } else if constexpr (EK != EK_F32) {
+        // 16-bit layers with 8 or 16 input channels stage chunks of that width
+        // (the image and the first feature levels)
+        if (NTsel == 32) {
+            if (a.Cin <= 8) conv_mr<EK, EKO, S, 32, 8>(a, mr, s);
+            else if (a.Cin <= 16) conv_mr<EK, EKO, S, 32, 16>(a, mr, s);
+            else conv_mr<EK, EKO, S, 32, KC>(a, mr, s);
+        } else {
+            if (a.Cin <= 8) conv_mr<EK, EKO, S, 16, 8>(a, mr, s);
+            else if (a.Cin <= 16) conv_mr<EK, EKO, S, 16, 16>(a, mr, s);
+            else conv_mr<EK, EKO, S, 16, KC>(a, mr, s);
+        }
+    } else if (NTsel == 32) {
+        conv_mr<EK, EKO, S, 32, KC>(a, mr, s);
+    } else {
+        conv_mr<EK, EKO, S, 16, KC>(a, mr, s);
+    }
 }
 
 template <int EK, int EKO>
