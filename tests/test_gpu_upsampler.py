@@ -46,9 +46,12 @@ def test_upsampler_gradients_match_oracle(vsc, cuda):
         o += n
 
 
-def test_upsampler_inference_matches_oracle(vsc, cuda):
+@pytest.mark.parametrize("size", [64, 160])
+def test_upsampler_inference_matches_oracle(vsc, cuda, size):
+    """64: every 16 x 32 tile (with its 3-pixel halo) touches the frame edge;
+    160: most tiles lie inside and take the kernel's path without frame tests."""
     import torch
-    ut, (fr, gt, lo) = _data(vsc, 2, 64, seed=5)
+    ut, (fr, gt, lo) = _data(vsc, 2, size, seed=5)
     p = _random_params(vsc, 2)
     ups = vsc.Upsampler(seed=0, params=p)
     p16 = p.astype(np.float16).astype(np.float32)
