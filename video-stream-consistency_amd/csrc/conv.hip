@@ -327,11 +327,6 @@ static void conv_go(const ConvArgs& a, hipStream_t s)
         hipLaunchKernelGGL((k_conv_finalize<EK, EKO>), dim3((unsigned)((stride + 255) / 256)), dim3(256), 0, s, b, parts);
 }
 
-// tile choice: the widest channel tile that the layer fills; 256-pixel tiles
-// from two blocks per CU; 128-pixel tiles from two blocks per CU (FP32) or
-// from half a block per CU (16-bit: its MFMAs are 16x faster, so the weights
-// every block re-reads dominate, and halving that traffic pays for the idle
-// CUs of a smaller grid); else 64-pixel tiles
 template <int EK, int EKO, int S, int NT, int KCX>
 static void conv_mr(const ConvArgs& a, int mr, hipStream_t s)
 {
@@ -340,22 +335,33 @@ static void conv_mr(const ConvArgs& a, int mr, hipStream_t s)
     else conv_go<EK, EKO, S, NT, 1, 4, KCX>(a, s);
 }
 
+// Tile choice.  Channel tile: FP32 layers take the widest tile they fill (64,
+// 32 or 16 output channels); 16-bit layers take 32 (16 when the layer has at
+// most 16): twice the blocks of 64-channel tiles on the small grids of the
+// coarse levels and half the weights staged per block (1080p frames: 556 vs
+// 521 frames/s on one stream, 670 vs 656 frame-pipelined).  Pixel tile:
+// 256 pixels from two blocks per CU; 128 pixels from two blocks per CU (FP32)
+// or from half a block per CU (16-bit: its MFMAs are 16x faster, so the weights
+// every block re-reads dominate, and halving that traffic pays for the idle
+// CUs of a smaller grid); else 64 pixels.
 template <int EK, int EKO, int S>
 static void conv_pick(const ConvArgs& a, hipStream_t s)
 {
     const int ps = a.pstep;
     const long long px = (long long)((a.Ho + ps - 1) / ps) * ps * ((a.Wo + ps - 1) / ps) * ps * a.N;
-    const int NTsel = a.Cout > 32 ? 64 : (a.Cout > 16 ? 32 : 16);
+    const int NTsel = EK == EK_F32 ? (a.Cout > 32 ? 64 : (a.Cout > 16 ? 32 : 16)) : (a.Cout > 16 ? 32 : 16);
     const long long ntiles = (a.Cout + NTsel - 1) / NTsel;
     auto blocks = [&](int mr) { return (px / (64LL * mr)) * ntiles; };
     int mr = (S == 1 && blocks(4) >= 512) ? 4 : (blocks(2) >= (EK == EK_F32 ? 512 : 128) ? 2 : 1);
     if (S == 2 && mr > 2) mr = 2;
     constexpr int KC = ConvK<EK>::KC;
-    if (NTsel == 64) {
-        conv_mr<EK, EKO, S, 64, KC>(a, mr, s);
-    } else if constexpr (EK != EK_F32) {
-        // 16-bit layers with 8 or 16 input channels stage chunks of that width
-        // (the image and the first feature levels)
+    if constexpr (EK == EK_F32) {
+        if (NTsel == 64) conv_mr<EK, EKO, S, 64, KC>(a, mr, s);
+        else if (NTsel == 32) conv_mr<EK, EKO, S, 32, KC>(a, mr, s);
+        else conv_mr<EK, EKO, S, 16, KC>(a, mr, s);
+    } else {
+        // layers with 8 or 16 input channels (the image and the first feature
+        // level) stage chunks of that width
         if (NTsel == 32) {
             if (a.Cin <= 8) conv_mr<EK, EKO, S, 32, 8>(a, mr, s);
             else if (a.Cin <= 16) conv_mr<EK, EKO, S, 32, 16>(a, mr, s);
@@ -365,10 +371,6 @@ static void conv_pick(const ConvArgs& a, hipStream_t s)
             else if (a.Cin <= 16) conv_mr<EK, EKO, S, 16, 16>(a, mr, s);
             else conv_mr<EK, EKO, S, 16, KC>(a, mr, s);
         }
-    } else if (NTsel == 32) {
-        conv_mr<EK, EKO, S, 32, KC>(a, mr, s);
-    } else {
-        conv_mr<EK, EKO, S, 16, KC>(a, mr, s);
     }
 }
 
