@@ -80,6 +80,12 @@ __global__ __launch_bounds__(256) void k_bilinear(View in, View out)
     for (int c = 0; c < out.ch; ++c) out.wr(x, y, c, bilinear_sample(in, t, c));
 }
 
+// planar frame with rows of a multiple of 4 pixels at a 16-byte aligned base
+static bool planar4(const View& v)
+{
+    return v.ps == 1 && v.cs == (long long)v.w * v.h && v.w % 4 == 0 && reinterpret_cast<uintptr_t>(v.d) % 16 == 0;
+}
+
 int launch_bilinear(View in, View out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_bilinear, grid2d(out.w, out.h), dim3(BX, BY), 0, s, in, out);
@@ -144,9 +150,29 @@ __global__ __launch_bounds__(256) void k_to_rgba(View in, uint8_t* out)
     reinterpret_cast<uchar4*>(out)[(long long)y * in.w + x] = v;
 }
 
+// Planar frames whose rows are multiples of 4 pixels: 4 pixels per thread,
+// 16-byte loads and stores (the same conversion as k_to_rgba).
+__global__ __launch_bounds__(256) void k_to_rgba_planar4(View in, uint8_t* out)
+{
+    const int x0 = 4 * (blockIdx.x * 64 + threadIdx.x % 64);
+    const int y = blockIdx.y * 4 + threadIdx.x / 64;
+    if (x0 >= in.w || y >= in.h) return;
+    const float4 r = *reinterpret_cast<const float4*>(in.d + in.idx(x0, y, 0));
+    const float4 g = *reinterpret_cast<const float4*>(in.d + in.idx(x0, y, 1));
+    const float4 b = *reinterpret_cast<const float4*>(in.d + in.idx(x0, y, 2));
+    auto px = [](float cr, float cg, float cb) {
+        return (uint32_t)to_u8(cr) | ((uint32_t)to_u8(cg) << 8) | ((uint32_t)to_u8(cb) << 16) | (1u << 24);
+    };
+    const uint4 v = make_uint4(px(r.x, g.x, b.x), px(r.y, g.y, b.y), px(r.z, g.z, b.z), px(r.w, g.w, b.w));
+    *reinterpret_cast<uint4*>(out + ((long long)y * in.w + x0) * 4) = v;
+}
+
 int launch_to_rgba(View in, uint8_t* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_to_rgba, grid2d(in.w, in.h), dim3(BX, BY), 0, s, in, out);
+    if (planar4(in) && reinterpret_cast<uintptr_t>(out) % 16 == 0)
+        hipLaunchKernelGGL(k_to_rgba_planar4, dim3((in.w / 4 + 63) / 64, (in.h + 3) / 4), dim3(256), 0, s, in, out);
+    else
+        hipLaunchKernelGGL(k_to_rgba, grid2d(in.w, in.h), dim3(BX, BY), 0, s, in, out);
     return check_launch("float_to_rgba");
 }
 
