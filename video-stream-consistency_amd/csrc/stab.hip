@@ -241,8 +241,12 @@ int launch_consist_sweep(View P, View A, View Wt, View Oin, View Oout, View U, f
 // Arithmetic per pixel and sweep is that of k_consist_sweep; lapP (the
 // Laplacian of the processed frame, a constant of the level) is an input,
 // computed once per level with the same operations, so the result is identical.
-// Waves whose 64 x R pixels all have four neighbours take a branch-free path;
-// the others evaluate the reference's neighbour conditions (flowconsistency.cu:215-236).
+// The reference's neighbour conditions (flowconsistency.cu:215-236) cost no
+// per-pixel work where they are uniform: a column whose left (right) neighbour
+// is excluded reads a zero column from LDS instead (the neighbour sum gains a
+// +0 term, which changes no nonzero sum), so only waves whose strip meets the
+// image's top or bottom rows test rows per pixel, and waves wholly outside the
+// image (never stored, never read by an image pixel) skip the arithmetic.
 // ---------------------------------------------------------------------------
 static constexpr int SB_TW = 128;
 
@@ -252,11 +256,13 @@ static constexpr int SB_TW = 128;
 // one register; the kernel alternates the two orders, and a descending sweep
 // shifts them back, so a pair of sweeps needs no register copies.  Each
 // pixel's arithmetic (and its operation order) is the same in both orders.
-template <int R, bool MOM, bool INTERIOR, bool REV>
+// colL / colR hold the left / right neighbours, or zeros where the column has
+// none (cntc = number of horizontal neighbours); ROWMASK tests the rows.
+template <int R, bool MOM, bool ROWMASK, bool REV>
 __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const float (&a)[R], const float (&w)[R],
                                             const float (&lp)[R], const float* colL, const float* colR,
-                                            float up_edge, float dn_edge, float step, float mom, bool hr, bool hl,
-                                            int rU, int rD)
+                                            float up_edge, float dn_edge, float step, float mom, int cntc, int rU,
+                                            int rD)
 {
     constexpr int NK = R / 4;
     float4 l4 = *reinterpret_cast<const float4*>(colL + (REV ? R - 4 : 0));
@@ -280,21 +286,17 @@ __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const 
             const float cur = o[r];
             const float up = REV ? (r > 0 ? o[r - 1] : up_edge) : prev;
             const float dn = REV ? prev : (r < R - 1 ? o[r + 1] : dn_edge);
-            float lo;
-            if (INTERIOR) {
-                lo = rv[j] + lv[j];
-                lo += dn;
-                lo += up;
-                lo -= 4.0f * cur;
-            } else {
-                int cnt = 0;
-                lo = 0.0f;
-                if (hr) { lo += rv[j]; cnt += 1; }
-                if (hl) { lo += lv[j]; cnt += 1; }
+            float lo = rv[j] + lv[j];
+            int cnt = cntc + 2;
+            if (ROWMASK) {
+                cnt = cntc;
                 if (r <= rD) { lo += dn; cnt += 1; }
                 if (r >= rU) { lo += up; cnt += 1; }
-                lo -= (float)cnt * cur;
+            } else {
+                lo += dn;
+                lo += up;
             }
+            lo -= (float)cnt * cur;
             const float t1 = w[r] * (cur - a[r]);
             const float t2 = lo - lp[r];
             const float g = t2 - t1;
@@ -312,11 +314,10 @@ __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const 
 // per-pixel constants, and per pixel
 //   X = fma(-q, O, sum of neighbours); U' = fma(step, X, -kk); O' = O + U' (+ mom*U)
 // (7 VALU operations in the interior instead of 13).
-template <int R, bool MOM, bool INTERIOR, bool REV>
+template <int R, bool MOM, bool ROWMASK, bool REV>
 __device__ __forceinline__ void sweep_strip_fma(float (&o)[R], float (&u)[R], const float (&q)[R],
                                                 const float (&kk)[R], const float* colL, const float* colR,
-                                                float up_edge, float dn_edge, float step, float mom, bool hr, bool hl,
-                                                int rU, int rD)
+                                                float up_edge, float dn_edge, float step, float mom, int rU, int rD)
 {
     constexpr int NK = R / 4;
     float4 l4 = *reinterpret_cast<const float4*>(colL + (REV ? R - 4 : 0));
@@ -340,17 +341,13 @@ __device__ __forceinline__ void sweep_strip_fma(float (&o)[R], float (&u)[R], co
             const float cur = o[r];
             const float up = REV ? (r > 0 ? o[r - 1] : up_edge) : prev;
             const float dn = REV ? prev : (r < R - 1 ? o[r + 1] : dn_edge);
-            float sum;
-            if (INTERIOR) {
-                sum = rv[j] + lv[j];
-                sum += dn;
-                sum += up;
-            } else {
-                sum = 0.0f;
-                if (hr) sum += rv[j];
-                if (hl) sum += lv[j];
+            float sum = rv[j] + lv[j];
+            if (ROWMASK) {
                 if (r <= rD) sum += dn;
                 if (r >= rU) sum += up;
+            } else {
+                sum += dn;
+                sum += up;
             }
             const float X = __builtin_fmaf(-q[r], cur, sum);
             const float sg = __builtin_fmaf(step, X, -kk[r]);
@@ -375,7 +372,8 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     constexpr int RS = ((R / 4) % 2 == 0) ? R + 4 : R + 8;
     const int OW = TW - 2 * halo, OH = TH - 2 * halo;
     static_assert(R % 4 == 0 && (RS / 4) % 2 == 1, "strip stride must be an odd number of 16-byte groups");
-    __shared__ __attribute__((aligned(16))) float sO[2][STRIPS][TW][RS];
+    // column TW of every plane and strip holds zeros: the excluded neighbour
+    __shared__ __attribute__((aligned(16))) float sO[2][STRIPS][TW + 1][RS];
 
     // XCD-aware tile order: blocks are dispatched to the 8 XCDs round robin
     // (block b runs on XCD b % 8), so give XCD x the contiguous run of tiles
@@ -407,9 +405,11 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     const bool colin = gx >= 0 && gx < W;
     const bool hr = (gx + 1) < (W - 1), hl = (gx - 1) >= 0;
     const int gys = gy0 + ty0;
-    // every pixel of this wave (64 columns x R rows) has all four neighbours
-    const bool interior = (gx0 + wave_tx0 >= 1) && (gx0 + wave_tx0 + 63 <= W - 3) && (gys >= 1) &&
-                          (gys + R - 1 <= H - 3);
+    // every row of this wave's strip has its up and down neighbours
+    const bool rows_in = (gys >= 1) && (gys + R - 1 <= H - 3);
+    // no pixel of this wave (64 columns x R rows) lies in the image
+    const bool wave_out = (gx0 + wave_tx0 >= W) || (gx0 + wave_tx0 + 63 < 0) || (gys >= H) || (gys + R - 1 < 0);
+    const int cntc = (hr ? 1 : 0) + (hl ? 1 : 0);
 
     // FMA: w holds q = cnt + w, lp holds kk; a is unused (and not loaded)
     float o[R], u[R], a[FMA ? 1 : R], w[R], lp[R];
@@ -437,8 +437,11 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
         for (int r = 0; r < R; ++r) u[r] = 0.0f;
     }
 
-    const int txl = tx > 0 ? tx - 1 : 0;
-    const int txr = tx < TW - 1 ? tx + 1 : TW - 1;
+    // zero columns, published by the first sweep's barrier
+    for (int i = threadIdx.x; i < 2 * STRIPS * RS; i += TW * STRIPS) sO[i / (STRIPS * RS)][(i / RS) % STRIPS][TW][i % RS] = 0.0f;
+    // neighbour columns (clamped at the tile edge, where the lane is halo)
+    const int txl = !hl ? TW : tx > 0 ? tx - 1 : 0;
+    const int txr = !hr ? TW : tx < TW - 1 ? tx + 1 : TW - 1;
     const int stU = strip > 0 ? strip - 1 : 0;
     const int stD = strip < STRIPS - 1 ? strip + 1 : STRIPS - 1;
     // row r has an up neighbour iff r >= rU and a down neighbour iff r <= rD
@@ -446,10 +449,10 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     const int rD0 = (H - 3) - gys;
 
     // One sweep: publish this strip's O, then update it from the neighbours.
-    auto sweep = [&](int s, auto mom_c, auto rev_c, auto interior_c) {
+    auto sweep = [&](int s, auto mom_c, auto rev_c, auto rowmask_c) {
         constexpr bool MOM = decltype(mom_c)::value, REV = decltype(rev_c)::value;
-        constexpr bool INTERIOR = decltype(interior_c)::value;
-        float (*buf)[TW][RS] = sO[s & 1];
+        constexpr bool ROWMASK = decltype(rowmask_c)::value;
+        float (*buf)[TW + 1][RS] = sO[s & 1];
         float* own = buf[strip][tx];
 #pragma unroll
         for (int k = 0; k < R / 4; ++k)
@@ -464,28 +467,34 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
         // predicates out of the sweep loop into scalar registers
         asm volatile("" : "+s"(rU), "+s"(rD));
         if constexpr (FMA)
-            sweep_strip_fma<R, MOM, INTERIOR, REV>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+            sweep_strip_fma<R, MOM, ROWMASK, REV>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, rU, rD);
         else
-            sweep_strip<R, MOM, INTERIOR, REV>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, hr, hl, rU, rD);
+            sweep_strip<R, MOM, ROWMASK, REV>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, cntc, rU, rD);
     };
     // The level's first sweep has no momentum term; after it, ascending and
     // descending sweeps alternate in pairs.
-    auto run = [&](auto interior_c) {
+    auto run = [&](auto rowmask_c) {
         const std::true_type yes{};
         const std::false_type no{};
         int s = 0;
         if (first_sweep == 0) {
-            sweep(0, no, no, interior_c);
+            sweep(0, no, no, rowmask_c);
             s = 1;
         }
         for (; s + 1 < sweeps; s += 2) {
-            sweep(s, yes, no, interior_c);
-            sweep(s + 1, yes, yes, interior_c);
+            sweep(s, yes, no, rowmask_c);
+            sweep(s + 1, yes, yes, rowmask_c);
         }
-        if (s < sweeps) sweep(s, yes, no, interior_c);
+        if (s < sweeps) sweep(s, yes, no, rowmask_c);
     };
-    if (interior) run(std::true_type{});
-    else run(std::false_type{});
+    // Every wave passes one barrier per sweep, whichever branch it takes.
+    if (wave_out) {
+        for (int s = 0; s < sweeps; ++s) __syncthreads();
+    } else if (rows_in) {
+        run(std::false_type{});
+    } else {
+        run(std::true_type{});
+    }
 
     if (tx >= halo && tx < TW - halo && colin) {
         const int wbase = gys * W + gx;
