@@ -162,7 +162,7 @@ def test_frame_params(vsc, orc, cuda, params, mode):
         assert np.array_equal(outs[1][..., :3], proc[1][..., :3])
 
 
-@pytest.mark.parametrize("W,H", [(37, 29), (130, 4), (4, 70), (255, 129)])
+@pytest.mark.parametrize("W,H", [(37, 29), (130, 4), (4, 70), (255, 129), (128, 128), (300, 64)])
 def test_frame_odd_sizes(vsc, orc, cuda, W, H):
     orig, proc, fwd, bwd = vsc.synth_video(nframes=4, width=W, height=H, nblobs=20, seed=W, patch_size=8)
     for mode in (0, 1, 2):
@@ -222,12 +222,18 @@ def test_frame_errors(vsc, cuda):
 
 
 @pytest.mark.parametrize("H,W,iters", [(37, 53, 19), (300, 500, 8), (1080, 1920, 9), (3, 700, 5),
-                                        (40, 70, 1), (40, 70, 2), (130, 260, 4)])
+                                        (40, 70, 1), (40, 70, 2), (130, 260, 4), (128, 128, 30),
+                                        (129, 257, 33), (200, 300, 45), (64, 700, 40), (540, 960, 75),
+                                        (1080, 1920, 40)])
 def test_consist_out_planar_blocked(vsc, orc, cuda, H, W, iters):
     """The temporally blocked solver on planar images, checked against the
     oracle (HWC) over tile and launch boundaries.  Sweep counts 1, 2 and 4
     cover a launch with only the level's first sweep, the first sweep plus an
-    odd remainder, and the first sweep plus ascending/descending pairs."""
+    odd remainder, and the first sweep plus ascending/descending pairs.  Sizes
+    of one tile (128 x 128), one tile plus one pixel, two tiles and several,
+    and a strip-high image (64 rows) cover the tile placements flush with the
+    image edges and the padded placement; the larger sweep counts take
+    several launches at the chosen halo."""
     import torch
     rng = np.random.default_rng(H + W)
     P, A, O = (rng.random((H, W, 3)).astype(np.float32) for _ in range(3))

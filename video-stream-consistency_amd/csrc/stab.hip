@@ -250,6 +250,40 @@ int launch_consist_sweep(View P, View A, View Wt, View Oin, View Oout, View U, f
 // ---------------------------------------------------------------------------
 static constexpr int SB_TW = 128;
 
+// Tile placement along one axis of length L: tiles of T pixels whose outer
+// `h` pixels on a side facing another tile are halo.  When L >= T the first
+// tile starts at 0 and the last ends at L (flush with the image: an image
+// edge needs no halo, its missing neighbours are the reference's boundary),
+// the tiles between output T - 2h pixels each; otherwise tile i starts at
+// i*(T - 2h) - h and pixels outside the image are padding.  A tile computes
+// [start, start + T) and stores [lo, hi); the output ranges partition [0, L).
+struct TileSpan {
+    int start, lo, hi;
+};
+
+__host__ __device__ inline int tile_count(int L, int T, int h)
+{
+    const int O = T - 2 * h;
+    if (L < T) return (L + O - 1) / O;
+    if (L == T) return 1;
+    const int rest = L - 2 * (T - h);
+    return 2 + (rest > 0 ? (rest + O - 1) / O : 0);
+}
+
+__host__ __device__ inline TileSpan tile_span(int i, int n, int L, int T, int h)
+{
+    const int O = T - 2 * h;
+    if (L < T) {
+        const int s = i * O - h;
+        return TileSpan{s, s + h, s + T - h < L ? s + T - h : L};
+    }
+    if (L == T) return TileSpan{0, 0, L};
+    if (i == 0) return TileSpan{0, 0, T - h};
+    const int lo = T - h + (i - 1) * O;
+    if (i == n - 1) return TileSpan{L - T, lo, L};
+    return TileSpan{lo - h, lo, lo + O};
+}
+
 // Rows of a strip are visited in ascending (REV false) or descending (REV
 // true) order.  A row's old value is still needed by the next row visited
 // after the row itself is updated, so a sweep leaves its O and U shifted by
@@ -257,8 +291,13 @@ static constexpr int SB_TW = 128;
 // shifts them back, so a pair of sweeps needs no register copies.  Each
 // pixel's arithmetic (and its operation order) is the same in both orders.
 // colL / colR hold the left / right neighbours, or zeros where the column has
-// none (cntc = number of horizontal neighbours); ROWMASK tests the rows.
-template <int R, bool MOM, bool ROWMASK, bool REV>
+// none (cntc = number of horizontal neighbours).  EDGE says which rows lack
+// a vertical neighbour: SE_NONE none, SE_TOP row 0 (the image's top row),
+// SE_BOTTOM rows R-2 and R-1 (the image's last two rows), SE_ROWS any row,
+// tested per pixel against rU / rD.
+enum StripEdge { SE_NONE = 0, SE_TOP = 1, SE_BOTTOM = 2, SE_ROWS = 3 };
+
+template <int R, bool MOM, int EDGE, bool REV>
 __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const float (&a)[R], const float (&w)[R],
                                             const float (&lp)[R], const float* colL, const float* colR,
                                             float up_edge, float dn_edge, float step, float mom, int cntc, int rU,
@@ -287,14 +326,13 @@ __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const 
             const float up = REV ? (r > 0 ? o[r - 1] : up_edge) : prev;
             const float dn = REV ? prev : (r < R - 1 ? o[r + 1] : dn_edge);
             float lo = rv[j] + lv[j];
-            int cnt = cntc + 2;
-            if (ROWMASK) {
-                cnt = cntc;
+            int cnt = cntc;
+            if (EDGE == SE_ROWS) {
                 if (r <= rD) { lo += dn; cnt += 1; }
                 if (r >= rU) { lo += up; cnt += 1; }
             } else {
-                lo += dn;
-                lo += up;
+                if (!(EDGE == SE_BOTTOM && r >= R - 2)) { lo += dn; cnt += 1; }
+                if (!(EDGE == SE_TOP && r == 0)) { lo += up; cnt += 1; }
             }
             lo -= (float)cnt * cur;
             const float t1 = w[r] * (cur - a[r]);
@@ -314,7 +352,7 @@ __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const 
 // per-pixel constants, and per pixel
 //   X = fma(-q, O, sum of neighbours); U' = fma(step, X, -kk); O' = O + U' (+ mom*U)
 // (7 VALU operations in the interior instead of 13).
-template <int R, bool MOM, bool ROWMASK, bool REV>
+template <int R, bool MOM, int EDGE, bool REV>
 __device__ __forceinline__ void sweep_strip_fma(float (&o)[R], float (&u)[R], const float (&q)[R],
                                                 const float (&kk)[R], const float* colL, const float* colR,
                                                 float up_edge, float dn_edge, float step, float mom, int rU, int rD)
@@ -342,12 +380,12 @@ __device__ __forceinline__ void sweep_strip_fma(float (&o)[R], float (&u)[R], co
             const float up = REV ? (r > 0 ? o[r - 1] : up_edge) : prev;
             const float dn = REV ? prev : (r < R - 1 ? o[r + 1] : dn_edge);
             float sum = rv[j] + lv[j];
-            if (ROWMASK) {
+            if (EDGE == SE_ROWS) {
                 if (r <= rD) sum += dn;
                 if (r >= rU) sum += up;
             } else {
-                sum += dn;
-                sum += up;
+                if (!(EDGE == SE_BOTTOM && r >= R - 2)) sum += dn;
+                if (!(EDGE == SE_TOP && r == 0)) sum += up;
             }
             const float X = __builtin_fmaf(-q[r], cur, sum);
             const float sg = __builtin_fmaf(step, X, -kk[r]);
@@ -370,7 +408,6 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     constexpr int TW = SB_TW;
     constexpr int TH = STRIPS * R;
     constexpr int RS = ((R / 4) % 2 == 0) ? R + 4 : R + 8;
-    const int OW = TW - 2 * halo, OH = TH - 2 * halo;
     static_assert(R % 4 == 0 && (RS / 4) % 2 == 1, "strip stride must be an odd number of 16-byte groups");
     // column TW of every plane and strip holds zeros: the excluded neighbour
     __shared__ __attribute__((aligned(16))) float sO[2][STRIPS][TW + 1][RS];
@@ -399,16 +436,25 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     const int strip = __builtin_amdgcn_readfirstlane(threadIdx.x / TW);
     const int wave_tx0 = __builtin_amdgcn_readfirstlane((threadIdx.x % TW) & ~63);
     const int ty0 = strip * R;
-    const int gx0 = btx * OW - halo;
-    const int gy0 = bty * OH - halo;
+    const TileSpan sx = tile_span(btx, tiles_x, W, TW, halo);
+    const TileSpan sy = tile_span(bty, tiles_y, H, TH, halo);
+    const int gx0 = sx.start, gy0 = sy.start;
     const int gx = gx0 + tx;
-    const bool colin = gx >= 0 && gx < W;
     const bool hr = (gx + 1) < (W - 1), hl = (gx - 1) >= 0;
     const int gys = gy0 + ty0;
-    // every row of this wave's strip has its up and down neighbours
-    const bool rows_in = (gys >= 1) && (gys + R - 1 <= H - 3);
     // no pixel of this wave (64 columns x R rows) lies in the image
     const bool wave_out = (gx0 + wave_tx0 >= W) || (gx0 + wave_tx0 + 63 < 0) || (gys >= H) || (gys + R - 1 < 0);
+    // Which rows of this wave's strip lack a vertical neighbour.  With tiles
+    // flush with the image (H >= TH), only the top tile's first strip (row 0)
+    // and the bottom tile's last strip (rows R-2, R-1) hold such rows whose
+    // values can reach a stored pixel: a tile's other rows with a missing
+    // neighbour lie on its own edge towards a neighbouring tile, inside the
+    // halo, whose values are not exact anyway.
+    int edge;
+    if (H >= TH)
+        edge = gys == 0 ? SE_TOP : gys + R == H ? SE_BOTTOM : SE_NONE;
+    else
+        edge = (gys >= 1 && gys + R - 1 <= H - 3) ? SE_NONE : SE_ROWS;
     const int cntc = (hr ? 1 : 0) + (hl ? 1 : 0);
 
     // FMA: w holds q = cnt + w, lp holds kk; a is unused (and not loaded)
@@ -449,9 +495,9 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
     const int rD0 = (H - 3) - gys;
 
     // One sweep: publish this strip's O, then update it from the neighbours.
-    auto sweep = [&](int s, auto mom_c, auto rev_c, auto rowmask_c) {
+    auto sweep = [&](int s, auto mom_c, auto rev_c, auto edge_c) {
         constexpr bool MOM = decltype(mom_c)::value, REV = decltype(rev_c)::value;
-        constexpr bool ROWMASK = decltype(rowmask_c)::value;
+        constexpr int EDGE = decltype(edge_c)::value;
         float (*buf)[TW + 1][RS] = sO[s & 1];
         float* own = buf[strip][tx];
 #pragma unroll
@@ -467,42 +513,46 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
         // predicates out of the sweep loop into scalar registers
         asm volatile("" : "+s"(rU), "+s"(rD));
         if constexpr (FMA)
-            sweep_strip_fma<R, MOM, ROWMASK, REV>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, rU, rD);
+            sweep_strip_fma<R, MOM, EDGE, REV>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, rU, rD);
         else
-            sweep_strip<R, MOM, ROWMASK, REV>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, cntc, rU, rD);
+            sweep_strip<R, MOM, EDGE, REV>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, cntc, rU, rD);
     };
     // The level's first sweep has no momentum term; after it, ascending and
     // descending sweeps alternate in pairs.
-    auto run = [&](auto rowmask_c) {
+    auto run = [&](auto edge_c) {
         const std::true_type yes{};
         const std::false_type no{};
         int s = 0;
         if (first_sweep == 0) {
-            sweep(0, no, no, rowmask_c);
+            sweep(0, no, no, edge_c);
             s = 1;
         }
         for (; s + 1 < sweeps; s += 2) {
-            sweep(s, yes, no, rowmask_c);
-            sweep(s + 1, yes, yes, rowmask_c);
+            sweep(s, yes, no, edge_c);
+            sweep(s + 1, yes, yes, edge_c);
         }
-        if (s < sweeps) sweep(s, yes, no, rowmask_c);
+        if (s < sweeps) sweep(s, yes, no, edge_c);
     };
     // Every wave passes one barrier per sweep, whichever branch it takes.
     if (wave_out) {
         for (int s = 0; s < sweeps; ++s) __syncthreads();
-    } else if (rows_in) {
-        run(std::false_type{});
+    } else if (edge == SE_NONE) {
+        run(std::integral_constant<int, SE_NONE>{});
+    } else if (edge == SE_TOP) {
+        run(std::integral_constant<int, SE_TOP>{});
+    } else if (edge == SE_BOTTOM) {
+        run(std::integral_constant<int, SE_BOTTOM>{});
     } else {
-        run(std::true_type{});
+        run(std::integral_constant<int, SE_ROWS>{});
     }
 
-    if (tx >= halo && tx < TW - halo && colin) {
+    // each image pixel is stored by exactly one tile (its span's output range)
+    if (gx >= sx.lo && gx < sx.hi) {
         const int wbase = gys * W + gx;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int ty = ty0 + r;
-            const int gy = gy0 + ty;
-            if (ty >= halo && ty < TH - halo && gy >= 0 && gy < H) {
+            const int gy = gys + r;
+            if (gy >= sy.lo && gy < sy.hi) {
                 Oo[wbase + r * W] = o[r];
                 if (store_u) Uo[wbase + r * W] = u[r];
             }
@@ -575,28 +625,41 @@ static constexpr int SB_STRIPS = 4;
 // area.  Half-height tiles (two blocks per CU) when full-height ones would
 // leave a quarter of the CUs or more idle (at 960x540, 198 and 252 full-height
 // tiles, full height measured faster: equal alone, 2% faster beside the flow
-// network of frame pipelining).
+// network of frame pipelining).  Launches of 209 to 256 tiles (one nearly
+// full round) are taken only when no other halo fits: they cost a whole round
+// anyway and leave almost no CU to work on another stream (at 960x540, 252
+// tiles in 3 launches at halo 25 against 198 in 4 at halo 19: 0.8% faster
+// alone, 4% slower frame-pipelined).
 namespace {
 
 BlockedPlan best_plan(int W, int H, int iters, int R, int strips, long long* tiles_out)
 {
     constexpr double A = 24.0;
-    BlockedPlan best{12, strips};
-    double best_cost = 1e300;
-    for (int h = 4; h <= 24; ++h) {
-        const int OW = SB_TW - 2 * h, OH = strips * R - 2 * h;
-        if (OH < 8) break;
-        const long long tiles = 3LL * ((W + OW - 1) / OW) * ((H + OH - 1) / OH);
+    BlockedPlan best{12, strips}, best_band{12, strips};
+    double best_cost = 1e300, band_cost = 1e300;
+    long long band_tiles = 0;
+    for (int h = 4; h <= 30; ++h) {
+        if (strips * R - 2 * h < 8) break;
+        const long long tiles = 3LL * tile_count(W, SB_TW, h) * tile_count(H, strips * R, h);
         const double per_cu = 0.5 * ((double)tiles / 256.0 + (double)((tiles + 255) / 256)) * strips / SB_STRIPS;
         const int full = iters / h, rem = iters % h;
         const double cost = full * (A + per_cu * h) + (rem ? A + per_cu * rem : 0.0);
-        if (cost < best_cost - 1e-9) {
+        const bool near_full = tiles > 208 && tiles <= 256;
+        if (near_full) {
+            if (cost < band_cost - 1e-9) {
+                band_cost = cost;
+                best_band = BlockedPlan{h, strips};
+                band_tiles = tiles;
+            }
+        } else if (cost < best_cost - 1e-9) {
             best_cost = cost;
             best = BlockedPlan{h, strips};
             *tiles_out = tiles;
         }
     }
-    return best;
+    if (best_cost < 1e299) return best;
+    *tiles_out = band_tiles;
+    return best_band;
 }
 
 }  // namespace
@@ -617,10 +680,8 @@ int launch_blocked(const float* lapP, const float* A, const float* Wt, const flo
                    float* Uout, int W, int H, int sweeps, int first_sweep, float step, float mom,
                    const BlockedPlan& plan, int store_u, hipStream_t s)
 {
-    const int OW = SB_TW - 2 * plan.halo;
-    const int OH = plan.strips * R - 2 * plan.halo;
-    const int tiles_x = (W + OW - 1) / OW;
-    const int tiles_y = (H + OH - 1) / OH;
+    const int tiles_x = tile_count(W, SB_TW, plan.halo);
+    const int tiles_y = tile_count(H, plan.strips * R, plan.halo);
     const int per_xcd = (tiles_x * tiles_y * 3 + 7) / 8;
     // algorithmic bytes: read O, U and the per-pixel constants (lapP, A, w; FMA
     // form: q, kk) and write O (and U unless it is the level's last launch)
