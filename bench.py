@@ -33,9 +33,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FLOW_HEAD_SCALE = 0.05
-# consistency solver arithmetic per 1080p frame: 14 operations per pixel, channel
-# and sweep; 150 sweeps at 1920x1080 and 75 at 960x540 (default parameters)
-SOLVER_FLOPS_PER_FRAME = 14 * 3 * (1920 * 1080 * 150 + 960 * 540 * 75)
+# consistency solver arithmetic per 1080p frame: the reference formula's 13 FP32
+# operations per pixel, channel and sweep (Laplacian of O 5, w*(O - A) 2, the
+# difference with lapP 1, the gradient 1, the step 1, the momentum update 3; lapP
+# is a constant of the level); 150 sweeps at 1920x1080 and 75 at 960x540
+SOLVER_FLOPS_PER_FRAME = 13 * 3 * (1920 * 1080 * 150 + 960 * 540 * 75)
+# FP32 vector peak without packed FP32 instructions (the library is built without
+# them): 256 CUs x 64 lanes x 2 (FMA) x 2.4 GHz
+VALU_FP32_PEAK_TFLOPS = 78.6
 RING = 24
 W, H = 1920, 1080
 
@@ -452,13 +457,14 @@ def bench_pipeline(args):
             "launches": n_solver, "avg_launch_us": round(1000.0 * ms_solver / n_solver, 3),
             "algorithmic_bytes_per_launch": round(by_solver / n_solver),
             "share_of_timed_region": round(ms_solver / (1000.0 * el_c), 4),
-            # the solver's algorithmic arithmetic (the reference formula's 14 FP32
+            # the solver's algorithmic arithmetic (the reference formula's 13 FP32
             # operations per pixel, channel and sweep; the FMA form issues 7
             # instructions for them): its rate against the FP32 vector peak,
             # beside the HBM figure
             "valu": {"algorithmic_tflops": round(SOLVER_FLOPS_PER_FRAME * args.steps / (ms_solver * 1e-3) / 1e12, 2),
-                     "peak_tflops": 157.3,
-                     "frac": round(SOLVER_FLOPS_PER_FRAME * args.steps / (ms_solver * 1e-3) / 157.3e12, 4)},
+                     "peak_tflops": VALU_FP32_PEAK_TFLOPS,
+                     "frac": round(SOLVER_FLOPS_PER_FRAME * args.steps / (ms_solver * 1e-3)
+                                   / (VALU_FP32_PEAK_TFLOPS * 1e12), 4)},
         }
     if n_front:
         ach = by_front / (ms_front * 1e-3) / 1e9
