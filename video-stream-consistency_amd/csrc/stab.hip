@@ -86,9 +86,47 @@ static bool planar4(const View& v)
     return v.ps == 1 && v.cs == (long long)v.w * v.h && v.w % 4 == 0 && reinterpret_cast<uintptr_t>(v.d) % 16 == 0;
 }
 
+// Exact 2x upsampling (out = 2*in in both axes, 2*in.w*in.w and 2*in.h*in.h
+// below 2^24): bilinear_tap then computes x*in.w/out.w = x/2 exactly, so the
+// output quad (2x..2x+1, 2y..2y+1) takes taps (x, y), (x+1, y), (x, y+1),
+// (x+1, y+1) (clamped) with weights fx, fy in {0, 1/2}.  One thread per quad
+// loads the 4 taps once per channel and blends them as bilinear_sample does;
+// VEC writes each output row pair with one 8-byte store (planar output).
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_bilinear_up2(View in, View out)
+{
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    if (x >= in.w || y >= in.h) return;
+    const int x1 = min(x + 1, in.w - 1), y1 = min(y + 1, in.h - 1);
+    for (int c = 0; c < out.ch; ++c) {
+        const float v00 = in.rd(x, y, c), v10 = in.rd(x1, y, c);
+        const float v01 = in.rd(x, y1, c), v11 = in.rd(x1, y1, c);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const float fy = b ? 0.5f : 0.0f;
+            const float p0 = bilinear_mix(v00, v10, v01, v11, 0.0f, fy);
+            const float p1 = bilinear_mix(v00, v10, v01, v11, 0.5f, fy);
+            if (VEC) {
+                *reinterpret_cast<float2*>(out.d + out.idx(2 * x, 2 * y + b, c)) = make_float2(p0, p1);
+            } else {
+                out.wr(2 * x, 2 * y + b, c, p0);
+                out.wr(2 * x + 1, 2 * y + b, c, p1);
+            }
+        }
+    }
+}
+
 int launch_bilinear(View in, View out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bilinear, grid2d(out.w, out.h), dim3(BX, BY), 0, s, in, out);
+    const bool up2 = out.w == 2 * in.w && out.h == 2 * in.h && 2LL * in.w * in.w < (1LL << 24) &&
+                     2LL * in.h * in.h < (1LL << 24);
+    if (up2 && out.ps == 1 && out.cs % 2 == 0 && reinterpret_cast<uintptr_t>(out.d) % 8 == 0)
+        hipLaunchKernelGGL(k_bilinear_up2<true>, grid2d(in.w, in.h), dim3(BX, BY), 0, s, in, out);
+    else if (up2)
+        hipLaunchKernelGGL(k_bilinear_up2<false>, grid2d(in.w, in.h), dim3(BX, BY), 0, s, in, out);
+    else
+        hipLaunchKernelGGL(k_bilinear, grid2d(out.w, out.h), dim3(BX, BY), 0, s, in, out);
     return check_launch("bilinear");
 }
 

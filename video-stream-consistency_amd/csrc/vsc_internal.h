@@ -186,6 +186,12 @@ __device__ inline BiTap bilinear_tap(int ox, int oy, int inW, int inH, int outW,
     t.iy1 = min(t.iy + 1, inH - 1);
     return t;
 }
+// the four taps' blend, in the reference's operation order
+__device__ inline float bilinear_mix(float v00, float v10, float v01, float v11, float fx, float fy)
+{
+    return (((v00 * (1.0f - fx)) * (1.0f - fy) + (v10 * fx) * (1.0f - fy)) + (v01 * (1.0f - fx)) * fy) +
+           (v11 * fx) * fy;
+}
 template <class Img>
 __device__ inline float bilinear_sample(const Img& im, const BiTap& t, int c)
 {
@@ -193,8 +199,7 @@ __device__ inline float bilinear_sample(const Img& im, const BiTap& t, int c)
     const float v10 = im.rd(t.ix1, t.iy, c);
     const float v01 = im.rd(t.ix, t.iy1, c);
     const float v11 = im.rd(t.ix1, t.iy1, c);
-    return (((v00 * (1.0f - t.fx)) * (1.0f - t.fy) + (v10 * t.fx) * (1.0f - t.fy)) + (v01 * (1.0f - t.fx)) * t.fy) +
-           (v11 * t.fx) * t.fy;
+    return bilinear_mix(v00, v10, v01, v11, t.fx, t.fy);
 }
 
 // kernel_to_char_image (gpuimage.cu:62-65): floor(|v|*255) to uint32 with
