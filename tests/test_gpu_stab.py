@@ -197,6 +197,17 @@ def test_frame_1080p_single(vsc, orc, cuda, mode):
     assert np.array_equal(outs[1], r_outs[1])
 
 
+def test_frame_4k_single_fma(vsc, orc, cuda):
+    """A 3840x2160 frame with the FMA-form solver (its tile plan at this size
+    differs from 1080p's) against the oracle."""
+    print("frame 3840x2160 mode=2", flush=True)
+    orig, proc, fwd, bwd = vsc.synth_video(**dict(vsc.SYNTH_1080P, width=3840, height=2160, nframes=3, nblobs=1600))
+    outs, floats = _run_sequence(vsc, orig, proc, fwd, bwd, cuda, mode=2)
+    r_outs, r_floats = orc.stabilize_sequence(orig, proc, fwd, bwd, solver="fma")
+    assert np.array_equal(floats[1], r_floats[1])
+    assert np.array_equal(outs[1], r_outs[1])
+
+
 def test_frame_deterministic_and_stream_independent(vsc, cuda):
     orig, proc, fwd, bwd = vsc.synth_video(nframes=5, width=320, height=180, nblobs=60, seed=3)
     a, fa = _run_sequence(vsc, orig, proc, fwd, bwd, cuda)
@@ -224,7 +235,7 @@ def test_frame_errors(vsc, cuda):
 @pytest.mark.parametrize("H,W,iters", [(37, 53, 19), (300, 500, 8), (1080, 1920, 9), (3, 700, 5),
                                         (40, 70, 1), (40, 70, 2), (130, 260, 4), (128, 128, 30),
                                         (129, 257, 33), (200, 300, 45), (64, 700, 40), (540, 960, 75),
-                                        (1080, 1920, 40)])
+                                        (1080, 1920, 40), (2160, 3840, 25)])
 def test_consist_out_planar_blocked(vsc, orc, cuda, H, W, iters):
     """The temporally blocked solver on planar images, checked against the
     oracle (HWC) over tile and launch boundaries.  Sweep counts 1, 2 and 4
@@ -233,7 +244,8 @@ def test_consist_out_planar_blocked(vsc, orc, cuda, H, W, iters):
     of one tile (128 x 128), one tile plus one pixel, two tiles and several,
     and a strip-high image (64 rows) cover the tile placements flush with the
     image edges and the padded placement; the larger sweep counts take
-    several launches at the chosen halo."""
+    several launches at the chosen halo (3840 x 2160: the largest serving
+    size)."""
     import torch
     rng = np.random.default_rng(H + W)
     P, A, O = (rng.random((H, W, 3)).astype(np.float32) for _ in range(3))
