@@ -83,6 +83,42 @@ def test_trained_upsampler_beats_bilinear(vsc, cuda):
     assert el < 0.93 * eb
 
 
+def _fb_count_np(f, b):
+    """The forward-backward criterion (Sundaram et al. 2010) as the kernel
+    evaluates it, in float32 and the same operation order."""
+    one, h, w = np.float32(1), f.shape[0], f.shape[1]
+    ys, xs = np.mgrid[0:h, 0:w]
+    fx, fy = f[..., 0], f[..., 1]
+    tx, ty = xs.astype(np.float32) + fx, ys.astype(np.float32) + fy
+    inside = (tx >= 0) & (ty >= 0) & (tx <= np.float32(w - 1)) & (ty <= np.float32(h - 1))
+    x0 = np.minimum(np.clip(tx, 0, w - 1).astype(np.int32), w - 1)
+    y0 = np.minimum(np.clip(ty, 0, h - 1).astype(np.int32), h - 1)
+    x1, y1 = np.minimum(x0 + 1, w - 1), np.minimum(y0 + 1, h - 1)
+    ax, ay = tx - x0.astype(np.float32), ty - y0.astype(np.float32)
+    bv = [(one - ay) * ((one - ax) * b[y0, x0, c] + ax * b[y0, x1, c]) + ay * ((one - ax) * b[y1, x0, c] + ax * b[y1, x1, c])
+          for c in range(2)]
+    sx, sy = fx + bv[0], fy + bv[1]
+    lhs = sx * sx + sy * sy
+    rhs = np.float32(0.01) * (fx * fx + fy * fy + bv[0] * bv[0] + bv[1] * bv[1]) + np.float32(0.5)
+    return int(((~inside) | ~(lhs < rhs)).sum())
+
+
+@pytest.mark.parametrize("ch", [2, 3])
+def test_flow_fb_count_random(vsc, cuda, ch):
+    """Count of inconsistent pixels for random flows (a backward flow near
+    -forward plus noise: a mix of both outcomes) equals the criterion
+    evaluated in numpy, for 2- and 3-channel flow layouts."""
+    import torch
+    h, w = 71, 93
+    rng = np.random.default_rng(ch)
+    f = rng.uniform(-3, 3, (h, w, ch)).astype(np.float32)
+    b = (-f + rng.normal(0, 0.6, f.shape)).astype(np.float32)
+    count, _ = vsc.flow_fb_gate(torch.from_numpy(f).to(cuda), torch.from_numpy(b).to(cuda), 0.25)
+    want = _fb_count_np(f, b)
+    assert 0.05 * h * w < want < 0.95 * h * w
+    assert int(count.cpu()[0]) == want
+
+
 def test_flow_fb_gate(vsc, cuda):
     import torch
     h, w = 60, 80
