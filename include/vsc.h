@@ -167,6 +167,20 @@ int vsc_stab_consist_out_planar(const float* crnt_pr, const float* prev_stab_war
                                 int width, int height, int num_iter, float step_size, float mom_fac,
                                 float* consist_out, void* workspace, vsc_hip_stream stream);
 
+/* Launch plan of the temporally blocked solver (diagnostic; no counterpart in
+ * the reference, which launches one kernel per sweep): for one pyramid level
+ * of width x height pixels and num_iter sweeps, the most sweeps per launch
+ * (halo), the tile height in rows and the tile grid (per channel); fma
+ * selects the FMA-form solver (VSC_FRAME_FMA), else the reference order. */
+int vsc_stab_blocked_plan(int width, int height, int num_iter, int fma, int* halo, int* tile_rows, int* tiles_x,
+                          int* tiles_y);
+
+/* Tile placement along one axis of `length` pixels (diagnostic): tiles of
+ * `tile` pixels whose `halo` pixels facing another tile are not stored.
+ * *count receives the number of tiles; for 0 <= i < *count, tile i computes
+ * [*start, *start + tile) and stores [*lo, *hi). */
+int vsc_stab_tile_span(int length, int tile, int halo, int i, int* count, int* start, int* lo, int* hi);
+
 /* get_bilinear (flowconsistency.cu:50-75,288-299): resample in -> out. */
 int vsc_stab_bilinear(vsc_image in, vsc_image out, vsc_hip_stream stream);
 

@@ -45,6 +45,7 @@ EXPORTED = (
     "vsc_float_to_rgba", "vsc_stream_create", "vsc_stream_destroy", "vsc_stream_footprint",
     "vsc_stream_set_last_rgba", "vsc_stream_set_last", "vsc_stream_copy_last", "vsc_stab_frame",
     "vsc_profile_begin", "vsc_profile_end", "vsc_stab_consist_out_planar_workspace_size",
+    "vsc_stab_blocked_plan", "vsc_stab_tile_span",
     "vsc_stab_consist_out_planar", "vsc_conv3x3_nhwc", "vsc_flownet_create", "vsc_flownet_destroy",
     "vsc_flownet_footprint", "vsc_flownet_flow_size", "vsc_flownet_param_count", "vsc_flownet_get_params",
     "vsc_flownet_set_params", "vsc_flownet_extract", "vsc_flownet_flow", "vsc_flownet_intermediate",
@@ -147,6 +148,9 @@ def lib():
         L.vsc_stab_consist_out_planar_workspace_size.argtypes = [i, i]
         L.vsc_stab_consist_out_planar_workspace_size.restype = ctypes.c_size_t
         L.vsc_stab_consist_out_planar.argtypes = [vp, vp, vp, i, i, i, f, f, vp, vp, vp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.vsc_stab_blocked_plan.argtypes = [i, i, i, i, ip, ip, ip, ip]
+        L.vsc_stab_tile_span.argtypes = [i, i, i, i, ip, ip, ip, ip]
         L.vsc_stab_bilinear.argtypes = [img, img, vp]
         L.vsc_average_frames.argtypes = [img, img, img, vp]
         L.vsc_rgba_to_float.argtypes = [vp, img, vp]
@@ -612,6 +616,23 @@ def get_consist_out_planar(crnt_pr, prev_stab_warp, cons_wt, num_iter, step_size
     check(lib().vsc_stab_consist_out_planar(_ptr(crnt_pr), _ptr(prev_stab_warp), _ptr(cons_wt), w, h,
                                             int(num_iter), float(step_size), float(mom_fac), _ptr(consis_out),
                                             ws, _stream()))
+
+
+def blocked_plan(width, height, num_iter, fma=True):
+    """Launch plan of the temporally blocked solver for one pyramid level:
+    (sweeps per launch, tile rows, tiles across, tiles down)."""
+    out = [ctypes.c_int() for _ in range(4)]
+    check(lib().vsc_stab_blocked_plan(int(width), int(height), int(num_iter), int(bool(fma)),
+                                      *[ctypes.byref(v) for v in out]))
+    return tuple(v.value for v in out)
+
+
+def tile_span(length, tile, halo, i):
+    """(tile count, first computed pixel, stored range lo, hi) of tile i along
+    an axis of the blocked solver."""
+    out = [ctypes.c_int() for _ in range(4)]
+    check(lib().vsc_stab_tile_span(int(length), int(tile), int(halo), int(i), *[ctypes.byref(v) for v in out]))
+    return tuple(v.value for v in out)
 
 
 def consist_out_workspace_bytes(width, height):

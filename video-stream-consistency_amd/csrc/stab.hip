@@ -1003,6 +1003,35 @@ extern "C" int vsc_stab_consist_out(vsc_image crnt_pr, vsc_image prev_stab_warp,
     return rc;
 }
 
+extern "C" int vsc_stab_blocked_plan(int width, int height, int num_iter, int fma, int* halo, int* tile_rows,
+                                     int* tiles_x, int* tiles_y)
+{
+    if (!halo || !tile_rows || !tiles_x || !tiles_y) return set_error(VSC_E_INVALID, "vsc_stab_blocked_plan: NULL");
+    if (width <= 0 || height <= 0 || num_iter < 1)
+        return set_error(VSC_E_INVALID, "vsc_stab_blocked_plan: empty level or no sweeps");
+    const BlockedPlan plan = plan_consist_blocked(width, height, num_iter, fma);
+    const int th = plan.strips * (fma ? SB_R_FMA : SB_R);
+    *halo = plan.halo;
+    *tile_rows = th;
+    *tiles_x = tile_count(width, SB_TW, plan.halo);
+    *tiles_y = tile_count(height, th, plan.halo);
+    return VSC_OK;
+}
+
+extern "C" int vsc_stab_tile_span(int length, int tile, int halo, int i, int* count, int* start, int* lo, int* hi)
+{
+    if (!count || !start || !lo || !hi) return set_error(VSC_E_INVALID, "vsc_stab_tile_span: NULL");
+    if (length <= 0 || halo < 0 || tile - 2 * halo < 1)
+        return set_error(VSC_E_INVALID, "vsc_stab_tile_span: no output per tile");
+    *count = tile_count(length, tile, halo);
+    if (i < 0 || i >= *count) return set_error(VSC_E_INVALID, "vsc_stab_tile_span: tile index out of range");
+    const TileSpan t = tile_span(i, *count, length, tile, halo);
+    *start = t.start;
+    *lo = t.lo;
+    *hi = t.hi;
+    return VSC_OK;
+}
+
 extern "C" size_t vsc_stab_consist_out_planar_workspace_size(int width, int height)
 {
     if (width < 1 || height < 1) return 0;
