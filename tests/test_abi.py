@@ -70,3 +70,66 @@ def test_python_errors_without_gpu(vsc):
     with pytest.raises(vsc.VscError) as e:
         vsc.check(-1)
     assert e.value.code == -1
+
+
+def test_default_cfgs_fill_every_field(vsc):
+    """vsc_pipeline_default_cfg / vsc_server_default_cfg set every field: a
+    struct pre-filled with 0xFF bytes comes back equal to a zero-filled one."""
+    L = vsc.lib()
+    for T, fill in ((vsc.PipelineCfg, L.vsc_pipeline_default_cfg), (vsc.ServerCfg, L.vsc_server_default_cfg)):
+        a, b = T(), T()
+        ctypes.memset(ctypes.byref(a), 0xFF, ctypes.sizeof(T))
+        ctypes.memset(ctypes.byref(b), 0x00, ctypes.sizeof(T))
+        fill(ctypes.byref(a))
+        fill(ctypes.byref(b))
+        # compare field by field (padding bytes may differ)
+        def fields(s):
+            out = []
+            for n, t in s._fields_:
+                v = getattr(s, n)
+                out.extend(fields(v) if isinstance(v, ctypes.Structure) else [(n, v)])
+            return out
+        assert fields(a) == fields(b), T.__name__
+    c = vsc.PipelineCfg()
+    ctypes.memset(ctypes.byref(c), 0xFF, ctypes.sizeof(c))
+    L.vsc_pipeline_default_cfg(ctypes.byref(c))
+    assert c.overlap == 0
+    # the pipeline's footprint accepts the defaults (a field left unset would
+    # make the configuration invalid and the footprint 0)
+    assert L.vsc_pipeline_footprint(640, 360, ctypes.byref(c)) > 0
+
+
+def test_server_rejects_overlap_form(vsc):
+    L = vsc.lib()
+    c = vsc.ServerCfg()
+    L.vsc_server_default_cfg(ctypes.byref(c))
+    c.form.overlap = 1
+    h = ctypes.c_void_p()
+    ups = (ctypes.c_float * L.vsc_upsampler_param_count())()
+    assert L.vsc_server_create(ctypes.byref(c), ups, ctypes.byref(h)) == -1
+    assert b"overlap" in L.vsc_last_error()
+
+
+@pytest.mark.parametrize("size", [(200, 120, 100, 60), (37, 29, 37, 29), (160, 96, 160, 96), (999, 555, 500, 278),
+                                  (1279, 719, 640, 360), (1920, 1080, 960, 540)])
+def test_stream_footprint_covers_the_carve(vsc, size):
+    """The stream's block is carved into 8 planar images per pyramid level
+    (plus two frame-size flows when the flows arrive downscaled), each starting
+    on a 256-byte boundary.  The footprint must hold every piece with its
+    padding: at least sum(ceil256(piece)) bytes, counted here from the level
+    dimensions W>>j x H>>j down to 1 pixel."""
+    L = vsc.lib()
+    W, H, fw, fh = size
+    need = 0
+    w, h = W, H
+    levels = 0
+    while levels < 4 and w >= 1 and h >= 1:
+        need += 8 * (-(-(3 * w * h * 4) // 256) * 256)
+        w //= 2
+        h //= 2
+        levels += 1
+    if (fw, fh) != (W, H):
+        need += 2 * (-(-(3 * W * H * 4) // 256) * 256)
+    got = L.vsc_stream_footprint(W, H, fw, fh)
+    assert got >= need, (got, need)
+    assert got % 256 == 0

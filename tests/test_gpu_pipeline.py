@@ -132,3 +132,57 @@ def test_pipeline_overlap_matches_serial(vsc, cuda, form):
     assert res[False][2] == res[True][2]
     if form == "plain":
         assert res[True][2] == (0.0, 0)     # no learned upsampler: bilinear, nothing counted
+
+
+def test_pipeline_refused_params_leave_state_unchanged(vsc, cuda):
+    """A frame with invalid parameters is refused before any state change
+    (feature slots, previous flow, buffer parity): the next frame with valid
+    parameters gives the same bits as a pipeline that never saw the refusal."""
+    import torch
+    orig, proc = _video(vsc, n=6, seed=23)
+    o = torch.from_numpy(orig).to(cuda)
+    p = torch.from_numpy(proc).to(cuda)
+    ref_pipe = vsc.Pipeline(W, H, form="chosen", seed=2)
+    ref_pipe.set_last_rgba(p[2])
+    ref = {t: ref_pipe.frame(_window(o, p, t), t).clone() for t in range(1, 5)}
+    pipe = vsc.Pipeline(W, H, form="chosen", seed=2)
+    pipe.set_last_rgba(p[2])
+    bad = vsc.HyperParams()
+    bad.pyramidLevels = 9
+    neg = vsc.HyperParams()
+    neg.numIter = -1
+    for t in range(1, 5):
+        for prm in (bad, neg):
+            with pytest.raises(vsc.VscError) as e:
+                pipe.frame(_window(o, p, t), t, params=prm)
+            assert e.value.code == vsc.VSC_E_INVALID
+        got = pipe.frame(_window(o, p, t), t)
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref[t]), f"frame {t}"
+
+
+@pytest.mark.parametrize("size", [(200, 120, 100, 60), (37, 29, 37, 29), (999, 555, 500, 278)])
+def test_stream_create_odd_sizes(vsc, cuda, size):
+    """Streams whose level images are not multiples of 256 bytes: creation
+    checks that its carve ends exactly at the footprint, and a frame at every
+    pyramid depth runs (flows upsampled when they arrive downscaled)."""
+    import torch
+    Wf, Hf, fw, fh = size
+    st = vsc.VideoStabilizer(Wf, Hf, fw, fh, mode=vsc.VSC_FRAME_FMA)
+    orig, proc, fwd, bwd = vsc.synth_video(width=Wf, height=Hf, nframes=3, nblobs=20, seed=4)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    st.set_last_rgba(T(proc[2]))
+    iy, ix = (np.arange(fh) * Hf) // fh, (np.arange(fw) * Wf) // fw
+    lo = lambda f: T(f[iy][:, ix])            # nearest resample to the flow size
+    nlev = 0
+    w, h = Wf, Hf
+    while nlev < 4 and w >= 1 and h >= 1:
+        nlev, w, h = nlev + 1, w // 2, h // 2
+    for L in range(1, nlev + 1):
+        hp = vsc.HyperParams()
+        hp.pyramidLevels = L
+        hp.numIter = 8
+        out = st.do_one_step([T(x) for x in orig], [T(x) for x in proc], lo(fwd[1]), lo(bwd[1]), params=hp)
+        torch.cuda.synchronize()
+        assert out.shape == (Hf, Wf, 4)
+    st.close()

@@ -180,3 +180,39 @@ def test_server_read_into_pinned_host(vsc, cuda):
         sv.read(sid, 3, out=torch.empty((H, W, 4), dtype=torch.uint8))
     sv.close()
     assert n == 3
+
+
+def test_server_refuses_bad_params(vsc, cuda):
+    """vsc_server_set_params validates against the stream (pyramid depth of
+    its frame size, num_iter >= 0); a refused change leaves the stream's
+    parameters and its captured graphs as they were."""
+    import torch
+    W, H = SIZES[2]
+    orig, proc = _video(vsc, W, H, 7, seed=12)
+    o = torch.from_numpy(orig).to(cuda)
+    p = torch.from_numpy(proc).to(cuda)
+    outs = {}
+    for refuse in (False, True):
+        sv = vsc.Server(form="plain")
+        sid = sv.open(W, H)
+        got = {}
+        for t in range(7):
+            if refuse and t == 4:
+                bad = vsc.HyperParams()
+                bad.pyramidLevels = 9
+                with pytest.raises(vsc.VscError):
+                    sv.set_params(sid, bad)
+                bad = vsc.HyperParams()
+                bad.numIter = -3
+                with pytest.raises(vsc.VscError):
+                    sv.set_params(sid, bad)
+            sv.submit(sid, o[t], p[t], tag=t)
+            sv.tick()
+            sv.drain()
+            for _, _, frame, _ in sv.poll():
+                got[frame] = sv.read(sid, frame).cpu()
+        sv.close()
+        outs[refuse] = got
+    assert sorted(outs[True]) == [1, 2, 3, 4, 5]
+    for f in outs[False]:
+        assert torch.equal(outs[False][f], outs[True][f]), f"frame {f}"

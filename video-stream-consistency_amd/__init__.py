@@ -30,7 +30,7 @@ LIB_PATH = os.path.join(_HERE, "libvsc.so")
 SYNTH_PATH = os.path.join(_HERE, "libvsc_synth.so")
 
 VSC_OK = 0
-VSC_E_AGAIN = -6
+VSC_E_INVALID, VSC_E_DTYPE, VSC_E_HIP, VSC_E_NOMEM, VSC_E_UNSUPPORTED, VSC_E_AGAIN = -1, -2, -3, -4, -5, -6
 VSC_F32, VSC_F64, VSC_BF16, VSC_F16, VSC_FP8E4M3 = 0, 1, 2, 3, 4
 VSC_FRAME_FUSED, VSC_FRAME_STAGED, VSC_FRAME_FMA = 0, 1, 2
 SOLVERS = {"exact": 0, "fma": 1}    # stabilization solver: reference op order | FMA form

@@ -101,6 +101,7 @@ extern "C" void vsc_pipeline_default_cfg(vsc_pipeline_cfg* c)
     c->ref_bwd_index = 0;
     c->seed = 0;
     c->solver = 0;
+    c->overlap = 0;
 }
 
 static int cfg_ok(int W, int H, const vsc_pipeline_cfg* c)
@@ -261,21 +262,35 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
 {
     if (!p || !w || !w->orig_prev || !w->orig_cur || !w->orig_next)
         return set_error(VSC_E_INVALID, "vsc_pipeline_frame: NULL argument");
+    // every check before any state change: a refused frame leaves the
+    // pipeline as it was
+    int rc = stream_frame_check(p->stab, params);
+    if (rc) return rc;
     hipStream_t cs = as_stream(stream);
     const bool ov = p->cfg.overlap && !capturing(cs);
     hipStream_t fs = ov ? p->sf : cs, ss = ov ? p->ss : cs;
     const int f = p->cfg.downscale;
     const int k = parity(frame);
-    int rc;
+    // host state as on entry: restored when enqueueing (or capturing) fails
+    // part-way, so a later call recomputes what this one did not finish
+    long long slot_frame0[3] = {p->slot_frame[0], p->slot_frame[1], p->slot_frame[2]};
+    const long long prev_fwd0 = p->prev_fwd_frame;
+    const int parity0 = p->last_parity;
+    auto fail = [&](int code) {
+        for (int i = 0; i < 3; ++i) p->slot_frame[i] = slot_frame0[i];
+        p->prev_fwd_frame = prev_fwd0;
+        p->last_parity = parity0;
+        return code;
+    };
     if (ov) {
         // flows wait for the caller's inputs and for the stabilization that
         // last read buffer set k (frame - 2)
         if (check_hip(hipEventRecord(p->ev_in, cs), "vsc_pipeline_frame") ||
             check_hip(hipStreamWaitEvent(fs, p->ev_in, 0), "vsc_pipeline_frame") ||
             (p->s_rec[k] && check_hip(hipStreamWaitEvent(fs, p->ev_s[k], 0), "vsc_pipeline_frame")))
-            return VSC_E_HIP;
+            return fail(VSC_E_HIP);
     }
-    if ((rc = flows_on(p, w, frame, p->lo[k][0], p->lo[k][1], fs))) return rc;
+    if ((rc = flows_on(p, w, frame, p->lo[k][0], p->lo[k][1], fs))) return fail(rc);
     const vsc_hip_stream fsv = reinterpret_cast<vsc_hip_stream>(fs);
     // previous forward flow for the consistency check of this frame's backward flow
     const bool have_prev = p->prev_fwd_frame == frame - 1;
@@ -284,17 +299,17 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
     if (learned(p->cfg)) {
         if (p->cfg.fb_max_inconsistent < 0.0f || (!have_prev && !p->cfg.ref_bwd_index)) {
             hipLaunchKernelGGL(k_set_gate, dim3(1), dim3(64), 0, fs, gate, 1.0f);
-            if ((rc = check_launch("pipeline gate"))) return rc;
+            if ((rc = check_launch("pipeline gate"))) return fail(rc);
         } else if (p->cfg.ref_bwd_index) {
             // fwd: cur -> next, bwd: next -> cur are each other's inverse
             if ((rc = vsc_flow_fb_gate(p->lo[k][0], p->lo[k][1], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent, count,
                                        gate, fsv)))
-                return rc;
+                return fail(rc);
         } else {
             // bwd: cur -> prev against the previous frame's fwd: prev -> cur
             if ((rc = vsc_flow_fb_gate(p->lo[k][1], p->lo[1 - k][0], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent,
                                        count, gate, fsv)))
-                return rc;
+                return fail(rc);
         }
     }
     p->prev_fwd_frame = frame;
@@ -302,7 +317,7 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
     if (ov) {
         if (check_hip(hipEventRecord(p->ev_f, fs), "vsc_pipeline_frame") ||
             check_hip(hipStreamWaitEvent(ss, p->ev_f, 0), "vsc_pipeline_frame"))
-            return VSC_E_HIP;
+            return fail(VSC_E_HIP);
         p->f_rec = true;
     }
     // upsampling to the frame, on the stabilization stream (it reads only this
@@ -315,7 +330,7 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
         const float* const lo[2] = {p->lo[k][0], p->lo[k][1]};
         float* const hi[2] = {p->hi[k][0], p->hi[k][1]};
         if ((rc = upsampler_run_flows(p->ups, 2, lo, p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, hi, 2, gate, ss)))
-            return rc;
+            return fail(rc);
         ff = p->hi[k][0];
         fb = p->hi[k][1];
     } else if (upsample_here(p->cfg)) {
@@ -324,19 +339,20 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
                            (float)f);
         hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, ss, p->lo[k][1], p->Wd, p->Hd, p->hi[k][1], p->W, p->H,
                            (float)f);
-        if ((rc = check_launch("pipeline flow upscale"))) return rc;
+        if ((rc = check_launch("pipeline flow upscale"))) return fail(rc);
         ff = p->hi[k][0];
         fb = p->hi[k][1];
     }
     vsc_frame_in in{w->orig_prev, w->orig_cur, w->orig_next, w->proc_prev, w->proc_cur, w->proc_next, ff, fb, 2};
     rc = vsc_stab_frame(p->stab, &in, params, out_rgba, out_float, p->cfg.solver ? VSC_FRAME_FMA : VSC_FRAME_FUSED,
                         reinterpret_cast<vsc_hip_stream>(ss));
-    if (rc || !ov) return rc;
-    if (check_hip(hipEventRecord(p->ev_s[k], ss), "vsc_pipeline_frame")) return VSC_E_HIP;
+    if (rc) return fail(rc);
+    if (!ov) return VSC_OK;
+    if (check_hip(hipEventRecord(p->ev_s[k], ss), "vsc_pipeline_frame")) return fail(VSC_E_HIP);
     p->s_rec[k] = true;
     // the caller's stream is ordered after the previous frame (frame - 1)
     if (p->s_rec[1 - k] && check_hip(hipStreamWaitEvent(cs, p->ev_s[1 - k], 0), "vsc_pipeline_frame"))
-        return VSC_E_HIP;
+        return fail(VSC_E_HIP);
     return VSC_OK;
 }
 

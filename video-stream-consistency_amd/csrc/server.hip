@@ -20,6 +20,7 @@
 
 #include <deque>
 #include <new>
+#include <string>
 #include <vector>
 
 namespace vsc {
@@ -28,8 +29,7 @@ namespace {
 struct Done {
     long long tag;
     long long frame;
-    int slot;
-    hipEvent_t ev;
+    int slot;   // output slot; its completion event is Stream::ev_done[slot]
 };
 
 struct Stream {
@@ -41,6 +41,10 @@ struct Stream {
     uint8_t* proc[3] = {nullptr, nullptr, nullptr};
     uint8_t* out[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipGraphExec_t ge[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // completion event per output slot (a slot holds at most one frame in
+    // flight) and one for reads: created at open, reused for every frame
+    hipEvent_t ev_done[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_read = nullptr;
     hipStream_t hs = nullptr;       // borrowed from the pool
     long long next_frame = 0;       // index of the next submitted frame
     long long ready = -1;           // highest window centre ready to run
@@ -86,8 +90,13 @@ void drop_graphs(Stream& st)
 void release(Stream& st)
 {
     drop_graphs(st);
-    for (Done& d : st.inflight) (void)hipEventDestroy(d.ev);
     st.inflight.clear();
+    for (int i = 0; i < R_OUT; ++i) {
+        if (st.ev_done[i]) (void)hipEventDestroy(st.ev_done[i]);
+        st.ev_done[i] = nullptr;
+    }
+    if (st.ev_read) (void)hipEventDestroy(st.ev_read);
+    st.ev_read = nullptr;
     if (st.pipe) vsc_pipeline_destroy(st.pipe);
     st.pipe = nullptr;
     if (st.orig[0]) (void)hipFree(st.orig[0]);
@@ -121,6 +130,9 @@ extern "C" int vsc_server_create(const vsc_server_cfg* cfg, const float* upsampl
     *out = nullptr;
     if (cfg->max_streams < 1 || cfg->max_streams > 4096 || cfg->hip_streams < 1 || cfg->hip_streams > 16)
         return set_error(VSC_E_INVALID, "vsc_server_create: max_streams in [1, 4096], hip_streams in [1, 16]");
+    if (cfg->form.overlap != 0)
+        return set_error(VSC_E_INVALID, "vsc_server_create: form.overlap must be 0 (the server orders each stream's "
+                                        "frames on its own HIP stream)");
     if (cfg->form.downscale > 1 && cfg->form.upsample == 1 && !upsampler_params)
         return set_error(VSC_E_INVALID, "vsc_server_create: the learned upsampler needs its parameters");
     vsc_server* sv = new (std::nothrow) vsc_server();
@@ -191,6 +203,9 @@ extern "C" int vsc_server_open(vsc_server* sv, int width, int height, int* id)
             for (int i = 0; i < R_OUT; ++i) st.out[i] = ring + fb * (6 + i);
         }
     }
+    for (int i = 0; i < R_OUT && !rc; ++i)
+        rc = check_hip(hipEventCreateWithFlags(&st.ev_done[i], hipEventDisableTiming), "vsc_server_open");
+    if (!rc) rc = check_hip(hipEventCreateWithFlags(&st.ev_read, hipEventDisableTiming), "vsc_server_open");
     if (rc) {
         release(st);
         return rc;
@@ -224,6 +239,9 @@ extern "C" int vsc_server_set_params(vsc_server* sv, int id, const vsc_params* p
 {
     Stream* st = get(sv, id);
     if (!st || !p) return set_error(VSC_E_INVALID, "vsc_server_set_params: no such stream or NULL params");
+    vsc_stream* vs = nullptr;
+    int rc = vsc_pipeline_stream(st->pipe, &vs);
+    if (rc || (rc = stream_params_check(vs, p))) return rc;   // refused: the stream keeps its parameters
     st->params = *p;   // read by the next tick (hyperParams are copied per frame, videostabilizer.cpp:192)
     drop_graphs(*st);  // captured graphs hold the old values as kernel arguments
     return VSC_OK;
@@ -256,53 +274,76 @@ extern "C" int vsc_server_submit(vsc_server* sv, int id, const uint8_t* orig, co
     return VSC_OK;
 }
 
+namespace {
+
+// Launch every ready window of one stream whose output slot is free.
+int tick_stream(vsc_server* sv, Stream& st, int* n)
+{
+    while (st.done_to < st.ready && st.done_to + 1 - R_OUT <= st.polled_to) {
+        const long long c = st.done_to + 1;
+        vsc_window w{st.orig[(c - 1) % 3], st.orig[c % 3], st.orig[(c + 1) % 3],
+                     st.proc[(c - 1) % 3], st.proc[c % 3], st.proc[(c + 1) % 3]};
+        int rc;
+        const int ph = (int)(c % N_PHASES);
+        if (sv->cfg.use_graphs && c >= 3 && st.ge[ph]) {
+            // steady state: replay the phase's graph, step the host state
+            if ((rc = pipeline_advance(st.pipe, c))) return rc;
+            if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
+        } else if (sv->cfg.use_graphs && c >= 3) {
+            hipGraph_t g = nullptr;
+            if (check_hip(hipStreamBeginCapture(st.hs, hipStreamCaptureModeThreadLocal), "vsc_server_tick capture"))
+                return VSC_E_HIP;
+            // a failed capture leaves the pipeline's host state as it was
+            // (vsc_pipeline_frame restores it), so the frame is retried whole
+            rc = vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
+                                    reinterpret_cast<vsc_hip_stream>(st.hs));
+            const int rce = check_hip(hipStreamEndCapture(st.hs, &g), "vsc_server_tick capture end");
+            if (rc || rce) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc ? rc : rce;
+            }
+            rc = check_hip(hipGraphInstantiate(&st.ge[ph], g, nullptr, nullptr, 0), "vsc_server_tick instantiate");
+            (void)hipGraphDestroy(g);
+            if (rc) {
+                st.ge[ph] = nullptr;
+                return rc;
+            }
+            if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
+        } else {
+            rc = vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
+                                    reinterpret_cast<vsc_hip_stream>(st.hs));
+            if (rc) return rc;
+        }
+        const int slot = (int)(c % R_OUT);
+        if (check_hip(hipEventRecord(st.ev_done[slot], st.hs), "vsc_server_tick")) return VSC_E_HIP;
+        st.inflight.push_back(Done{st.tags.front(), c, slot});
+        st.tags.pop_front();
+        st.done_to = c;
+        ++*n;
+    }
+    return VSC_OK;
+}
+
+}  // namespace
+
 extern "C" int vsc_server_tick(vsc_server* sv, int* launched)
 {
     if (!sv) return set_error(VSC_E_INVALID, "vsc_server_tick: NULL server");
-    int n = 0;
+    // a stream whose frame fails keeps that window ready (it is retried at the
+    // next tick) and does not hold up the other streams; the first failure's
+    // status is returned, its message in vsc_last_error
+    int n = 0, first_rc = VSC_OK;
+    std::string first_msg;
     for (Stream& st : sv->s) {
         if (!st.open) continue;
-        while (st.done_to < st.ready && st.done_to + 1 - R_OUT <= st.polled_to) {
-            const long long c = st.done_to + 1;
-            vsc_window w{st.orig[(c - 1) % 3], st.orig[c % 3], st.orig[(c + 1) % 3],
-                         st.proc[(c - 1) % 3], st.proc[c % 3], st.proc[(c + 1) % 3]};
-            int rc;
-            const int ph = (int)(c % N_PHASES);
-            if (sv->cfg.use_graphs && c >= 3 && st.ge[ph]) {
-                // steady state: replay the phase's graph, step the host state
-                if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
-                if ((rc = pipeline_advance(st.pipe, c))) return rc;
-            } else if (sv->cfg.use_graphs && c >= 3) {
-                hipGraph_t g = nullptr;
-                if (check_hip(hipStreamBeginCapture(st.hs, hipStreamCaptureModeThreadLocal), "vsc_server_tick capture"))
-                    return VSC_E_HIP;
-                rc = vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
-                                        reinterpret_cast<vsc_hip_stream>(st.hs));
-                const int rce = check_hip(hipStreamEndCapture(st.hs, &g), "vsc_server_tick capture end");
-                if (rc || rce) {
-                    if (g) (void)hipGraphDestroy(g);
-                    return rc ? rc : rce;
-                }
-                rc = check_hip(hipGraphInstantiate(&st.ge[ph], g, nullptr, nullptr, 0), "vsc_server_tick instantiate");
-                (void)hipGraphDestroy(g);
-                if (rc) return rc;
-                if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
-            } else {
-                rc = vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
-                                        reinterpret_cast<vsc_hip_stream>(st.hs));
-                if (rc) return rc;
-            }
-            Done d{st.tags.front(), c, (int)(c % R_OUT), nullptr};
-            st.tags.pop_front();
-            if (check_hip(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming), "vsc_server_tick") ||
-                check_hip(hipEventRecord(d.ev, st.hs), "vsc_server_tick"))
-                return VSC_E_HIP;
-            st.inflight.push_back(d);
-            st.done_to = c;
-            ++n;
+        const int rc = tick_stream(sv, st, &n);
+        if (rc && first_rc == VSC_OK) {
+            first_rc = rc;
+            first_msg = vsc_last_error();
         }
     }
     if (launched) *launched = n;
+    if (first_rc) return set_error(first_rc, first_msg);
     return VSC_OK;
 }
 
@@ -316,7 +357,7 @@ extern "C" int vsc_server_poll(vsc_server* sv, vsc_server_done* done, int max, i
         size_t k = 0;
         while (k < st.inflight.size() && n < max) {
             Done& d = st.inflight[k];
-            const hipError_t q = hipEventQuery(d.ev);
+            const hipError_t q = hipEventQuery(st.ev_done[d.slot]);
             if (q == hipErrorNotReady) break;   // later frames of this stream are not done either
             if (q != hipSuccess) return check_hip(q, "vsc_server_poll");
             done[n].stream = id;
@@ -325,7 +366,6 @@ extern "C" int vsc_server_poll(vsc_server* sv, vsc_server_done* done, int max, i
             done[n].out_rgba = st.out[d.slot];
             st.polled_to = d.frame;
             ++n;
-            (void)hipEventDestroy(d.ev);
             ++k;
         }
         st.inflight.erase(st.inflight.begin(), st.inflight.begin() + k);
@@ -349,14 +389,11 @@ extern "C" int vsc_server_read(vsc_server* sv, int id, long long frame, uint8_t*
     if (frame < 1 || frame > st->polled_to || frame + R_OUT <= st->done_to)
         return set_error(VSC_E_INVALID, "vsc_server_read: frame not reported or no longer in the output ring");
     const size_t fb = (size_t)st->W * st->H * 4;
-    hipEvent_t ev = nullptr;
+    // the read event is re-recorded by each read: a wait enqueued on `stream`
+    // holds the state it had when the wait was enqueued
     if (check_hip(hipMemcpyAsync(dst, st->out[frame % R_OUT], fb, hipMemcpyDefault, st->hs), "vsc_server_read") ||
-        check_hip(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "vsc_server_read") ||
-        check_hip(hipEventRecord(ev, st->hs), "vsc_server_read") ||
-        check_hip(hipStreamWaitEvent(as_stream(stream), ev, 0), "vsc_server_read")) {
-        if (ev) (void)hipEventDestroy(ev);
+        check_hip(hipEventRecord(st->ev_read, st->hs), "vsc_server_read") ||
+        check_hip(hipStreamWaitEvent(as_stream(stream), st->ev_read, 0), "vsc_server_read"))
         return VSC_E_HIP;
-    }
-    (void)hipEventDestroy(ev);
     return VSC_OK;
 }

@@ -46,15 +46,20 @@ int level_dims(int W, int H, int* lw, int* lh)
     return n;
 }
 
+// Bytes one carved piece of n floats occupies: every piece starts on a 256-byte
+// boundary.  footprint() and vsc_stream_create's carve both use it, so the
+// block's size is exactly the sum of the pieces carved from it.
+inline size_t piece_bytes(size_t nfloats) { return ((nfloats * sizeof(float) + 255) / 256) * 256; }
+
 size_t footprint(int W, int H, int fw, int fh, int* nlev_out = nullptr)
 {
     int lw[VSC_MAX_LEVELS], lh[VSC_MAX_LEVELS];
     const int n = level_dims(W, H, lw, lh);
-    size_t floats = 0;
-    for (int j = 0; j < n; ++j) floats += (size_t)8 * 3 * lw[j] * lh[j];
-    if (fw != W || fh != H) floats += (size_t)2 * 3 * W * H;
+    size_t bytes = 0;
+    for (int j = 0; j < n; ++j) bytes += 8 * piece_bytes((size_t)3 * lw[j] * lh[j]);
+    if (fw != W || fh != H) bytes += 2 * piece_bytes((size_t)3 * W * H);
     if (nlev_out) *nlev_out = n;
-    return floats * sizeof(float) + 64 * 16;
+    return bytes;
 }
 
 inline View pl(float* d, int w, int h) { return planar(d, w, h, 3); }
@@ -91,7 +96,7 @@ extern "C" int vsc_stream_create(int width, int height, int flow_width, int flow
     char* p = static_cast<char*>(s->block);
     auto take = [&](size_t nfloats) {
         float* r = reinterpret_cast<float*>(p);
-        p += ((nfloats * sizeof(float) + 255) / 256) * 256;
+        p += piece_bytes(nfloats);
         return r;
     };
     for (int j = 0; j < s->nlev; ++j) {
@@ -108,6 +113,11 @@ extern "C" int vsc_stream_create(int width, int height, int flow_width, int flow
     if (flow_width != width || flow_height != height) {
         s->flow_up[0] = take((size_t)3 * width * height);
         s->flow_up[1] = take((size_t)3 * width * height);
+    }
+    if ((size_t)(p - static_cast<char*>(s->block)) != s->bytes) {
+        hipFree(s->block);
+        delete s;
+        return set_error(VSC_E_INVALID, "vsc_stream_create: internal error: carve does not match the footprint");
     }
     *out = s;
     return VSC_OK;
@@ -182,6 +192,35 @@ int check_params(const vsc_stream* s, const vsc_params& p)
     if (p.num_iter < 0) return set_error(VSC_E_INVALID, "vsc_stab_frame: num_iter < 0");
     return VSC_OK;
 }
+
+}  // namespace
+
+// The checks vsc_stab_frame makes before it enqueues anything, for callers
+// that change their own state first (the pipeline, the server): parameters
+// (NULL = defaults) valid for this stream, and a previous frame set.
+namespace vsc {
+
+int stream_params_check(const vsc_stream* s, const vsc_params* params)
+{
+    if (!s) return set_error(VSC_E_INVALID, "vsc_stab_frame: null stream");
+    vsc_params p;
+    if (params) p = *params;
+    else vsc_default_params(&p);
+    return check_params(s, p);
+}
+
+int stream_frame_check(const vsc_stream* s, const vsc_params* params)
+{
+    int rc = stream_params_check(s, params);
+    if (rc) return rc;
+    if (!s->has_last)
+        return set_error(VSC_E_INVALID, "vsc_stab_frame: no previous stabilized frame (call vsc_stream_set_last*)");
+    return VSC_OK;
+}
+
+}  // namespace vsc
+
+namespace {
 
 // Flows at frame size: either the caller's buffers or the get_bilinear
 // upsample of the FLOWDOWNSCALE flows (flowmodel.cpp:158-161; vectors are not

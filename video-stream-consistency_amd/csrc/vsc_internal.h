@@ -327,6 +327,10 @@ int upsampler_run_flows(vsc_upsampler* u, int n, const float* const flow_lo[2], 
 
 // host-state step of a pipeline frame replayed from a graph (pipeline.hip)
 int pipeline_advance(vsc_pipeline* p, long long frame);
+// parameter and state checks of vsc_stab_frame / vsc_pipeline_frame, made
+// before any state change (NULL params = defaults)
+int stream_params_check(const vsc_stream* s, const vsc_params* params);
+int stream_frame_check(const vsc_stream* s, const vsc_params* params);
 
 // reduced-precision (BF16 / F16 / FP8 e4m3) correlation on MFMA, corr_mfma.hip
 int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
