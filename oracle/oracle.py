@@ -72,6 +72,9 @@ def lib():
             g = getattr(L, "vsco_stab_frame_" + name)
             g.restype = None
             g.argtypes = [p] * 8 + [_i, p, _f, _f, _f, _i, _i, _f, _f, p, _i, _i]
+        L.vsco_stab_frame_mutant_f64.restype = None
+        L.vsco_stab_frame_mutant_f64.argtypes = [_f64p] * 8 + [_i, _f64p, _f, _f, _f, _i, _i, _f, _f, _f64p, _i,
+                                                               _i, _i]
         L.vsco_stab_consist_out_fma_f32.restype = None
         L.vsco_stab_consist_out_fma_f32.argtypes = [_f32p, _f32p, _f32p, _i, _f, _f, _f32p, _i, _i]
         L.vsco_stab_frame_fma_f32.restype = None
@@ -245,6 +248,25 @@ def stab_frame(o_prev, o_cur, o_next, p_prev, p_cur, p_next, fwd, bwd, last, par
     _solver_fn("vsco_stab_frame_", solver, dtype)(
         *ims, fwd, bwd, fwd.shape[2], last, prm["alpha"], prm["beta"], prm["gamma"],
         prm["pyramid_levels"], prm["num_iter"], prm["step_size"], prm["mom_fac"], out, W, H)
+    return out
+
+
+def stab_frame_mutant(o_prev, o_cur, o_next, p_prev, p_cur, p_next, fwd, bwd, last, params=None, mutation=0):
+    """stab_frame in float64 with a planted schedule mistake (see
+    FrameParams::mutation in vsc_oracle.cpp); for tests that show the
+    schedule pins detect such mistakes."""
+    prm = dict(DEFAULT_PARAMS)
+    if params:
+        prm.update(params)
+    ims = [_c(a, np.float64) for a in (o_prev, o_cur, o_next, p_prev, p_cur, p_next)]
+    fwd = _c(fwd, np.float64)
+    bwd = _c(bwd, np.float64)
+    last = _c(last, np.float64)
+    H, W, _ = ims[0].shape
+    out = np.empty((H, W, 3), dtype=np.float64)
+    lib().vsco_stab_frame_mutant_f64(*ims, fwd, bwd, fwd.shape[2], last, prm["alpha"], prm["beta"], prm["gamma"],
+                                     prm["pyramid_levels"], prm["num_iter"], prm["step_size"], prm["mom_fac"], out,
+                                     W, H, int(mutation))
     return out
 
 

@@ -577,6 +577,13 @@ struct FrameParams {
     int pyramid_levels, num_iter;
     float step_size, mom_fac;
     bool fma;  /* solver in FMA form (stab_consist_out_fma) */
+    /* Planted schedule mistakes, used only by the tests that show the
+     * schedule pins detect them (0 = the reference schedule):
+     * 1 prev frames warped with the forward flow, 2 next frames warped with
+     * the backward flow, 3 last output warped with the forward flow,
+     * 4 every level runs num_iter sweeps, 5 level j runs num_iter/(j+1)+1,
+     * 6 the finer level keeps P instead of the upsampled coarse solution. */
+    int mutation = 0;
 };
 
 /* Inputs: the 3-frame window of originals and processed frames as RGB float
@@ -596,11 +603,15 @@ static void stab_frame(const T* oPrev, const T* oCur, const T* oNext, const T* p
 {
     const size_t n = (size_t)W * H * 3;
     std::vector<T> pwi(n), pwp(n), nwi(n), nwp(n), lw(n), adIn(n), adPr(n), wt(n);
-    stab_warp<T>(oPrev, bwd, flow_ch, pwi.data(), W, H);
-    stab_warp<T>(pPrev, bwd, flow_ch, pwp.data(), W, H);
-    stab_warp<T>(oNext, fwd, flow_ch, nwi.data(), W, H);
-    stab_warp<T>(pNext, fwd, flow_ch, nwp.data(), W, H);
-    stab_warp<T>(last, bwd, flow_ch, lw.data(), W, H);
+    const int mu = prm.mutation;
+    const T* prevFlow = mu == 1 ? fwd : bwd;
+    const T* nextFlow = mu == 2 ? bwd : fwd;
+    const T* lastFlow = mu == 3 ? fwd : bwd;
+    stab_warp<T>(oPrev, prevFlow, flow_ch, pwi.data(), W, H);
+    stab_warp<T>(pPrev, prevFlow, flow_ch, pwp.data(), W, H);
+    stab_warp<T>(oNext, nextFlow, flow_ch, nwi.data(), W, H);
+    stab_warp<T>(pNext, nextFlow, flow_ch, nwp.data(), W, H);
+    stab_warp<T>(last, lastFlow, flow_ch, lw.data(), W, H);
     stab_adap_comb<T>(oCur, pCur, pwi.data(), pwp.data(), nwi.data(), nwp.data(), adIn.data(), adPr.data(), lw.data(), W,
                       H, (T)prm.alpha);
     stab_consist_wt<T>(adIn.data(), oCur, wt.data(), W, H, (T)prm.beta, (T)prm.gamma);
@@ -629,14 +640,15 @@ static void stab_frame(const T* oPrev, const T* oCur, const T* oNext, const T* p
         }
     }
     for (int j = L - 1; j >= 0; --j) {
-        if (j != L - 1)
+        if (j != L - 1 && mu != 6)
             stab_bilinear<T>(pO[j + 1].data(), lw_[j + 1], lh_[j + 1], 3, pO[j].data(), lw_[j], lh_[j], 3);
+        const int iters = mu == 4 ? prm.num_iter : mu == 5 ? prm.num_iter / (j + 1) + 1 : prm.num_iter / (j + 1);
         if (prm.fma)
-            stab_consist_out_fma<T>(pP[j].data(), pA[j].data(), pW[j].data(), prm.num_iter / (j + 1),
-                                    (T)prm.step_size, (T)prm.mom_fac, pO[j].data(), lw_[j], lh_[j]);
+            stab_consist_out_fma<T>(pP[j].data(), pA[j].data(), pW[j].data(), iters, (T)prm.step_size,
+                                    (T)prm.mom_fac, pO[j].data(), lw_[j], lh_[j]);
         else
-            stab_consist_out<T>(pP[j].data(), pA[j].data(), pW[j].data(), prm.num_iter / (j + 1), (T)prm.step_size,
-                                (T)prm.mom_fac, pO[j].data(), lw_[j], lh_[j]);
+            stab_consist_out<T>(pP[j].data(), pA[j].data(), pW[j].data(), iters, (T)prm.step_size, (T)prm.mom_fac,
+                                pO[j].data(), lw_[j], lh_[j]);
     }
     std::memcpy(out, pO[0].data(), n * sizeof(T));
 }
@@ -665,6 +677,19 @@ VSCO_API void vsco_stab_frame_f64(const double* oPrev, const double* oCur, const
                                   int iters, float step, float mom, double* out, int W, int H)
 {
     FrameParams p{alpha, beta, gamma, levels, iters, step, mom, false};
+    stab_frame<double>(oPrev, oCur, oNext, pPrev, pCur, pNext, fwd, bwd, flow_ch, last, p, out, W, H);
+}
+
+/* stab_frame with a planted schedule mistake (FrameParams::mutation), float64:
+ * only for the tests that show the schedule pins fail on it. */
+VSCO_API void vsco_stab_frame_mutant_f64(const double* oPrev, const double* oCur, const double* oNext,
+                                         const double* pPrev, const double* pCur, const double* pNext,
+                                         const double* fwd, const double* bwd, int flow_ch, const double* last,
+                                         float alpha, float beta, float gamma, int levels, int iters, float step,
+                                         float mom, double* out, int W, int H, int mutation)
+{
+    FrameParams p{alpha, beta, gamma, levels, iters, step, mom, false};
+    p.mutation = mutation;
     stab_frame<double>(oPrev, oCur, oNext, pPrev, pCur, pNext, fwd, bwd, flow_ch, last, p, out, W, H);
 }
 
