@@ -102,15 +102,34 @@ int vsc_default_params(vsc_params* out);
 int vsc_correlation_forward(const void* f1, const void* f2, void* out, int N, int C, int H, int W,
                             int max_displacement, int legacy, int in_dtype, const float* scale,
                             vsc_hip_stream stream);
+/* The same op with a caller-owned workspace (no allocation in the call; the
+ * reference allocates and frees its channels-last temporaries on every call,
+ * correlation_cuda.cu:365-370,440-441).  BF16 / F16 / FP8E4M3 inputs are
+ * copied channels-last into the workspace (two tensors of N*H*W*ceil32(C)
+ * elements); F32 and F64 need none (size 0, workspace may be NULL).
+ * vsc_correlation_forward above allocates that workspace stream-ordered
+ * (hipMallocAsync / hipFreeAsync) when the type needs it.
+ * Errors: as vsc_correlation_forward, and VSC_E_INVALID when the workspace is
+ * NULL or smaller than vsc_correlation_workspace_size().                    */
+size_t vsc_correlation_workspace_size(int N, int C, int H, int W, int in_dtype);
+int vsc_correlation_forward_ws(const void* f1, const void* f2, void* out, int N, int C, int H, int W,
+                               int max_displacement, int legacy, int in_dtype, const float* scale, void* workspace,
+                               size_t workspace_bytes, vsc_hip_stream stream);
 
 /* Quantiser for the FP8 path, on the device: per-channel equalisation
  * r_c = sqrt(max|f2_c| / max|f1_c|) (SmoothQuant), f1' = f1*r_c, f2' = f2/r_c,
  * which leaves every product f1_c*f2_c unchanged, then one per-tensor scale
  * per operand (s1 = max|f1'|/448, s2 = max|f2'|/448) and round-to-nearest-even
  * E4M3.  Writes q1, q2 (1 byte per element) and scale[0] = s1*s2.
- * Allocates a small stream-ordered workspace.                               */
+ * Allocates a small stream-ordered workspace (vsc_quantize_fp8_pair_ws takes
+ * the caller's instead).                                                    */
 int vsc_quantize_fp8_pair(const float* f1, const float* f2, uint8_t* q1, uint8_t* q2, float* scale,
                           int N, int C, int H, int W, vsc_hip_stream stream);
+/* The quantiser with a caller-owned workspace of
+ * vsc_quantize_fp8_pair_workspace_size(C) bytes (no allocation in the call). */
+size_t vsc_quantize_fp8_pair_workspace_size(int C);
+int vsc_quantize_fp8_pair_ws(const float* f1, const float* f2, uint8_t* q1, uint8_t* q2, float* scale, int N, int C,
+                             int H, int W, void* workspace, size_t workspace_bytes, vsc_hip_stream stream);
 
 /* FP32 -> BF16 / F16 / FP8E4M3 element conversion on the device, round to
  * nearest even (FP8 saturates at +-448).                                   */
@@ -501,6 +520,36 @@ int vsc_server_read(vsc_server* sv, int stream_id, long long frame, uint8_t* dst
  * them.  Flows are HWC (u, v) float32, little-endian.                        */
 int vsc_flo_read(const char* path, float* flow, size_t capacity_floats, int* width, int* height);
 int vsc_flo_write(const char* path, const float* flow, int width, int height);
+/* Flow-file mode of the file stabilizer (`-f <directory>`, main.cpp:60-63):
+ * FileStabilizer::retrieveOpticalFlow (stabilizefiles.cpp:134-149) reads, for
+ * window `frame` = i, the forward flow i -> i+1 from frame_<i+1>.flo and the
+ * backward flow i -> i-1 from frame_<i>_bwd.flo (indices zero-padded to 6
+ * digits, formatIndex, stabilizefiles.cpp:99-101).
+ * vsc_flow_dir_path: that file's path (backward 0: forward flow).
+ * vsc_flow_dir_read: both flows (host, HWC (u, v) float32; either pointer may
+ * be NULL, both NULL = sizes only); VSC_E_INVALID when a file is missing or
+ * malformed, or the two differ in size.                                     */
+int vsc_flow_dir_path(const char* dir, long long frame, int backward, char* path, size_t capacity);
+int vsc_flow_dir_read(const char* dir, long long frame, float* fwd, float* bwd, size_t capacity_floats, int* width,
+                      int* height);
+/* Receives each output frame in order (host RGBA8, valid during the call);
+ * a nonzero return stops the loop. */
+typedef int (*vsc_frame_sink)(void* user, long long index, const uint8_t* rgba, int width, int height);
+/* The -f frame loop: FileStabilizer::stabilizeAll (stabilizefiles.cpp:108-132)
+ * with preloadProcessedFrames / doOneStep / outputFinalFrames
+ * (videostabilizer.cpp:136-238), k = 1, batch 1.  orig_rgba / proc_rgba:
+ * nframes (>= 3) host RGBA8 frames of width x height (decoded by the caller).
+ * Frame 0 and frame nframes-1 leave as the processed frames, converted to
+ * float and back as the reference's gpuToImage does (alpha 1); frame i in
+ * 1..nframes-2 is stabilized with the flows of vsc_flow_dir_read(flow_dir, i),
+ * the first `last` being processed frame 2.  A flow whose size differs from
+ * the frame is refused (VSC_E_INVALID), as initializeFlowImage throws
+ * (imagehelpers.cpp:42-50).  params: NULL = defaults; mode VSC_FRAME_FUSED
+ * or VSC_FRAME_FMA.  Allocates its device buffers once per call and
+ * synchronises `stream` once per frame (before the sink sees the frame).  */
+int vsc_stabilize_flow_dir(const uint8_t* const* orig_rgba, const uint8_t* const* proc_rgba, int nframes, int width,
+                           int height, const char* flow_dir, const vsc_params* params, int mode, vsc_frame_sink sink,
+                           void* user, vsc_hip_stream stream);
 
 /* ---------------------------------------------------------------------------
  * Per-kernel timing (replaces the per-stage QElapsedTimer accounting of

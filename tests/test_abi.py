@@ -133,3 +133,21 @@ def test_stream_footprint_covers_the_carve(vsc, size):
     got = L.vsc_stream_footprint(W, H, fw, fh)
     assert got >= need, (got, need)
     assert got % 256 == 0
+
+
+def test_workspace_entry_points_check_on_the_host(vsc):
+    """The workspace forms refuse a missing or short workspace before any GPU
+    work; FP32 and FP64 correlation need none."""
+    L = vsc.lib()
+    dummy = ctypes.c_void_p(16)
+    n = L.vsc_correlation_workspace_size(1, 33, 17, 30, vsc.VSC_BF16)
+    assert n == 2 * (-(-(17 * 30 * 64 * 2) // 256) * 256)          # channels padded to 64, 2 tensors
+    assert L.vsc_correlation_workspace_size(1, 32, 17, 30, vsc.VSC_FP8E4M3) == 2 * (-(-(17 * 30 * 32) // 256) * 256)
+    assert L.vsc_correlation_workspace_size(1, 32, 17, 30, vsc.VSC_F32) == 0
+    assert L.vsc_correlation_forward_ws(dummy, dummy, dummy, 1, 33, 17, 30, 4, 0, vsc.VSC_BF16, None, None, 0,
+                                        None) == -1
+    assert L.vsc_correlation_forward_ws(dummy, dummy, dummy, 1, 33, 17, 30, 4, 0, vsc.VSC_BF16, None, dummy, n - 1,
+                                        None) == -1
+    assert b"workspace" in L.vsc_last_error()
+    assert L.vsc_quantize_fp8_pair_workspace_size(196) == (3 * 196 + 4) * 4
+    assert L.vsc_quantize_fp8_pair_ws(dummy, dummy, dummy, dummy, dummy, 1, 196, 4, 4, dummy, 100, None) == -1

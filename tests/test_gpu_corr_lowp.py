@@ -131,3 +131,31 @@ def test_corr_fp8_equalised(vsc, orc, cuda, C, H, W):
     rel = np.max(np.abs(got - ref) / (b0 + 1e-30))
     print(f"fp8 max |err|/sum|ab| = {rel:.4f}")
     assert rel <= 2 * 2.0 ** -4 * (1 + 2.0 ** -4) + 0.02
+
+
+@pytest.mark.parametrize("kind", ["bf16", "f16", "fp8"])
+def test_corr_workspace_form_equals_allocating_form(vsc, cuda, kind):
+    """vsc_correlation_forward_ws (caller's workspace, no allocation) gives the
+    allocating entry point's bits at every pyramid level, with one workspace of
+    the largest level's size reused across calls; likewise the FP8 quantiser."""
+    import torch
+    rng = np.random.default_rng(9)
+    code = {"bf16": vsc.VSC_BF16, "f16": vsc.VSC_F16, "fp8": vsc.VSC_FP8E4M3}[kind]
+    need = max(vsc.correlation_workspace_bytes((1, C, H, W), code) for C, H, W in LEVELS)
+    ws = torch.empty(need, dtype=torch.uint8, device=cuda)
+    qws = torch.empty(vsc.lib().vsc_quantize_fp8_pair_workspace_size(196), dtype=torch.uint8, device=cuda)
+    for C, H, W in LEVELS:
+        a = _t(rng.standard_normal((1, C, H, W)).astype(np.float32), cuda)
+        b = _t(rng.standard_normal((1, C, H, W)).astype(np.float32), cuda)
+        if kind == "fp8":
+            q1, q2, sc = vsc.quantize_fp8_pair(a, b)
+            w1, w2, wsc = vsc.quantize_fp8_pair(a, b, workspace=qws)
+            assert torch.equal(q1, w1) and torch.equal(q2, w2) and torch.equal(sc, wsc)
+            ref = vsc.correlation_fp8(q1, q2, sc, 4, 0)
+            got = vsc.correlation_fp8(q1, q2, sc, 4, 0, workspace=ws)
+        else:
+            ca, cb = vsc.convert(a, kind), vsc.convert(b, kind)
+            ref = vsc.correlation(ca, cb, 4, 0)
+            got = vsc.correlation(ca, cb, 4, 0, workspace=ws)
+        torch.cuda.synchronize()
+        assert torch.equal(ref, got), (kind, C, H, W)

@@ -20,7 +20,7 @@ def _run(vsc, cuda, streams, order, form="chosen", hip_streams=4):
     """streams: {name: (W, H, orig, proc)}; order: list of names, one submission
     each (in that order), ticks after every submission.  Returns {name: {frame: out}}."""
     import torch
-    sv = vsc.Server(form=form, hip_streams=hip_streams)
+    sv = vsc.Server(form=form, hip_streams=hip_streams, max_streams=max(64, len(streams)))
     ids = {k: sv.open(v[0], v[1]) for k, v in streams.items()}
     frames = {k: torch.from_numpy(np.concatenate([v[2][None], v[3][None]])).to(cuda) for k, v in streams.items()}
     nxt = {k: 0 for k in streams}
@@ -216,3 +216,41 @@ def test_server_refuses_bad_params(vsc, cuda):
     assert sorted(outs[True]) == [1, 2, 3, 4, 5]
     for f in outs[False]:
         assert torch.equal(outs[False][f], outs[True][f]), f"frame {f}"
+
+
+def test_serving_mix_each_stream_equals_its_pipeline(vsc, cuda):
+    """The serving mix at its real sizes: 24 streams (8 x 640x360,
+    8 x 1280x720, 6 x 1920x1080, 2 x 3840x2160), each its own seeded video,
+    frames submitted in a shuffled interleaving (each stream in order).  Every
+    stabilized frame equals the bits of that stream run alone through
+    vsc_pipeline, and the committed device memory is the sum of the streams'
+    footprints."""
+    import torch
+    nfr = 5
+    vids = {}
+    for i, (w, h) in enumerate(vsc.SERVE_MIX):
+        o, p = vsc.serve_stream_video(i, nfr)
+        vids[f"s{i:02d}"] = (w, h, o, p)
+    probe = vsc.Server(hbm_budget=1 << 40, max_streams=len(vids))
+    need = sum(probe.footprint(w, h) for (w, h) in vsc.SERVE_MIX)
+    probe.close()
+    rng = np.random.default_rng(11)
+    order = [k for k in vids for _ in range(nfr)]
+    rng.shuffle(order)
+    sv = vsc.Server(hbm_budget=need, max_streams=len(vids))
+    ids = {k: sv.open(v[0], v[1]) for k, v in vids.items()}
+    assert sv.used_bytes() == need
+    with pytest.raises(vsc.VscError):
+        sv.open(640, 360)                      # the budget is exactly the mix
+    sv.close()
+    mix = _run(vsc, cuda, vids, order)
+    for k, (w, h, orig, proc) in vids.items():
+        assert sorted(mix[k]) == list(range(1, nfr - 1)), k
+        o = torch.from_numpy(orig).to(cuda)
+        p = torch.from_numpy(proc).to(cuda)
+        pipe = vsc.Pipeline(w, h, form="chosen")
+        pipe.set_last_rgba(p[2])
+        for t in range(1, nfr - 1):
+            ref = pipe.frame((o[t - 1], o[t], o[t + 1], p[t - 1], p[t], p[t + 1]), t).cpu()
+            assert torch.equal(ref, mix[k][t]), f"{k} ({w}x{h}) frame {t}"
+        pipe.close()

@@ -256,3 +256,72 @@ def test_consist_out_planar_blocked(vsc, orc, cuda, H, W, iters):
     vsc.get_consist_out_planar(pl(P), pl(A), pl(Wt), iters, 0.15, 0.15, out)
     got = out.cpu().numpy().transpose(1, 2, 0)
     assert np.array_equal(got, orc.stab_consist_out(P, A, Wt, O, iters))
+
+
+@pytest.mark.parametrize("mode_name", ["VSC_FRAME_FUSED", "VSC_FRAME_FMA"])
+def test_frame_uniform_invariant_and_zero_iterations(vsc, cuda, mode_name):
+    """Through the fused device frame at 640x360: uniform frames with zero flow
+    and processed = original come out within 1 LSB of the input, at 2 and 3
+    pyramid levels; with num_iter = 0 the output is the pyramid's 2x
+    resampling of the processed frame (decimation, then midpoints)."""
+    import torch
+    W, H = 640, 360
+    mode = getattr(vsc, mode_name)
+    zero = torch.zeros((H, W, 2), dtype=torch.float32, device=cuda)
+    for colour in ((37, 200, 128), (254, 3, 99)):
+        rgba = torch.tensor(colour + (255,), dtype=torch.uint8, device=cuda).expand(H, W, 4).contiguous()
+        for levels in (2, 3):
+            st = vsc.VideoStabilizer(W, H, mode=mode)
+            st.set_last_rgba(rgba)
+            hp = vsc.HyperParams()
+            hp.pyramidLevels = levels
+            out = st.do_one_step([rgba] * 3, [rgba] * 3, zero, zero, params=hp).cpu().numpy().astype(int)
+            st.close()
+            assert np.abs(out[..., :3] - np.array(colour)).max() <= 1, (colour, levels)
+    rng = np.random.default_rng(2)
+    rgba = [_t(rng.integers(0, 256, (H, W, 4), dtype=np.uint8), cuda) for _ in range(6)]
+    st = vsc.VideoStabilizer(W, H, mode=mode)
+    st.set_last_rgba(rgba[5])
+    hp = vsc.HyperParams()
+    hp.numIter = 0
+    of = torch.empty((H, W, 3), dtype=torch.float32, device=cuda)
+    fl = _t(rng.uniform(-3, 3, (H, W, 2)).astype(np.float32), cuda)
+    st.do_one_step(rgba[0:3], rgba[3:6], fl, fl, out_float=of, params=hp)
+    got = of.cpu().numpy().astype(np.float64)
+    st.close()
+    d = (rgba[4].cpu().numpy()[..., :3].astype(np.float32) / np.float32(255.0)).astype(np.float64)[::2, ::2]
+    nx = np.minimum(np.arange(d.shape[1]) + 1, d.shape[1] - 1)
+    ny = np.minimum(np.arange(d.shape[0]) + 1, d.shape[0] - 1)
+    ref = np.empty((H, W, 3))
+    ref[0::2, 0::2] = d
+    ref[0::2, 1::2] = 0.5 * (d + d[:, nx])
+    ref[1::2, 0::2] = 0.5 * (d + d[ny, :])
+    ref[1::2, 1::2] = 0.25 * (d + d[:, nx] + d[ny, :] + d[ny][:, nx])
+    assert np.abs(got - ref).max() <= 3e-7
+
+
+def test_flow_dir_frame_loop_matches_oracle(vsc, orc, cuda, tmp_path):
+    """The -f frame loop (FileStabilizer with a flow directory): flows written
+    as .flo files under the reference's names, frames from host memory; the
+    outputs are the oracle's sequence bit for bit (frame 0 and the last frame
+    processed, the others stabilized with `last` carried), and a flow of the
+    wrong size is refused."""
+    W, H, n = 160, 96, 6
+    orig, proc, fwd, bwd = vsc.synth_video(width=W, height=H, nframes=n, nblobs=40, motion=3.0, npatches=1,
+                                           patch_size=16, seed=8)
+    for i in range(1, n - 1):
+        vsc.write_flo(tmp_path / f"frame_{i + 1:06d}.flo", fwd[i])
+        vsc.write_flo(tmp_path / f"frame_{i:06d}_bwd.flo", bwd[i])
+    outs = vsc.stabilize_flow_dir(orig, proc, tmp_path, mode=vsc.VSC_FRAME_FUSED)
+    ref, _ = orc.stabilize_sequence(orig, proc, fwd, bwd)
+    assert len(outs) == n
+    for i in range(n):
+        assert np.array_equal(outs[i], ref[i]), f"frame {i}"
+    fm = vsc.stabilize_flow_dir(orig, proc, tmp_path, mode=vsc.VSC_FRAME_FMA)
+    reff, _ = orc.stabilize_sequence(orig, proc, fwd, bwd, solver="fma")
+    for i in range(n):
+        assert np.array_equal(fm[i], reff[i]), f"FMA frame {i}"
+    vsc.write_flo(tmp_path / f"frame_{3:06d}.flo", np.zeros((H // 2, W // 2, 2), np.float32))
+    vsc.write_flo(tmp_path / f"frame_{2:06d}_bwd.flo", np.zeros((H // 2, W // 2, 2), np.float32))
+    with pytest.raises(vsc.VscError):
+        vsc.stabilize_flow_dir(orig, proc, tmp_path)

@@ -58,7 +58,14 @@ EXPORTED = (
     "vsc_server_default_cfg", "vsc_server_create", "vsc_server_destroy", "vsc_server_stream_footprint",
     "vsc_server_used_bytes", "vsc_server_open", "vsc_server_close", "vsc_server_set_params", "vsc_server_submit",
     "vsc_server_tick", "vsc_server_poll", "vsc_server_drain", "vsc_server_read", "vsc_flo_read", "vsc_flo_write",
+    "vsc_flow_dir_path", "vsc_flow_dir_read", "vsc_stabilize_flow_dir", "vsc_correlation_workspace_size",
+    "vsc_correlation_forward_ws", "vsc_quantize_fp8_pair_workspace_size", "vsc_quantize_fp8_pair_ws",
 )
+
+
+#: vsc_frame_sink: (user, index, rgba, width, height) -> int
+FRAME_SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_int,
+                              ctypes.c_int)
 
 
 class VscImage(ctypes.Structure):
@@ -137,6 +144,12 @@ def lib():
         L.vsc_default_params.argtypes = [ctypes.POINTER(VscParams)]
         L.vsc_correlation_forward.argtypes = [vp, vp, vp, i, i, i, i, i, i, i, vp, vp]
         L.vsc_quantize_fp8_pair.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, vp]
+        L.vsc_quantize_fp8_pair_workspace_size.argtypes = [i]
+        L.vsc_quantize_fp8_pair_workspace_size.restype = ctypes.c_size_t
+        L.vsc_quantize_fp8_pair_ws.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, vp, ctypes.c_size_t, vp]
+        L.vsc_correlation_workspace_size.argtypes = [i, i, i, i, i]
+        L.vsc_correlation_workspace_size.restype = ctypes.c_size_t
+        L.vsc_correlation_forward_ws.argtypes = [vp, vp, vp, i, i, i, i, i, i, i, vp, vp, ctypes.c_size_t, vp]
         L.vsc_convert_f32.argtypes = [vp, vp, ctypes.c_size_t, i, vp]
         L.vsc_warp_forward.argtypes = [vp, vp, vp, i, i, i, i, i, vp]
         L.vsc_stab_warp.argtypes = [img, img, img, vp]
@@ -221,6 +234,11 @@ def lib():
         L.vsc_server_read.argtypes = [vp, i, ctypes.c_longlong, vp, vp]
         L.vsc_flo_read.argtypes = [ctypes.c_char_p, vp, ctypes.c_size_t, ctypes.POINTER(i), ctypes.POINTER(i)]
         L.vsc_flo_write.argtypes = [ctypes.c_char_p, vp, i, i]
+        L.vsc_flow_dir_path.argtypes = [ctypes.c_char_p, ctypes.c_longlong, i, ctypes.c_char_p, ctypes.c_size_t]
+        L.vsc_flow_dir_read.argtypes = [ctypes.c_char_p, ctypes.c_longlong, vp, vp, ctypes.c_size_t,
+                                        ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.vsc_stabilize_flow_dir.argtypes = [vp, vp, i, i, i, ctypes.c_char_p, ctypes.POINTER(VscParams), i,
+                                             FRAME_SINK, vp, vp]
         L.vsc_profile_begin.argtypes = [i]
         L.vsc_profile_end.argtypes = [ctypes.c_char_p, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
@@ -464,12 +482,20 @@ def _dtype_code(t):
     return _DT[t.dtype]
 
 
-def correlation(input1, input2, max_displacement: int = 4, legacy: int = 0, scale=None, out=None):
+def correlation_workspace_bytes(shape, dtype_code):
+    """Device bytes vsc_correlation_forward_ws needs for NCHW `shape`."""
+    N, C, H, W = shape
+    return int(lib().vsc_correlation_workspace_size(N, C, H, W, int(dtype_code)))
+
+
+def correlation(input1, input2, max_displacement: int = 4, legacy: int = 0, scale=None, out=None, workspace=None):
     """``custom::Correlation`` (correlation.h:60-84; correlation.cc:45-143).
 
     input1, input2: NCHW device tensors of float32 / float64 / bfloat16 /
     float16 / float8_e4m3fn.  Returns [N, 2md+1, 2md+1, H, W] (legacy=0) or
-    [N, (2md+1)^2, H, W] (legacy=1), float32 (float64 for float64 inputs)."""
+    [N, (2md+1)^2, H, W] (legacy=1), float32 (float64 for float64 inputs).
+    workspace: a device uint8 tensor of correlation_workspace_bytes() bytes
+    (vsc_correlation_forward_ws: no allocation in the call)."""
     torch = _torch()
     if input1.shape != input2.shape or input1.dim() != 4:
         raise VscError(-1, "Correlation expects two NCHW tensors of the same shape")
@@ -482,6 +508,11 @@ def correlation(input1, input2, max_displacement: int = 4, legacy: int = 0, scal
     if out is None:
         out = torch.empty(shape, dtype=odt, device=a.device)
     cs = None if scale is None else _ptr(scale.contiguous())
+    if workspace is not None:
+        check(lib().vsc_correlation_forward_ws(_ptr(a), _ptr(b), _ptr(out), N, C, H, W, int(max_displacement),
+                                               int(bool(legacy)), _dtype_code(a), cs, _ptr(workspace),
+                                               workspace.numel() * workspace.element_size(), _stream()))
+        return out
     check(lib().vsc_correlation_forward(_ptr(a), _ptr(b), _ptr(out), N, C, H, W, int(max_displacement),
                                         int(bool(legacy)), _dtype_code(a), cs, _stream()))
     return out
@@ -501,9 +532,10 @@ def warp(input, flow, out=None):
     return out
 
 
-def quantize_fp8_pair(f1, f2):
+def quantize_fp8_pair(f1, f2, workspace=None):
     """Equalised E4M3 operands for the FP8 correlation: returns q1, q2 (uint8
-    tensors of E4M3 bits) and scale (1 float): f1.f2 ~= scale * q1.q2."""
+    tensors of E4M3 bits) and scale (1 float): f1.f2 ~= scale * q1.q2.
+    workspace: optional device uint8 tensor (vsc_quantize_fp8_pair_ws)."""
     torch = _torch()
     a = f1.contiguous().float()
     b = f2.contiguous().float()
@@ -511,17 +543,28 @@ def quantize_fp8_pair(f1, f2):
     q1 = torch.empty(a.shape, dtype=torch.uint8, device=a.device)
     q2 = torch.empty(a.shape, dtype=torch.uint8, device=a.device)
     sc = torch.empty(1, dtype=torch.float32, device=a.device)
+    if workspace is not None:
+        check(lib().vsc_quantize_fp8_pair_ws(_ptr(a), _ptr(b), _ptr(q1), _ptr(q2), _ptr(sc), N, C, H, W,
+                                             _ptr(workspace), workspace.numel() * workspace.element_size(),
+                                             _stream()))
+        return q1, q2, sc
     check(lib().vsc_quantize_fp8_pair(_ptr(a), _ptr(b), _ptr(q1), _ptr(q2), _ptr(sc), N, C, H, W, _stream()))
     return q1, q2, sc
 
 
-def correlation_fp8(q1, q2, scale, max_displacement=4, legacy=0):
+def correlation_fp8(q1, q2, scale, max_displacement=4, legacy=0, workspace=None):
     """Correlation of e4m3 operands given as uint8 bit patterns."""
     torch = _torch()
     N, C, H, W = q1.shape
     P = 2 * int(max_displacement) + 1
     shape = (N, P * P, H, W) if legacy else (N, P, P, H, W)
     out = torch.empty(shape, dtype=torch.float32, device=q1.device)
+    if workspace is not None:
+        check(lib().vsc_correlation_forward_ws(_ptr(q1.contiguous()), _ptr(q2.contiguous()), _ptr(out), N, C, H, W,
+                                               int(max_displacement), int(bool(legacy)), VSC_FP8E4M3,
+                                               _ptr(scale.contiguous()), _ptr(workspace),
+                                               workspace.numel() * workspace.element_size(), _stream()))
+        return out
     check(lib().vsc_correlation_forward(_ptr(q1.contiguous()), _ptr(q2.contiguous()), _ptr(out), N, C, H, W,
                                         int(max_displacement), int(bool(legacy)), VSC_FP8E4M3,
                                         _ptr(scale.contiguous()), _stream()))
@@ -968,6 +1011,50 @@ def write_flo(path, flow):
     check(lib().vsc_flo_write(str(path).encode(), a.ctypes.data, a.shape[1], a.shape[0]))
 
 
+def flow_dir_path(flow_dir, frame, backward=False):
+    """The .flo file the flow-file mode reads for window `frame`
+    (stabilizefiles.cpp:134-149): frame_<frame+1>.flo (forward) or
+    frame_<frame>_bwd.flo (backward)."""
+    buf = ctypes.create_string_buffer(4096)
+    check(lib().vsc_flow_dir_path(str(flow_dir).encode(), int(frame), int(bool(backward)), buf, 4096))
+    return buf.value.decode()
+
+
+def flow_dir_read(flow_dir, frame):
+    """(forward, backward) flows of window `frame` from a flow directory."""
+    w, h = ctypes.c_int(), ctypes.c_int()
+    d = str(flow_dir).encode()
+    check(lib().vsc_flow_dir_read(d, int(frame), None, None, 0, ctypes.byref(w), ctypes.byref(h)))
+    f = np.empty((h.value, w.value, 2), np.float32)
+    b = np.empty_like(f)
+    check(lib().vsc_flow_dir_read(d, int(frame), f.ctypes.data, b.ctypes.data, f.size, ctypes.byref(w),
+                                  ctypes.byref(h)))
+    return f, b
+
+
+def stabilize_flow_dir(orig, proc, flow_dir, params=None, mode=VSC_FRAME_FUSED):
+    """The -f frame loop (FileStabilizer::stabilizeAll with flows from
+    `flow_dir`): orig, proc uint8 [N, H, W, 4] host arrays.  Returns the N
+    output RGBA8 frames (frame 0 and N-1 processed, 1..N-2 stabilized)."""
+    orig = np.ascontiguousarray(orig, dtype=np.uint8)
+    proc = np.ascontiguousarray(proc, dtype=np.uint8)
+    n, H, W, _ = orig.shape
+    po = (ctypes.c_void_p * n)(*[orig[i].ctypes.data for i in range(n)])
+    pp = (ctypes.c_void_p * n)(*[proc[i].ctypes.data for i in range(n)])
+    outs = [None] * n
+
+    def sink(user, index, rgba, w, h):
+        outs[index] = np.ctypeslib.as_array(ctypes.cast(rgba, ctypes.POINTER(ctypes.c_uint8)),
+                                            shape=(h, w, 4)).copy()
+        return 0
+
+    cb = FRAME_SINK(sink)
+    prm = (params or HyperParams()).c()
+    check(lib().vsc_stabilize_flow_dir(po, pp, n, W, H, str(flow_dir).encode(), ctypes.byref(prm), int(mode), cb,
+                                       None, _stream()))
+    return outs
+
+
 def profile_begin(capacity=4096):
     """Start timing the instrumented kernels with HIP events (see vsc.h)."""
     check(lib().vsc_profile_begin(int(capacity)))
@@ -1005,6 +1092,19 @@ SYNTH_640x360 = dict(width=640, height=360, nframes=40, nblobs=200, sigma_min=5.
 SYNTH_1080P = dict(width=1920, height=1080, nframes=300, nblobs=800, sigma_min=5.0, sigma_max=40.0, motion=8.0,
                    npatches=4, patch_size=64, gain_min=0.8, gain_max=1.2, offset_max=20.0, noise_sigma=4.0,
                    seed=2)
+
+
+#: the serving mix: 24 concurrent streams of four resolutions
+SERVE_MIX = [(640, 360)] * 8 + [(1280, 720)] * 8 + [(1920, 1080)] * 6 + [(3840, 2160)] * 2
+
+
+def serve_stream_video(i, nframes, size=None):
+    """Frames of serving stream i (its own seed, the 1080p generator's content
+    scaled to the stream's size): (orig, proc) uint8 [nframes, H, W, 4]."""
+    w, h = size or SERVE_MIX[i]
+    o, p, _, _ = synth_video(width=w, height=h, nframes=nframes, nblobs=max(100, int(800 * w * h / (1920 * 1080))),
+                             motion=8.0, npatches=4, patch_size=max(16, w // 15), seed=1000 + i)
+    return o, p
 
 
 def synth_video(**kw):

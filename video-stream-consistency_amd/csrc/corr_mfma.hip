@@ -284,8 +284,19 @@ static void to_nhwc(const void* in, void* out, int C, int CP, long long HW, int 
     }
 }
 
+size_t corr_mfma_workspace_bytes(int N, int C, int H, int W, int in_dtype)
+{
+    if (in_dtype != VSC_BF16 && in_dtype != VSC_F16 && in_dtype != VSC_FP8E4M3) return 0;
+    const size_t eb = in_dtype == VSC_FP8E4M3 ? 1 : 2;
+    const size_t CP = (size_t)(C + CM_KC - 1) / CM_KC * CM_KC;
+    const size_t one = ((size_t)N * H * W * CP * eb + 255) / 256 * 256;
+    return 2 * one;
+}
+
+// workspace: corr_mfma_workspace_bytes() bytes of device memory, or NULL to
+// allocate it stream-ordered for this call
 int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
-                       int in_dtype, const float* chan_scale, hipStream_t s)
+                       int in_dtype, const float* chan_scale, void* workspace, hipStream_t s)
 {
     if (in_dtype != VSC_BF16 && in_dtype != VSC_F16 && in_dtype != VSC_FP8E4M3)
         return set_error(VSC_E_DTYPE, "vsc_correlation_forward: unsupported input type");
@@ -295,12 +306,12 @@ int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C,
     const int eb = in_dtype == VSC_FP8E4M3 ? 1 : 2;
     const int CP = (C + CM_KC - 1) / CM_KC * CM_KC;
     const long long HW = (long long)H * W;
-    const size_t one = (size_t)N * HW * CP * eb;
-    void* ws = nullptr;
-    int rc = check_hip(hipMallocAsync(&ws, 2 * one, s), "vsc_correlation_forward workspace");
-    if (rc) return rc;
+    const size_t bytes = corr_mfma_workspace_bytes(N, C, H, W, in_dtype);
+    void* ws = workspace;
+    int rc;
+    if (!ws && (rc = check_hip(hipMallocAsync(&ws, bytes, s), "vsc_correlation_forward workspace"))) return rc;
     void* t1 = ws;
-    void* t2 = static_cast<char*>(ws) + one;
+    void* t2 = static_cast<char*>(ws) + bytes / 2;
     if (eb == 2) {
         to_nhwc<uint16_t>(f1, t1, C, CP, HW, N, s);
         to_nhwc<uint16_t>(f2, t2, C, CP, HW, N, s);
@@ -314,6 +325,7 @@ int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C,
         else if (in_dtype == VSC_F16) rc = dispatch_md<CM_F16>(t1, t2, out, N, C, CP, H, W, md, legacy, nullptr, s);
         else rc = dispatch_md<CM_FP8>(t1, t2, out, N, C, CP, H, W, md, legacy, chan_scale, s);
     }
+    if (workspace) return rc;
     const int rc2 = check_hip(hipFreeAsync(ws, s), "vsc_correlation_forward workspace free");
     return rc ? rc : rc2;
 }
@@ -421,17 +433,25 @@ __global__ void k_fp8_quant(const float* f, uint8_t* q, const float* r, const fl
 
 using namespace vsc;
 
-extern "C" int vsc_quantize_fp8_pair(const float* f1, const float* f2, uint8_t* q1, uint8_t* q2, float* scale, int N,
-                                     int C, int H, int W, vsc_hip_stream stream)
+extern "C" size_t vsc_quantize_fp8_pair_workspace_size(int C)
+{
+    return C < 1 ? 0 : (size_t)(3 * C + 4) * 4;
+}
+
+// workspace: vsc_quantize_fp8_pair_workspace_size(C) bytes, or NULL to
+// allocate it stream-ordered for this call
+static int quantize_fp8_pair(const float* f1, const float* f2, uint8_t* q1, uint8_t* q2, float* scale, int N, int C,
+                             int H, int W, void* workspace, hipStream_t s)
 {
     if (!f1 || !f2 || !q1 || !q2 || !scale) return set_error(VSC_E_INVALID, "vsc_quantize_fp8_pair: null pointer");
     if (N < 1 || C < 1 || H < 1 || W < 1) return set_error(VSC_E_INVALID, "vsc_quantize_fp8_pair: empty tensor");
-    hipStream_t s = as_stream(stream);
     const long long HW = (long long)H * W, n = (long long)N * C * HW;
     // workspace: 2*C maxima + C equalisation factors + 3 scales
-    void* ws = nullptr;
-    int rc = check_hip(hipMallocAsync(&ws, (size_t)(3 * C + 4) * 4, s), "vsc_quantize_fp8_pair workspace");
-    if (rc) return rc;
+    void* ws = workspace;
+    int rc;
+    if (!ws && (rc = check_hip(hipMallocAsync(&ws, vsc_quantize_fp8_pair_workspace_size(C), s),
+                               "vsc_quantize_fp8_pair workspace")))
+        return rc;
     uint32_t* mx = static_cast<uint32_t*>(ws);
     float* r = reinterpret_cast<float*>(mx + 2 * C);
     float* sc = r + C;
@@ -445,8 +465,25 @@ extern "C" int vsc_quantize_fp8_pair(const float* f1, const float* f2, uint8_t* 
     hipLaunchKernelGGL(k_fp8_quant, dim3(g), dim3(256), 0, s, f2, q2, r, sc, 1, C, HW, n);
     rc = check_launch("vsc_quantize_fp8_pair");
     if (!rc) rc = check_hip(hipMemcpyAsync(scale, sc, sizeof(float), hipMemcpyDeviceToDevice, s), "vsc_quantize_fp8_pair");
+    if (workspace) return rc;
     const int rc2 = check_hip(hipFreeAsync(ws, s), "vsc_quantize_fp8_pair workspace free");
     return rc ? rc : rc2;
+}
+
+extern "C" int vsc_quantize_fp8_pair(const float* f1, const float* f2, uint8_t* q1, uint8_t* q2, float* scale, int N,
+                                     int C, int H, int W, vsc_hip_stream stream)
+{
+    return quantize_fp8_pair(f1, f2, q1, q2, scale, N, C, H, W, nullptr, as_stream(stream));
+}
+
+extern "C" int vsc_quantize_fp8_pair_ws(const float* f1, const float* f2, uint8_t* q1, uint8_t* q2, float* scale,
+                                        int N, int C, int H, int W, void* workspace, size_t workspace_bytes,
+                                        vsc_hip_stream stream)
+{
+    if (!workspace || workspace_bytes < vsc_quantize_fp8_pair_workspace_size(C))
+        return set_error(VSC_E_INVALID, "vsc_quantize_fp8_pair_ws: workspace missing or smaller than "
+                                        "vsc_quantize_fp8_pair_workspace_size()");
+    return quantize_fp8_pair(f1, f2, q1, q2, scale, N, C, H, W, workspace, as_stream(stream));
 }
 
 extern "C" int vsc_convert_f32(const float* in, void* out, size_t count, int out_dtype, vsc_hip_stream stream)

@@ -3,6 +3,8 @@
 // shape contract of src/ort_custom_ops.
 #include "vsc_internal.h"
 
+#include <string>
+
 namespace vsc {
 
 // ---------------------------------------------------------------------------
@@ -239,9 +241,12 @@ __global__ __launch_bounds__(256) void k_warp(const T* __restrict__ in, const T*
 
 using namespace vsc;
 
-extern "C" int vsc_correlation_forward(const void* f1, const void* f2, void* out, int N, int C, int H, int W,
-                                       int max_displacement, int legacy, int in_dtype, const float* chan_scale,
-                                       vsc_hip_stream stream)
+namespace {
+
+// workspace: corr_mfma_workspace_bytes() bytes for the reduced-precision
+// types (NULL: allocated stream-ordered for the call); unused otherwise
+int correlation_forward(const void* f1, const void* f2, void* out, int N, int C, int H, int W, int max_displacement,
+                        int legacy, int in_dtype, const float* chan_scale, void* workspace, vsc_hip_stream stream)
 {
     if (!f1 || !f2 || !out) return set_error(VSC_E_INVALID, "vsc_correlation_forward: null pointer");
     if (N < 1 || C < 1 || H < 1 || W < 1) return set_error(VSC_E_INVALID, "vsc_correlation_forward: empty tensor");
@@ -281,9 +286,38 @@ extern "C" int vsc_correlation_forward(const void* f1, const void* f2, void* out
         return check_launch("correlation f64");
     }
     if (in_dtype == VSC_BF16 || in_dtype == VSC_F16 || in_dtype == VSC_FP8E4M3) {
-        return corr_mfma_dispatch(f1, f2, static_cast<float*>(out), N, C, H, W, md, legacy, in_dtype, chan_scale, s);
+        return corr_mfma_dispatch(f1, f2, static_cast<float*>(out), N, C, H, W, md, legacy, in_dtype, chan_scale,
+                                  workspace, s);
     }
     return set_error(VSC_E_DTYPE, "vsc_correlation_forward: unsupported input type (float, double, bf16, f16, fp8e4m3)");
+}
+
+}  // namespace
+
+extern "C" int vsc_correlation_forward(const void* f1, const void* f2, void* out, int N, int C, int H, int W,
+                                       int max_displacement, int legacy, int in_dtype, const float* chan_scale,
+                                       vsc_hip_stream stream)
+{
+    return correlation_forward(f1, f2, out, N, C, H, W, max_displacement, legacy, in_dtype, chan_scale, nullptr,
+                               stream);
+}
+
+extern "C" size_t vsc_correlation_workspace_size(int N, int C, int H, int W, int in_dtype)
+{
+    if (N < 1 || C < 1 || H < 1 || W < 1) return 0;
+    return corr_mfma_workspace_bytes(N, C, H, W, in_dtype);
+}
+
+extern "C" int vsc_correlation_forward_ws(const void* f1, const void* f2, void* out, int N, int C, int H, int W,
+                                          int max_displacement, int legacy, int in_dtype, const float* chan_scale,
+                                          void* workspace, size_t workspace_bytes, vsc_hip_stream stream)
+{
+    const size_t need = vsc_correlation_workspace_size(N, C, H, W, in_dtype);
+    if (need && (!workspace || workspace_bytes < need))
+        return set_error(VSC_E_INVALID, "vsc_correlation_forward_ws: workspace missing or smaller than "
+                                        "vsc_correlation_workspace_size()");
+    return correlation_forward(f1, f2, out, N, C, H, W, max_displacement, legacy, in_dtype, chan_scale,
+                               need ? workspace : nullptr, stream);
 }
 
 extern "C" int vsc_warp_forward(const void* in, const void* flow, void* out, int N, int C, int H, int W, int dtype,

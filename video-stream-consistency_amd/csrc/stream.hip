@@ -115,7 +115,7 @@ extern "C" int vsc_stream_create(int width, int height, int flow_width, int flow
         s->flow_up[1] = take((size_t)3 * width * height);
     }
     if ((size_t)(p - static_cast<char*>(s->block)) != s->bytes) {
-        hipFree(s->block);
+        (void)hipFree(s->block);
         delete s;
         return set_error(VSC_E_INVALID, "vsc_stream_create: internal error: carve does not match the footprint");
     }

@@ -256,6 +256,7 @@ struct FrontArgs {
 };
 int launch_front_pass(const FrontArgs& a, hipStream_t s);
 
+
 // ---------------------------------------------------------------------------
 // 16-bit storage formats of the flow network (FP32 arithmetic in registers)
 // ---------------------------------------------------------------------------
@@ -331,9 +332,9 @@ int pipeline_advance(vsc_pipeline* p, long long frame);
 // before any state change (NULL params = defaults)
 int stream_params_check(const vsc_stream* s, const vsc_params* params);
 int stream_frame_check(const vsc_stream* s, const vsc_params* params);
-
 // reduced-precision (BF16 / F16 / FP8 e4m3) correlation on MFMA, corr_mfma.hip
+size_t corr_mfma_workspace_bytes(int N, int C, int H, int W, int in_dtype);
 int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
-                       int in_dtype, const float* chan_scale, hipStream_t s);
+                       int in_dtype, const float* chan_scale, void* workspace, hipStream_t s);
 
 }  // namespace vsc
