@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--quality-frames", type=int, default=30, help="frames of the FP64-oracle quality sequence")
     ap.add_argument("--serve-fps", type=float, default=30.0, help="arrival rate per stream (serve workload)")
     ap.add_argument("--serve-eager", action="store_true", help="serve workload: launch eagerly instead of replaying graphs")
+    ap.add_argument("--serve-unbatched", action="store_true",
+                    help="serve workload: stabilize each stream's frames on its own HIP stream instead of one "
+                         "batched launch sequence per tick")
     ap.add_argument("--plain-steps", type=int, default=0, help="timed frames of the plain form (default steps/3)")
     ap.add_argument("--solver", choices=["fma", "exact"], default="fma",
                     help="stabilize workload: solver in FMA form or in the reference op order")
@@ -562,7 +565,7 @@ def serve_run(args, vsc, dev, n):
         src.append((torch.from_numpy(o).to(dev), torch.from_numpy(p).to(dev)))
     gen_s = time.time() - t0
     sv = vsc.Server(hbm_budget=128 << 30, max_streams=len(SERVE_STREAMS), hip_streams=4, form="chosen",
-                    use_graphs=not args.serve_eager)
+                    use_graphs=not args.serve_eager, batch=not args.serve_unbatched)
     ids = [sv.open(w, h) for (w, h) in SERVE_STREAMS]
     sid_to_i = {sid: i for i, sid in enumerate(ids)}
     rng = np.random.default_rng(7)
@@ -619,7 +622,8 @@ def serve_run(args, vsc, dev, n):
         "span_s": round(span, 3), "gen_seconds": round(gen_s, 2),
         "workload": f"{len(ids)} streams (8x640x360, 8x1280x720, 6x1920x1080, 2x3840x2160), {n} frames each at "
                     f"{args.serve_fps} fps arrivals (zero-mean exponential jitter), chosen form on the multi-stream "
-                    f"server ({'eager launches' if args.serve_eager else 'hipGraph replay'})",
+                    f"server ({'eager launches' if args.serve_eager else 'hipGraph replay'}, "
+                    f"{'per-stream stabilization' if args.serve_unbatched else 'one batched stabilization per tick'})",
     }
 
 

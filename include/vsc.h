@@ -460,11 +460,13 @@ int vsc_pipeline_sync(vsc_pipeline* p, vsc_hip_stream stream);
  * by device-memory footprint against hbm_budget; each owns a vsc_pipeline
  * (form: the pipeline configuration, by default the chosen form: BF16 flow
  * network at FLOWDOWNSCALE 2 with the learned upsampler), a 3-slot ring of
- * input frames and a 3-slot output ring.  vsc_server_submit copies frame t of
+ * input frames and a 6-slot output ring.  vsc_server_submit copies frame t of
  * a stream into its ring (host or device memory) on the stream's HIP stream;
  * from frame 2 on, each submission makes the window centred at t-1 ready.
- * vsc_server_tick launches every ready window of every stream, streams spread
- * over hip_streams HIP streams (default priority) so their kernels overlap;
+ * vsc_server_tick launches every ready window of every stream: the streams'
+ * flow work spread over hip_streams HIP streams (default priority) so their
+ * kernels overlap, and (batch = 1) the stabilization of all of the tick's
+ * windows as one graph of persistent launches on the server's tick stream;
  * streams opened or frames submitted while earlier work runs join the next
  * tick.  vsc_server_poll reports finished frames in order per stream; a
  * reported frame's output stays in the stream's 6-slot output ring until the
@@ -482,6 +484,12 @@ typedef struct {
     vsc_pipeline_cfg form;
     int use_graphs;           /* 1: replay each stream's per-frame work from hipGraphs (6 per stream, one
                                  per frame phase, captured once, re-captured after a parameter change) */
+    int batch;                /* 1 (default): a tick stabilizes the windows of all its streams together:
+                                 each stream's flow work on its pool stream, then one graph on the
+                                 server's tick stream (work-list copy + persistent launches that walk
+                                 the frames of every stream); windows the batch cannot take (more than
+                                 2 pyramid levels, odd sizes at 2 levels, num_iter > 304) run per
+                                 stream.  0: every frame runs per stream on its pool stream. */
 } vsc_server_cfg;
 
 typedef struct {

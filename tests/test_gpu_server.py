@@ -16,11 +16,12 @@ def _video(vsc, W, H, n, seed):
     return orig, proc
 
 
-def _run(vsc, cuda, streams, order, form="chosen", hip_streams=4):
+def _run(vsc, cuda, streams, order, form="chosen", hip_streams=4, batch=True, params=None):
     """streams: {name: (W, H, orig, proc)}; order: list of names, one submission
-    each (in that order), ticks after every submission.  Returns {name: {frame: out}}."""
+    each (in that order), ticks after every submission; params: {name: {frame
+    submitted: HyperParams}} changes.  Returns {name: {frame: out}}."""
     import torch
-    sv = vsc.Server(form=form, hip_streams=hip_streams, max_streams=max(64, len(streams)))
+    sv = vsc.Server(form=form, hip_streams=hip_streams, max_streams=max(64, len(streams)), batch=batch)
     ids = {k: sv.open(v[0], v[1]) for k, v in streams.items()}
     frames = {k: torch.from_numpy(np.concatenate([v[2][None], v[3][None]])).to(cuda) for k, v in streams.items()}
     nxt = {k: 0 for k in streams}
@@ -33,6 +34,8 @@ def _run(vsc, cuda, streams, order, form="chosen", hip_streams=4):
 
     for name in order:
         t = nxt[name]
+        if params and t in params.get(name, {}):
+            sv.set_params(ids[name], params[name][t])
         while True:
             try:
                 sv.submit(ids[name], frames[name][0][t], frames[name][1][t], tag=t)
@@ -119,8 +122,9 @@ def test_server_params_apply_from_next_tick(vsc, cuda):
         assert torch.equal(ref, got[t]), f"frame {t}"
 
 
+@pytest.mark.parametrize("batch", [True, False])
 @pytest.mark.parametrize("form", ["chosen", "plain"])
-def test_server_graph_replay_matches_eager(vsc, cuda, form):
+def test_server_graph_replay_matches_eager(vsc, cuda, form, batch):
     """Each stream's per-frame work replayed from captured hipGraphs (6 per
     stream, one per frame phase) gives the eager launches' bits on every
     frame: several replays of each graph, eager work and synchronisations in
@@ -135,7 +139,7 @@ def test_server_graph_replay_matches_eager(vsc, cuda, form):
     hp.gamma = 1.25
     res = {}
     for graphs in (False, True):
-        sv = vsc.Server(form=form, use_graphs=graphs)
+        sv = vsc.Server(form=form, use_graphs=graphs, batch=batch)
         sid = sv.open(W, H)
         got = {}
         for t in range(n):
@@ -254,3 +258,28 @@ def test_serving_mix_each_stream_equals_its_pipeline(vsc, cuda):
             ref = pipe.frame((o[t - 1], o[t], o[t + 1], p[t - 1], p[t], p[t + 1]), t).cpu()
             assert torch.equal(ref, mix[k][t]), f"{k} ({w}x{h}) frame {t}"
         pipe.close()
+
+
+def test_server_batched_equals_unbatched_and_fallback(vsc, cuda):
+    """Batched ticks (one set of persistent launches stabilizing every stream's
+    window) give the bits of per-stream ticks, in a mix of sizes (odd ones
+    included) with a stream switching to 3 pyramid levels mid-stream (those
+    windows run per stream) and one to 1 level and fewer sweeps (batched)."""
+    import torch
+    sizes = {"a": (192, 128), "b": (130, 96), "c": (256, 144), "d": (101, 75)}
+    vids = {k: (W, H) + _video(vsc, W, H, 9, seed=50 + i) for i, (k, (W, H)) in enumerate(sizes.items())}
+    three = vsc.HyperParams()
+    three.pyramidLevels = 3
+    one = vsc.HyperParams()
+    one.pyramidLevels = 1
+    one.numIter = 37
+    params = {"a": {4: three, 7: vsc.HyperParams()}, "c": {5: one}}
+    rng = np.random.default_rng(5)
+    order = [k for k in vids for _ in range(9)]
+    rng.shuffle(order)
+    for form in ("chosen", "plain"):
+        res = {b: _run(vsc, cuda, vids, order, form=form, batch=b, params=params) for b in (True, False)}
+        for k in vids:
+            assert sorted(res[True][k]) == list(range(1, 8)), k
+            for f in res[False][k]:
+                assert torch.equal(res[True][k][f], res[False][k][f]), (form, k, f)

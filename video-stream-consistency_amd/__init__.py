@@ -99,7 +99,7 @@ class Window(ctypes.Structure):
 
 class ServerCfg(ctypes.Structure):
     _fields_ = [("max_streams", ctypes.c_int), ("hbm_budget", ctypes.c_size_t), ("hip_streams", ctypes.c_int),
-                ("form", PipelineCfg), ("use_graphs", ctypes.c_int)]
+                ("form", PipelineCfg), ("use_graphs", ctypes.c_int), ("batch", ctypes.c_int)]
 
 
 class ServerDone(ctypes.Structure):
@@ -908,14 +908,17 @@ class Pipeline:
 
 class Server:
     """Multi-stream frame server (vsc_server_*): admission by device-memory
-    budget, continuous batching of ready windows over a pool of HIP streams."""
+    budget, continuous batching of ready windows: each stream's flow work on a
+    pool of HIP streams, then (batch=True) the stabilization of every stream's
+    window of a tick in one graph of persistent work-list launches."""
 
     def __init__(self, hbm_budget=64 << 30, max_streams=64, hip_streams=4, form="chosen", upsampler_params=None,
-                 use_graphs=True, solver=None):
+                 use_graphs=True, solver=None, batch=True):
         c = ServerCfg()
         lib().vsc_server_default_cfg(ctypes.byref(c))
         c.max_streams, c.hbm_budget, c.hip_streams = max_streams, hbm_budget, hip_streams
         c.use_graphs = int(bool(use_graphs))
+        c.batch = int(bool(batch))
         fk = Pipeline.FORMS[form]
         c.form.flow_dtype = {"f32": VSC_F32, "bf16": VSC_BF16}[fk["flow_dtype"]]
         c.form.downscale = fk["downscale"]

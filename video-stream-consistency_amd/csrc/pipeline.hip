@@ -257,6 +257,103 @@ extern "C" int vsc_pipeline_flows(vsc_pipeline* p, const vsc_window* w, long lon
     return flows_on(p, w, frame, fwd_out ? fwd_out : p->lo[k][0], bwd_out ? bwd_out : p->lo[k][1], s);
 }
 
+namespace {
+
+// Host state of a pipeline that a frame's enqueueing changes; restored when
+// enqueueing (or capturing) fails part-way, so a later call recomputes what
+// the failed one did not finish.
+struct HostState {
+    long long slot_frame[3];
+    long long prev_fwd;
+    int parity;
+    explicit HostState(const vsc_pipeline* p)
+        : slot_frame{p->slot_frame[0], p->slot_frame[1], p->slot_frame[2]},
+          prev_fwd(p->prev_fwd_frame), parity(p->last_parity) {}
+    void restore(vsc_pipeline* p) const
+    {
+        for (int i = 0; i < 3; ++i) p->slot_frame[i] = slot_frame[i];
+        p->prev_fwd_frame = prev_fwd;
+        p->last_parity = parity;
+    }
+};
+
+// The flows of window `frame` at frame size: features, the batched
+// forward/backward decode and the consistency gate on fs, the upsampling on
+// ss (ov: the flows' completion is ordered before ss by ev_f).  *ff, *fb
+// receive the frame-size (or, without upsampling here, flow-size) flows.
+int frame_flows(vsc_pipeline* p, const vsc_window* w, long long frame, hipStream_t fs, hipStream_t ss, bool ov,
+                const float** ff, const float** fb)
+{
+    const int f = p->cfg.downscale;
+    const int k = parity(frame);
+    int rc;
+    if ((rc = flows_on(p, w, frame, p->lo[k][0], p->lo[k][1], fs))) return rc;
+    const vsc_hip_stream fsv = reinterpret_cast<vsc_hip_stream>(fs);
+    // previous forward flow for the consistency check of this frame's backward flow
+    const bool have_prev = p->prev_fwd_frame == frame - 1;
+    float* gate = p->gate + 16 * k;
+    uint32_t* count = p->count + 16 * k;
+    if (learned(p->cfg)) {
+        if (p->cfg.fb_max_inconsistent < 0.0f || (!have_prev && !p->cfg.ref_bwd_index)) {
+            hipLaunchKernelGGL(k_set_gate, dim3(1), dim3(64), 0, fs, gate, 1.0f);
+            if ((rc = check_launch("pipeline gate"))) return rc;
+        } else if (p->cfg.ref_bwd_index) {
+            // fwd: cur -> next, bwd: next -> cur are each other's inverse
+            if ((rc = vsc_flow_fb_gate(p->lo[k][0], p->lo[k][1], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent, count,
+                                       gate, fsv)))
+                return rc;
+        } else {
+            // bwd: cur -> prev against the previous frame's fwd: prev -> cur
+            if ((rc = vsc_flow_fb_gate(p->lo[k][1], p->lo[1 - k][0], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent,
+                                       count, gate, fsv)))
+                return rc;
+        }
+    }
+    p->prev_fwd_frame = frame;
+    p->last_parity = k;
+    if (ov) {
+        if (check_hip(hipEventRecord(p->ev_f, fs), "vsc_pipeline_frame") ||
+            check_hip(hipStreamWaitEvent(ss, p->ev_f, 0), "vsc_pipeline_frame"))
+            return VSC_E_HIP;
+        p->f_rec = true;
+    }
+    // upsampling to the frame, on the stabilization stream (it reads only this
+    // frame's buffer set)
+    *ff = p->lo[k][0];
+    *fb = p->lo[k][1];
+    if (learned(p->cfg)) {
+        // the learned upsampler is trained on full-resolution vectors: always scaled
+        const float sc = (float)f;
+        const float* const lo[2] = {p->lo[k][0], p->lo[k][1]};
+        float* const hi[2] = {p->hi[k][0], p->hi[k][1]};
+        if ((rc = upsampler_run_flows(p->ups, 2, lo, p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, hi, 2, gate, ss)))
+            return rc;
+        *ff = p->hi[k][0];
+        *fb = p->hi[k][1];
+    } else if (upsample_here(p->cfg)) {
+        dim3 grid((p->W + 255) / 256, p->H);
+        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, ss, p->lo[k][0], p->Wd, p->Hd, p->hi[k][0], p->W, p->H,
+                           (float)f);
+        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, ss, p->lo[k][1], p->Wd, p->Hd, p->hi[k][1], p->W, p->H,
+                           (float)f);
+        if ((rc = check_launch("pipeline flow upscale"))) return rc;
+        *ff = p->hi[k][0];
+        *fb = p->hi[k][1];
+    }
+    return VSC_OK;
+}
+
+// the flows frame_flows leaves for window `frame` (no launch)
+void frame_flow_ptrs(const vsc_pipeline* p, long long frame, const float** ff, const float** fb)
+{
+    const int k = parity(frame);
+    const bool hi = learned(p->cfg) || upsample_here(p->cfg);
+    *ff = hi ? p->hi[k][0] : p->lo[k][0];
+    *fb = hi ? p->hi[k][1] : p->lo[k][1];
+}
+
+}  // namespace
+
 extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long long frame, const vsc_params* params,
                                   uint8_t* out_rgba, float* out_float, vsc_hip_stream stream)
 {
@@ -269,17 +366,10 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
     hipStream_t cs = as_stream(stream);
     const bool ov = p->cfg.overlap && !capturing(cs);
     hipStream_t fs = ov ? p->sf : cs, ss = ov ? p->ss : cs;
-    const int f = p->cfg.downscale;
     const int k = parity(frame);
-    // host state as on entry: restored when enqueueing (or capturing) fails
-    // part-way, so a later call recomputes what this one did not finish
-    long long slot_frame0[3] = {p->slot_frame[0], p->slot_frame[1], p->slot_frame[2]};
-    const long long prev_fwd0 = p->prev_fwd_frame;
-    const int parity0 = p->last_parity;
+    const HostState saved(p);
     auto fail = [&](int code) {
-        for (int i = 0; i < 3; ++i) p->slot_frame[i] = slot_frame0[i];
-        p->prev_fwd_frame = prev_fwd0;
-        p->last_parity = parity0;
+        saved.restore(p);
         return code;
     };
     if (ov) {
@@ -290,59 +380,8 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
             (p->s_rec[k] && check_hip(hipStreamWaitEvent(fs, p->ev_s[k], 0), "vsc_pipeline_frame")))
             return fail(VSC_E_HIP);
     }
-    if ((rc = flows_on(p, w, frame, p->lo[k][0], p->lo[k][1], fs))) return fail(rc);
-    const vsc_hip_stream fsv = reinterpret_cast<vsc_hip_stream>(fs);
-    // previous forward flow for the consistency check of this frame's backward flow
-    const bool have_prev = p->prev_fwd_frame == frame - 1;
-    float* gate = p->gate + 16 * k;
-    uint32_t* count = p->count + 16 * k;
-    if (learned(p->cfg)) {
-        if (p->cfg.fb_max_inconsistent < 0.0f || (!have_prev && !p->cfg.ref_bwd_index)) {
-            hipLaunchKernelGGL(k_set_gate, dim3(1), dim3(64), 0, fs, gate, 1.0f);
-            if ((rc = check_launch("pipeline gate"))) return fail(rc);
-        } else if (p->cfg.ref_bwd_index) {
-            // fwd: cur -> next, bwd: next -> cur are each other's inverse
-            if ((rc = vsc_flow_fb_gate(p->lo[k][0], p->lo[k][1], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent, count,
-                                       gate, fsv)))
-                return fail(rc);
-        } else {
-            // bwd: cur -> prev against the previous frame's fwd: prev -> cur
-            if ((rc = vsc_flow_fb_gate(p->lo[k][1], p->lo[1 - k][0], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent,
-                                       count, gate, fsv)))
-                return fail(rc);
-        }
-    }
-    p->prev_fwd_frame = frame;
-    p->last_parity = k;
-    if (ov) {
-        if (check_hip(hipEventRecord(p->ev_f, fs), "vsc_pipeline_frame") ||
-            check_hip(hipStreamWaitEvent(ss, p->ev_f, 0), "vsc_pipeline_frame"))
-            return fail(VSC_E_HIP);
-        p->f_rec = true;
-    }
-    // upsampling to the frame, on the stabilization stream (it reads only this
-    // frame's buffer set)
-    const float* ff = p->lo[k][0];
-    const float* fb = p->lo[k][1];
-    if (learned(p->cfg)) {
-        // the learned upsampler is trained on full-resolution vectors: always scaled
-        const float sc = (float)f;
-        const float* const lo[2] = {p->lo[k][0], p->lo[k][1]};
-        float* const hi[2] = {p->hi[k][0], p->hi[k][1]};
-        if ((rc = upsampler_run_flows(p->ups, 2, lo, p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, hi, 2, gate, ss)))
-            return fail(rc);
-        ff = p->hi[k][0];
-        fb = p->hi[k][1];
-    } else if (upsample_here(p->cfg)) {
-        dim3 grid((p->W + 255) / 256, p->H);
-        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, ss, p->lo[k][0], p->Wd, p->Hd, p->hi[k][0], p->W, p->H,
-                           (float)f);
-        hipLaunchKernelGGL(k_flow_upscale, grid, dim3(256), 0, ss, p->lo[k][1], p->Wd, p->Hd, p->hi[k][1], p->W, p->H,
-                           (float)f);
-        if ((rc = check_launch("pipeline flow upscale"))) return fail(rc);
-        ff = p->hi[k][0];
-        fb = p->hi[k][1];
-    }
+    const float *ff = nullptr, *fb = nullptr;
+    if ((rc = frame_flows(p, w, frame, fs, ss, ov, &ff, &fb))) return fail(rc);
     vsc_frame_in in{w->orig_prev, w->orig_cur, w->orig_next, w->proc_prev, w->proc_cur, w->proc_next, ff, fb, 2};
     rc = vsc_stab_frame(p->stab, &in, params, out_rgba, out_float, p->cfg.solver ? VSC_FRAME_FMA : VSC_FRAME_FUSED,
                         reinterpret_cast<vsc_hip_stream>(ss));
@@ -355,6 +394,47 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
         return fail(VSC_E_HIP);
     return VSC_OK;
 }
+
+namespace vsc {
+
+int pipeline_batch_check(vsc_pipeline* p, const vsc_params* params)
+{
+    int rc = stream_frame_check(p->stab, params);
+    if (rc) return rc;
+    if (p->cfg.overlap) return set_error(VSC_E_UNSUPPORTED, "batched frame: the pipeline runs overlapped");
+    // the stream must receive frame-size flows (the pipeline upsamples them)
+    if (p->cfg.downscale > 1 && !upsample_here(p->cfg))
+        return set_error(VSC_E_UNSUPPORTED, "batched frame: the stream upsamples its own flows");
+    vsc_params q;
+    if (params) q = *params;
+    else vsc_default_params(&q);
+    if (q.pyramid_levels > 2 || (q.pyramid_levels == 2 && !batch_exact_2x(p->W, p->H)) ||
+        q.num_iter > BATCH_MAX_LAUNCH * BATCH_HALO)
+        return set_error(VSC_E_UNSUPPORTED, "batched frame: needs 1 level, or 2 with even frame sizes");
+    return VSC_OK;
+}
+
+int pipeline_flows_part(vsc_pipeline* p, const vsc_window* w, long long frame, hipStream_t s)
+{
+    if (!p || !w || !w->orig_prev || !w->orig_cur || !w->orig_next)
+        return set_error(VSC_E_INVALID, "vsc_pipeline_frame: NULL argument");
+    const HostState saved(p);
+    const float *ff = nullptr, *fb = nullptr;
+    const int rc = frame_flows(p, w, frame, s, s, false, &ff, &fb);
+    if (rc) saved.restore(p);
+    return rc;
+}
+
+int pipeline_batch_item(vsc_pipeline* p, const vsc_window* w, long long frame, const vsc_params* params,
+                        uint8_t* out_rgba, BatchFrame* bf)
+{
+    const float *ff = nullptr, *fb = nullptr;
+    frame_flow_ptrs(p, frame, &ff, &fb);
+    vsc_frame_in in{w->orig_prev, w->orig_cur, w->orig_next, w->proc_prev, w->proc_cur, w->proc_next, ff, fb, 2};
+    return stream_batch_frame(p->stab, &in, params, out_rgba, p->cfg.solver ? 1 : 0, nullptr, bf);
+}
+
+}  // namespace vsc
 
 extern "C" int vsc_pipeline_sync(vsc_pipeline* p, vsc_hip_stream stream)
 {

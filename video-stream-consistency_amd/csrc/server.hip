@@ -1,26 +1,38 @@
 // Multi-stream frame server: many concurrent video streams of different
 // resolutions on one GPU.  Each admitted stream owns a vsc_pipeline, a 3-slot
-// device ring of original and processed frames, a 3-slot output ring and its
-// hyper-parameters.  Frames are submitted as they arrive; every tick launches
-// the stabilization of each stream whose next window is complete, streams
-// spread over a pool of HIP streams so that their kernels overlap on the
-// device (continuous batching: a stream that arrives or is admitted between
-// ticks joins at the next tick, while earlier work is still running).
-// Admission is by device-memory footprint against a budget; a refused stream
-// can be opened again once others have closed.
+// device ring of original and processed frames, a 6-slot output ring and its
+// hyper-parameters.  Frames are submitted as they arrive; every tick takes the
+// next ready window of every stream (continuous batching: a stream that
+// arrives or is admitted between ticks joins at the next tick, while earlier
+// work is still running).  Admission is by device-memory footprint against a
+// budget; a refused stream can be opened again once others have closed.
+//
+// A tick, batched (the default): each stream's flow work (network features,
+// forward/backward decode, consistency gate, upsampling) is enqueued on the
+// stream's HIP stream from a pool, replayed from a captured graph in steady
+// state; then the stabilization of every window of the tick (front pass,
+// pyramid solve, RGBA8 out) runs as one captured graph on the server's tick
+// stream: a copy of the tick's work list into device memory followed by
+// persistent launches that walk it, one launch per solver step for all
+// streams together.  Unbatched (or for a window the batch cannot take: more
+// than 2 pyramid levels, odd sizes at 2 levels), a stream's whole frame runs
+// on its pool stream, replayed from per-stream graphs.
 //
 // Frame protocol per stream (k = 1, B = 1; videostabilizer.cpp:136-164): frame
 // t's original and processed images are copied into ring slot t % 3 on the
 // stream's HIP stream.  Submitting frame 2 sets `last` to processed frame 2
 // (preloadProcessedFrames); from then on, submitting frame t makes window
-// (t-2, t-1, t) ready, and the next tick stabilizes frame t-1 into output
-// slot (t-1) % 4 once that slot's previous frame has been reported by poll.  Frames 0 and the final frame are the processed frames
-// unchanged, as in the reference (the caller owns them).
+// (t-2, t-1, t) ready, and a tick stabilizes frame t-1 into output slot
+// (t-1) % 6 once that slot's previous frame has been reported by poll.  Frames
+// 0 and the final frame are the processed frames unchanged, as in the
+// reference (the caller owns them).
 #include "vsc_internal.h"
 
 #include <deque>
+#include <map>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace vsc {
@@ -32,6 +44,11 @@ struct Done {
     int slot;   // output slot; its completion event is Stream::ev_done[slot]
 };
 
+// 6 output slots: every per-frame pointer then repeats with period 6 in the
+// frame index (input ring t % 3, ping-pongs t % 2), so 6 graphs replay forever
+constexpr int R_OUT = 6;
+constexpr int N_PHASES = 6;
+
 struct Stream {
     bool open = false;
     int W = 0, H = 0;
@@ -39,11 +56,14 @@ struct Stream {
     vsc_pipeline* pipe = nullptr;
     uint8_t* orig[3] = {nullptr, nullptr, nullptr};
     uint8_t* proc[3] = {nullptr, nullptr, nullptr};
-    uint8_t* out[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    hipGraphExec_t ge[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint8_t* out[R_OUT] = {};
+    hipGraphExec_t ge[N_PHASES] = {};   // whole frames (unbatched)
+    hipGraphExec_t gf[N_PHASES] = {};   // flow work only (batched)
     // completion event per output slot (a slot holds at most one frame in
-    // flight) and one for reads: created at open, reused for every frame
-    hipEvent_t ev_done[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // flight), one for the flow work of the latest batched window, one for
+    // reads: created at open, reused for every frame
+    hipEvent_t ev_done[R_OUT] = {};
+    hipEvent_t ev_flow = nullptr;
     hipEvent_t ev_read = nullptr;
     hipStream_t hs = nullptr;       // borrowed from the pool
     long long next_frame = 0;       // index of the next submitted frame
@@ -65,38 +85,51 @@ struct vsc_server {
     std::vector<hipStream_t> pool;
     size_t used = 0;
     int rr = 0;
+    // batched ticks
+    hipStream_t ts = nullptr;                 // the tick stream
+    unsigned char* wl_dev = nullptr;          // device work list
+    unsigned char* wl_host[2] = {};           // pinned host work lists, used alternately
+    hipEvent_t ev_wl[2] = {};                 // the copy that last read wl_host[i]
+    bool wl_used[2] = {false, false};
+    int wl_next = 0;
+    int grid = 256;                           // blocks of a persistent launch (one per CU)
+    std::vector<vsc::BatchFrame> items;
+    std::vector<int> item_stream;
+    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> batch_graphs;  // (host list, frames, launches)
 };
 
 namespace vsc {
 namespace {
 
-// 6 output slots: every per-frame pointer then repeats with period 6 in the
-// frame index (input ring t % 3, ping-pongs t % 2), so 6 graphs replay forever
-constexpr int R_OUT = 6;
-constexpr int N_PHASES = 6;
 size_t ring_bytes(int W, int H) { return (size_t)W * H * 4 * (6 + R_OUT); }
 
-void drop_graphs(Stream& st)
+void drop_graphs(Stream& st, bool flows_too)
 {
     bool any = false;
-    for (int i = 0; i < N_PHASES; ++i) any = any || st.ge[i];
+    for (int i = 0; i < N_PHASES; ++i) any = any || st.ge[i] || (flows_too && st.gf[i]);
     if (any && st.hs) (void)hipStreamSynchronize(st.hs);
     for (int i = 0; i < N_PHASES; ++i) {
         if (st.ge[i]) (void)hipGraphExecDestroy(st.ge[i]);
         st.ge[i] = nullptr;
+        if (flows_too) {
+            if (st.gf[i]) (void)hipGraphExecDestroy(st.gf[i]);
+            st.gf[i] = nullptr;
+        }
     }
 }
 
 void release(Stream& st)
 {
-    drop_graphs(st);
+    drop_graphs(st, true);
     st.inflight.clear();
     for (int i = 0; i < R_OUT; ++i) {
         if (st.ev_done[i]) (void)hipEventDestroy(st.ev_done[i]);
         st.ev_done[i] = nullptr;
     }
-    if (st.ev_read) (void)hipEventDestroy(st.ev_read);
-    st.ev_read = nullptr;
+    for (hipEvent_t* e : {&st.ev_flow, &st.ev_read}) {
+        if (*e) (void)hipEventDestroy(*e);
+        *e = nullptr;
+    }
     if (st.pipe) vsc_pipeline_destroy(st.pipe);
     st.pipe = nullptr;
     if (st.orig[0]) (void)hipFree(st.orig[0]);
@@ -104,6 +137,138 @@ void release(Stream& st)
     for (int i = 0; i < R_OUT; ++i) st.out[i] = nullptr;
     st.tags.clear();
     st.open = false;
+}
+
+// make `h` wait until window `c` of the stream has been stabilized (if it was launched)
+int wait_window(const Stream& st, long long c, hipStream_t h)
+{
+    if (c < 1 || c > st.done_to) return VSC_OK;
+    return check_hip(hipStreamWaitEvent(h, st.ev_done[c % R_OUT], 0), "vsc_server: wait for a window");
+}
+
+vsc_window window_of(const Stream& st, long long c)
+{
+    return vsc_window{st.orig[(c - 1) % 3], st.orig[c % 3], st.orig[(c + 1) % 3],
+                      st.proc[(c - 1) % 3], st.proc[c % 3], st.proc[(c + 1) % 3]};
+}
+
+// capture `enqueue` (which enqueues on h) into a new graph exec
+template <class F>
+int capture(hipStream_t h, hipGraphExec_t* out, F enqueue)
+{
+    hipGraph_t g = nullptr;
+    if (check_hip(hipStreamBeginCapture(h, hipStreamCaptureModeThreadLocal), "vsc_server_tick capture"))
+        return VSC_E_HIP;
+    const int rc = enqueue();
+    const int rce = check_hip(hipStreamEndCapture(h, &g), "vsc_server_tick capture end");
+    if (rc || rce) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : rce;
+    }
+    const int rci = check_hip(hipGraphInstantiate(out, g, nullptr, nullptr, 0), "vsc_server_tick instantiate");
+    (void)hipGraphDestroy(g);
+    if (rci) *out = nullptr;
+    return rci;
+}
+
+// A whole frame of window c on the stream's pool stream (the unbatched path).
+int frame_alone(vsc_server* sv, Stream& st, long long c)
+{
+    const vsc_window w = window_of(st, c);
+    int rc;
+    // the previous frame may have been stabilized in a batch on the tick stream
+    if ((rc = wait_window(st, c - 1, st.hs))) return rc;
+    const int ph = (int)(c % N_PHASES);
+    auto enqueue = [&]() {
+        return vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
+                                  reinterpret_cast<vsc_hip_stream>(st.hs));
+    };
+    if (sv->cfg.use_graphs && c >= 3 && st.ge[ph]) {
+        // steady state: step the host state, replay the phase's graph
+        if ((rc = pipeline_advance(st.pipe, c))) return rc;
+        if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
+    } else if (sv->cfg.use_graphs && c >= 3) {
+        // a failed capture leaves the pipeline's host state as it was
+        // (vsc_pipeline_frame restores it), so the frame is retried whole
+        if ((rc = capture(st.hs, &st.ge[ph], enqueue))) return rc;
+        if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
+    } else if ((rc = enqueue())) {
+        return rc;
+    }
+    return check_hip(hipEventRecord(st.ev_done[c % R_OUT], st.hs), "vsc_server_tick");
+}
+
+// The flow work of window c on the stream's pool stream, and its batch item.
+int frame_flows_batched(vsc_server* sv, Stream& st, long long c, BatchFrame* item)
+{
+    const vsc_window w = window_of(st, c);
+    int rc;
+    // the flows of window c overwrite the buffers window c-2 stabilized from
+    if ((rc = wait_window(st, c - 2, st.hs))) return rc;
+    const int ph = (int)(c % N_PHASES);
+    auto enqueue = [&]() { return pipeline_flows_part(st.pipe, &w, c, st.hs); };
+    if (sv->cfg.use_graphs && c >= 3 && st.gf[ph]) {
+        if ((rc = pipeline_advance(st.pipe, c))) return rc;
+        if (check_hip(hipGraphLaunch(st.gf[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
+    } else if (sv->cfg.use_graphs && c >= 3) {
+        if ((rc = capture(st.hs, &st.gf[ph], enqueue))) return rc;
+        if (check_hip(hipGraphLaunch(st.gf[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
+    } else if ((rc = enqueue())) {
+        return rc;
+    }
+    if (check_hip(hipEventRecord(st.ev_flow, st.hs), "vsc_server_tick") ||
+        check_hip(hipStreamWaitEvent(sv->ts, st.ev_flow, 0), "vsc_server_tick"))
+        return VSC_E_HIP;
+    return pipeline_batch_item(st.pipe, &w, c, &st.params, st.out[c % R_OUT], item);
+}
+
+// The stabilization of this round's batch items on the tick stream.
+int run_batch(vsc_server* sv)
+{
+    const int n = (int)sv->items.size();
+    const int fma = sv->cfg.form.solver ? 1 : 0;
+    const int b = sv->wl_next;
+    sv->wl_next ^= 1;
+    // the host list is refilled only after its previous copy has run
+    if (sv->wl_used[b] && check_hip(hipEventSynchronize(sv->ev_wl[b]), "vsc_server_tick")) return VSC_E_HIP;
+    batch_build(sv->wl_host[b], n, sv->items.data(), n, fma);
+    const BatchHeader* hd = reinterpret_cast<const BatchHeader*>(sv->wl_host[b]);
+    const int nl[2] = {hd->n_launch[0], hd->n_launch[1]};
+    const size_t bytes = batch_worklist_bytes(n);
+    auto enqueue = [&]() {
+        if (check_hip(hipMemcpyAsync(sv->wl_dev, sv->wl_host[b], bytes, hipMemcpyHostToDevice, sv->ts),
+                      "vsc_server_tick work list"))
+            return VSC_E_HIP;
+        return launch_batch(sv->wl_dev, nl, fma, sv->grid, sv->ts);
+    };
+    int rc;
+    if (sv->cfg.use_graphs) {
+        const auto key = std::make_tuple(b, n, nl[0], nl[1]);
+        auto it = sv->batch_graphs.find(key);
+        if (it == sv->batch_graphs.end()) {
+            hipGraphExec_t ge = nullptr;
+            if ((rc = capture(sv->ts, &ge, enqueue))) return rc;
+            it = sv->batch_graphs.emplace(key, ge).first;
+        }
+        if (check_hip(hipGraphLaunch(it->second, sv->ts), "vsc_server_tick batch launch")) return VSC_E_HIP;
+    } else if ((rc = enqueue())) {
+        return rc;
+    }
+    if (check_hip(hipEventRecord(sv->ev_wl[b], sv->ts), "vsc_server_tick")) return VSC_E_HIP;
+    sv->wl_used[b] = true;
+    return VSC_OK;
+}
+
+void launched(Stream& st, long long c)
+{
+    st.inflight.push_back(Done{st.tags.front(), c, (int)(c % R_OUT)});
+    st.tags.pop_front();
+    st.done_to = c;
+}
+
+bool has_window(const Stream& st)
+{
+    return st.open && st.done_to < st.ready && st.done_to + 1 - R_OUT <= st.polled_to;
 }
 
 }  // namespace
@@ -122,6 +287,7 @@ extern "C" void vsc_server_default_cfg(vsc_server_cfg* c)
     c->form.downscale = 2;
     c->form.upsample = 1;
     c->use_graphs = 1;
+    c->batch = 1;
 }
 
 extern "C" int vsc_server_create(const vsc_server_cfg* cfg, const float* upsampler_params, vsc_server** out)
@@ -133,6 +299,7 @@ extern "C" int vsc_server_create(const vsc_server_cfg* cfg, const float* upsampl
     if (cfg->form.overlap != 0)
         return set_error(VSC_E_INVALID, "vsc_server_create: form.overlap must be 0 (the server orders each stream's "
                                         "frames on its own HIP stream)");
+    if (cfg->batch != 0 && cfg->batch != 1) return set_error(VSC_E_INVALID, "vsc_server_create: batch must be 0 or 1");
     if (cfg->form.downscale > 1 && cfg->form.upsample == 1 && !upsampler_params)
         return set_error(VSC_E_INVALID, "vsc_server_create: the learned upsampler needs its parameters");
     vsc_server* sv = new (std::nothrow) vsc_server();
@@ -140,13 +307,30 @@ extern "C" int vsc_server_create(const vsc_server_cfg* cfg, const float* upsampl
     sv->cfg = *cfg;
     if (upsampler_params) sv->ups_params.assign(upsampler_params, upsampler_params + vsc_upsampler_param_count());
     sv->s.resize(cfg->max_streams);
+    auto fail = [&](const char* what) {
+        vsc_server_destroy(sv);
+        return set_error(VSC_E_HIP, std::string("vsc_server_create: ") + what);
+    };
     for (int i = 0; i < cfg->hip_streams; ++i) {
         hipStream_t h = nullptr;
-        if (hipStreamCreateWithFlags(&h, hipStreamNonBlocking) != hipSuccess) {
-            vsc_server_destroy(sv);
-            return set_error(VSC_E_HIP, "vsc_server_create: stream creation failed");
-        }
+        if (hipStreamCreateWithFlags(&h, hipStreamNonBlocking) != hipSuccess) return fail("stream creation failed");
         sv->pool.push_back(h);
+    }
+    if (cfg->batch) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return fail("device query failed");
+        sv->grid = cus >= 8 ? cus / 8 * 8 : 8;
+        const size_t wl = batch_worklist_bytes(cfg->max_streams);
+        if (hipStreamCreateWithFlags(&sv->ts, hipStreamNonBlocking) != hipSuccess) return fail("stream creation failed");
+        if (hipMalloc(&sv->wl_dev, wl) != hipSuccess) return fail("work list allocation failed");
+        for (int i = 0; i < 2; ++i)
+            if (hipHostMalloc(&sv->wl_host[i], wl, hipHostMallocDefault) != hipSuccess ||
+                hipEventCreateWithFlags(&sv->ev_wl[i], hipEventDisableTiming) != hipSuccess)
+                return fail("work list allocation failed");
+        sv->items.reserve(cfg->max_streams);
+        sv->item_stream.reserve(cfg->max_streams);
     }
     *out = sv;
     return VSC_OK;
@@ -156,8 +340,16 @@ extern "C" int vsc_server_destroy(vsc_server* sv)
 {
     if (!sv) return VSC_OK;
     for (hipStream_t h : sv->pool) (void)hipStreamSynchronize(h);
+    if (sv->ts) (void)hipStreamSynchronize(sv->ts);
     for (Stream& st : sv->s)
         if (st.open) release(st);
+    for (auto& kv : sv->batch_graphs) (void)hipGraphExecDestroy(kv.second);
+    for (int i = 0; i < 2; ++i) {
+        if (sv->wl_host[i]) (void)hipHostFree(sv->wl_host[i]);
+        if (sv->ev_wl[i]) (void)hipEventDestroy(sv->ev_wl[i]);
+    }
+    if (sv->wl_dev) (void)hipFree(sv->wl_dev);
+    if (sv->ts) (void)hipStreamDestroy(sv->ts);
     for (hipStream_t h : sv->pool) (void)hipStreamDestroy(h);
     delete sv;
     return VSC_OK;
@@ -205,6 +397,7 @@ extern "C" int vsc_server_open(vsc_server* sv, int width, int height, int* id)
     }
     for (int i = 0; i < R_OUT && !rc; ++i)
         rc = check_hip(hipEventCreateWithFlags(&st.ev_done[i], hipEventDisableTiming), "vsc_server_open");
+    if (!rc) rc = check_hip(hipEventCreateWithFlags(&st.ev_flow, hipEventDisableTiming), "vsc_server_open");
     if (!rc) rc = check_hip(hipEventCreateWithFlags(&st.ev_read, hipEventDisableTiming), "vsc_server_open");
     if (rc) {
         release(st);
@@ -230,6 +423,7 @@ extern "C" int vsc_server_close(vsc_server* sv, int id)
     Stream* st = get(sv, id);
     if (!st) return set_error(VSC_E_INVALID, "vsc_server_close: no such stream");
     (void)hipStreamSynchronize(st->hs);
+    if (sv->ts) (void)hipStreamSynchronize(sv->ts);
     sv->used -= st->bytes;
     release(*st);
     return VSC_OK;
@@ -243,7 +437,7 @@ extern "C" int vsc_server_set_params(vsc_server* sv, int id, const vsc_params* p
     int rc = vsc_pipeline_stream(st->pipe, &vs);
     if (rc || (rc = stream_params_check(vs, p))) return rc;   // refused: the stream keeps its parameters
     st->params = *p;   // read by the next tick (hyperParams are copied per frame, videostabilizer.cpp:192)
-    drop_graphs(*st);  // captured graphs hold the old values as kernel arguments
+    drop_graphs(*st, false);   // whole-frame graphs hold the old values as kernel arguments
     return VSC_OK;
 }
 
@@ -255,9 +449,11 @@ extern "C" int vsc_server_submit(vsc_server* sv, int id, const uint8_t* orig, co
     if (t >= 3 && st->done_to < t - 2)
         return set_error(VSC_E_AGAIN, "vsc_server_submit: window t-2 not launched yet (poll, tick, retry)");
     const size_t fb = (size_t)st->W * st->H * 4;
-    // the slot's previous frame (t - 3) was last read by window t - 2, enqueued
-    // earlier on this same HIP stream: stream order makes the overwrite safe
-    if (check_hip(hipMemcpyAsync(st->orig[t % 3], orig, fb, hipMemcpyDefault, st->hs), "vsc_server_submit") ||
+    // the slot's previous frame (t - 3) was last read by window t - 2 (its
+    // flow work on this HIP stream, its stabilization here or on the tick
+    // stream): the copies wait for that window
+    if (wait_window(*st, t - 2, st->hs) ||
+        check_hip(hipMemcpyAsync(st->orig[t % 3], orig, fb, hipMemcpyDefault, st->hs), "vsc_server_submit") ||
         check_hip(hipMemcpyAsync(st->proc[t % 3], proc, fb, hipMemcpyDefault, st->hs), "vsc_server_submit"))
         return VSC_E_HIP;
     if (t == 2) {
@@ -274,59 +470,7 @@ extern "C" int vsc_server_submit(vsc_server* sv, int id, const uint8_t* orig, co
     return VSC_OK;
 }
 
-namespace {
-
-// Launch every ready window of one stream whose output slot is free.
-int tick_stream(vsc_server* sv, Stream& st, int* n)
-{
-    while (st.done_to < st.ready && st.done_to + 1 - R_OUT <= st.polled_to) {
-        const long long c = st.done_to + 1;
-        vsc_window w{st.orig[(c - 1) % 3], st.orig[c % 3], st.orig[(c + 1) % 3],
-                     st.proc[(c - 1) % 3], st.proc[c % 3], st.proc[(c + 1) % 3]};
-        int rc;
-        const int ph = (int)(c % N_PHASES);
-        if (sv->cfg.use_graphs && c >= 3 && st.ge[ph]) {
-            // steady state: replay the phase's graph, step the host state
-            if ((rc = pipeline_advance(st.pipe, c))) return rc;
-            if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
-        } else if (sv->cfg.use_graphs && c >= 3) {
-            hipGraph_t g = nullptr;
-            if (check_hip(hipStreamBeginCapture(st.hs, hipStreamCaptureModeThreadLocal), "vsc_server_tick capture"))
-                return VSC_E_HIP;
-            // a failed capture leaves the pipeline's host state as it was
-            // (vsc_pipeline_frame restores it), so the frame is retried whole
-            rc = vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
-                                    reinterpret_cast<vsc_hip_stream>(st.hs));
-            const int rce = check_hip(hipStreamEndCapture(st.hs, &g), "vsc_server_tick capture end");
-            if (rc || rce) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc ? rc : rce;
-            }
-            rc = check_hip(hipGraphInstantiate(&st.ge[ph], g, nullptr, nullptr, 0), "vsc_server_tick instantiate");
-            (void)hipGraphDestroy(g);
-            if (rc) {
-                st.ge[ph] = nullptr;
-                return rc;
-            }
-            if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
-        } else {
-            rc = vsc_pipeline_frame(st.pipe, &w, c, &st.params, st.out[c % R_OUT], nullptr,
-                                    reinterpret_cast<vsc_hip_stream>(st.hs));
-            if (rc) return rc;
-        }
-        const int slot = (int)(c % R_OUT);
-        if (check_hip(hipEventRecord(st.ev_done[slot], st.hs), "vsc_server_tick")) return VSC_E_HIP;
-        st.inflight.push_back(Done{st.tags.front(), c, slot});
-        st.tags.pop_front();
-        st.done_to = c;
-        ++*n;
-    }
-    return VSC_OK;
-}
-
-}  // namespace
-
-extern "C" int vsc_server_tick(vsc_server* sv, int* launched)
+extern "C" int vsc_server_tick(vsc_server* sv, int* launched_out)
 {
     if (!sv) return set_error(VSC_E_INVALID, "vsc_server_tick: NULL server");
     // a stream whose frame fails keeps that window ready (it is retried at the
@@ -334,15 +478,58 @@ extern "C" int vsc_server_tick(vsc_server* sv, int* launched)
     // status is returned, its message in vsc_last_error
     int n = 0, first_rc = VSC_OK;
     std::string first_msg;
-    for (Stream& st : sv->s) {
-        if (!st.open) continue;
-        const int rc = tick_stream(sv, st, &n);
+    auto note = [&](int rc) {
         if (rc && first_rc == VSC_OK) {
             first_rc = rc;
             first_msg = vsc_last_error();
         }
+    };
+    std::vector<char> failed(sv->s.size(), 0);
+    // rounds: each takes the next ready window of every stream
+    for (bool more = true; more;) {
+        more = false;
+        sv->items.clear();
+        sv->item_stream.clear();
+        for (int id = 0; id < (int)sv->s.size(); ++id) {
+            Stream& st = sv->s[id];
+            if (failed[id] || !has_window(st)) continue;
+            const long long c = st.done_to + 1;
+            int rc;
+            if (sv->cfg.batch && pipeline_batch_check(st.pipe, &st.params) == VSC_OK) {
+                sv->items.emplace_back();
+                rc = frame_flows_batched(sv, st, c, &sv->items.back());
+                if (rc) {
+                    sv->items.pop_back();
+                } else {
+                    sv->item_stream.push_back(id);
+                }
+            } else {
+                rc = frame_alone(sv, st, c);
+                if (!rc) {
+                    launched(st, c);
+                    ++n;
+                }
+            }
+            if (rc) {
+                failed[id] = 1;
+                note(rc);
+            }
+        }
+        if (!sv->items.empty()) {
+            const int rc = run_batch(sv);
+            if (rc) return rc;   // the device is in an unknown state: report at once
+            for (int id : sv->item_stream) {
+                Stream& st = sv->s[id];
+                const long long c = st.done_to + 1;
+                if (check_hip(hipEventRecord(st.ev_done[c % R_OUT], sv->ts), "vsc_server_tick")) return VSC_E_HIP;
+                launched(st, c);
+                ++n;
+            }
+        }
+        for (int id = 0; id < (int)sv->s.size() && !more; ++id)
+            more = !failed[id] && has_window(sv->s[id]);
     }
-    if (launched) *launched = n;
+    if (launched_out) *launched_out = n;
     if (first_rc) return set_error(first_rc, first_msg);
     return VSC_OK;
 }
@@ -379,6 +566,7 @@ extern "C" int vsc_server_drain(vsc_server* sv)
     if (!sv) return set_error(VSC_E_INVALID, "vsc_server_drain: NULL server");
     for (hipStream_t h : sv->pool)
         if (check_hip(hipStreamSynchronize(h), "vsc_server_drain")) return VSC_E_HIP;
+    if (sv->ts && check_hip(hipStreamSynchronize(sv->ts), "vsc_server_drain")) return VSC_E_HIP;
     return VSC_OK;
 }
 
@@ -391,7 +579,8 @@ extern "C" int vsc_server_read(vsc_server* sv, int id, long long frame, uint8_t*
     const size_t fb = (size_t)st->W * st->H * 4;
     // the read event is re-recorded by each read: a wait enqueued on `stream`
     // holds the state it had when the wait was enqueued
-    if (check_hip(hipMemcpyAsync(dst, st->out[frame % R_OUT], fb, hipMemcpyDefault, st->hs), "vsc_server_read") ||
+    if (wait_window(*st, frame, st->hs) ||
+        check_hip(hipMemcpyAsync(dst, st->out[frame % R_OUT], fb, hipMemcpyDefault, st->hs), "vsc_server_read") ||
         check_hip(hipEventRecord(st->ev_read, st->hs), "vsc_server_read") ||
         check_hip(hipStreamWaitEvent(as_stream(stream), st->ev_read, 0), "vsc_server_read"))
         return VSC_E_HIP;

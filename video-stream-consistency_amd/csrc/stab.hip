@@ -93,11 +93,8 @@ static bool planar4(const View& v)
 // loads the 4 taps once per channel and blends them as bilinear_sample does;
 // VEC writes each output row pair with one 8-byte store (planar output).
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_bilinear_up2(View in, View out)
+__device__ __forceinline__ void up2_px(const View& in, const View& out, int x, int y)
 {
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int y = blockIdx.y * BY + threadIdx.y;
-    if (x >= in.w || y >= in.h) return;
     const int x1 = min(x + 1, in.w - 1), y1 = min(y + 1, in.h - 1);
     for (int c = 0; c < out.ch; ++c) {
         const float v00 = in.rd(x, y, c), v10 = in.rd(x1, y, c);
@@ -115,6 +112,15 @@ __global__ __launch_bounds__(256) void k_bilinear_up2(View in, View out)
             }
         }
     }
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_bilinear_up2(View in, View out)
+{
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    if (x >= in.w || y >= in.h) return;
+    up2_px<VEC>(in, out, x, y);
 }
 
 int launch_bilinear(View in, View out, hipStream_t s)
@@ -175,11 +181,8 @@ int launch_rgba_to_view(const uint8_t* in, View out, hipStream_t s)
     return check_launch("rgba_to_float");
 }
 
-__global__ __launch_bounds__(256) void k_to_rgba(View in, uint8_t* out)
+__device__ __forceinline__ void to_rgba_px(const View& in, uint8_t* out, int x, int y)
 {
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int y = blockIdx.y * BY + threadIdx.y;
-    if (x >= in.w || y >= in.h) return;
     uchar4 v;
     v.x = to_u8(in.rd(x, y, 0));
     v.y = to_u8(in.rd(x, y, 1));
@@ -188,13 +191,18 @@ __global__ __launch_bounds__(256) void k_to_rgba(View in, uint8_t* out)
     reinterpret_cast<uchar4*>(out)[(long long)y * in.w + x] = v;
 }
 
+__global__ __launch_bounds__(256) void k_to_rgba(View in, uint8_t* out)
+{
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    if (x >= in.w || y >= in.h) return;
+    to_rgba_px(in, out, x, y);
+}
+
 // Planar frames whose rows are multiples of 4 pixels: 4 pixels per thread,
 // 16-byte loads and stores (the same conversion as k_to_rgba).
-__global__ __launch_bounds__(256) void k_to_rgba_planar4(View in, uint8_t* out)
+__device__ __forceinline__ void to_rgba4_px(const View& in, uint8_t* out, int x0, int y)
 {
-    const int x0 = 4 * (blockIdx.x * 64 + threadIdx.x % 64);
-    const int y = blockIdx.y * 4 + threadIdx.x / 64;
-    if (x0 >= in.w || y >= in.h) return;
     const float4 r = *reinterpret_cast<const float4*>(in.d + in.idx(x0, y, 0));
     const float4 g = *reinterpret_cast<const float4*>(in.d + in.idx(x0, y, 1));
     const float4 b = *reinterpret_cast<const float4*>(in.d + in.idx(x0, y, 2));
@@ -203,6 +211,14 @@ __global__ __launch_bounds__(256) void k_to_rgba_planar4(View in, uint8_t* out)
     };
     const uint4 v = make_uint4(px(r.x, g.x, b.x), px(r.y, g.y, b.y), px(r.z, g.z, b.z), px(r.w, g.w, b.w));
     *reinterpret_cast<uint4*>(out + ((long long)y * in.w + x0) * 4) = v;
+}
+
+__global__ __launch_bounds__(256) void k_to_rgba_planar4(View in, uint8_t* out)
+{
+    const int x0 = 4 * (blockIdx.x * 64 + threadIdx.x % 64);
+    const int y = blockIdx.y * 4 + threadIdx.x / 64;
+    if (x0 >= in.w || y >= in.h) return;
+    to_rgba4_px(in, out, x0, y);
 }
 
 int launch_to_rgba(View in, uint8_t* out, hipStream_t s)
@@ -437,25 +453,23 @@ __device__ __forceinline__ void sweep_strip_fma(float (&o)[R], float (&u)[R], co
     }
 }
 
+// LDS row stride of a strip (floats): an odd number of 16-byte groups
+template <int R> constexpr int sb_rs() { return ((R / 4) % 2 == 0) ? R + 4 : R + 8; }
+
+// One tile of the temporally blocked solver: `sweeps` Jacobi sweeps of
+// channel plane tile `tile` (tiles_x x tiles_y tiles per channel, channels
+// one after the other) of a planar 3-channel level.  sO: the block's LDS
+// exchange buffer.  Every thread of the block calls it with the same tile.
 template <int R, int STRIPS, bool FMA>
-__global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
+__device__ __forceinline__ void consist_tile(
     const float* __restrict__ LapP, const float* __restrict__ A, const float* __restrict__ Wt, const float* Oin,
     float* Oout, const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step, float mom,
-    int halo, int tiles_x, int tiles_y, int per_xcd, int store_u)
+    int halo, int tiles_x, int tiles_y, int store_u, int tile, float (*sO)[STRIPS][SB_TW + 1][sb_rs<R>()])
 {
     constexpr int TW = SB_TW;
     constexpr int TH = STRIPS * R;
-    constexpr int RS = ((R / 4) % 2 == 0) ? R + 4 : R + 8;
+    constexpr int RS = sb_rs<R>();
     static_assert(R % 4 == 0 && (RS / 4) % 2 == 1, "strip stride must be an odd number of 16-byte groups");
-    // column TW of every plane and strip holds zeros: the excluded neighbour
-    __shared__ __attribute__((aligned(16))) float sO[2][STRIPS][TW + 1][RS];
-
-    // XCD-aware tile order: blocks are dispatched to the 8 XCDs round robin
-    // (block b runs on XCD b % 8), so give XCD x the contiguous run of tiles
-    // [x*per_xcd, (x+1)*per_xcd) in raster order; neighbouring tiles then
-    // share that XCD's L2 for their overlapping halo reads.
-    const int tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    if (tile >= tiles_x * tiles_y * 3) return;
     const int c = tile / (tiles_x * tiles_y);
     const int bxy = tile - c * tiles_x * tiles_y;
     const int bty = bxy / tiles_x, btx = bxy - bty * tiles_x;
@@ -596,6 +610,24 @@ __global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
             }
         }
     }
+}
+
+template <int R, int STRIPS, bool FMA>
+__global__ __launch_bounds__(SB_TW * STRIPS) void k_consist_blocked(
+    const float* __restrict__ LapP, const float* __restrict__ A, const float* __restrict__ Wt, const float* Oin,
+    float* Oout, const float* Uin, float* Uout, int W, int H, int sweeps, int first_sweep, float step, float mom,
+    int halo, int tiles_x, int tiles_y, int per_xcd, int store_u)
+{
+    // column SB_TW of every plane and strip holds zeros: the excluded neighbour
+    __shared__ __attribute__((aligned(16))) float sO[2][STRIPS][SB_TW + 1][sb_rs<R>()];
+    // XCD-aware tile order: blocks are dispatched to the 8 XCDs round robin
+    // (block b runs on XCD b % 8), so give XCD x the contiguous run of tiles
+    // [x*per_xcd, (x+1)*per_xcd) in raster order; neighbouring tiles then
+    // share that XCD's L2 for their overlapping halo reads.
+    const int tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (tile >= tiles_x * tiles_y * 3) return;
+    consist_tile<R, STRIPS, FMA>(LapP, A, Wt, Oin, Oout, Uin, Uout, W, H, sweeps, first_sweep, step, mom, halo,
+                                 tiles_x, tiles_y, store_u, tile, sO);
 }
 
 // lapP of one planar level: the processed-frame half of the Laplacian in
@@ -809,12 +841,10 @@ __device__ inline int lap_rgba(const Rgba32& im, int x, int y, int st, int xl, i
     return cnt;
 }
 
-__global__ __launch_bounds__(256) void k_front(FrontArgs a)
+// the fused front pass at pixel (x, y) (inside the frame)
+__device__ __forceinline__ void front_px(const FrontArgs& a, int x, int y)
 {
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int y = blockIdx.y * BY + threadIdx.y;
     const int W = a.A0.w, H = a.A0.h;
-    if (x >= W || y >= H) return;
     const Rgba32 oPrev{reinterpret_cast<const uint32_t*>(a.oPrev.p), W};
     const Rgba32 oCur{reinterpret_cast<const uint32_t*>(a.oCur.p), W};
     const Rgba32 oNext{reinterpret_cast<const uint32_t*>(a.oNext.p), W};
@@ -870,6 +900,14 @@ __global__ __launch_bounds__(256) void k_front(FrontArgs a)
     }
 }
 
+__global__ __launch_bounds__(256) void k_front(FrontArgs a)
+{
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    if (x >= a.A0.w || y >= a.A0.h) return;
+    front_px(a, x, y);
+}
+
 int launch_front_pass(const FrontArgs& a, hipStream_t s)
 {
     // algorithmic bytes per pixel: 6 RGBA8 frames (24), previous output (12),
@@ -887,6 +925,218 @@ int launch_front_pass(const FrontArgs& a, hipStream_t s)
     hipLaunchKernelGGL(k_front, grid2d(a.A0.w, a.A0.h), dim3(BX, BY), 0, s, a);
     prof_stop(slot, s);
     return check_launch("front_pass");
+}
+
+// ---------------------------------------------------------------------------
+// Batched frames: persistent kernels over a work list (layout in
+// batch_worklist_bytes / batch_build).  Each kernel walks "virtual blocks"
+// (front pass, resample, RGBA8) or solver tiles of all frames of the batch;
+// a frame's share is found by binary search in the work list's prefix table.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+struct BatchLayout {
+    size_t frames, front, up, rgba, tiles, total;
+    int cap;
+    __host__ __device__ BatchLayout(int c) : cap(c)
+    {
+        frames = round256(sizeof(BatchHeader));
+        const size_t tab = round256((size_t)(c + 1) * sizeof(int));
+        front = frames + round256((size_t)c * sizeof(BatchFrame));
+        up = front + tab;
+        rgba = up + tab;
+        tiles = rgba + tab;
+        total = tiles + 2 * BATCH_MAX_LAUNCH * tab;
+    }
+    __host__ __device__ size_t tile_table(int slot, int launch) const
+    {
+        return tiles + (size_t)(slot * BATCH_MAX_LAUNCH + launch) * round256((size_t)(cap + 1) * sizeof(int));
+    }
+};
+
+// block-uniform values read from the work list, moved to scalar registers
+__device__ inline int ufirst(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float ufirst(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+template <typename T> __device__ inline T* ufirst(T* p)
+{
+    const uint64_t u = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+// largest f in [0, n) with prefix[f] <= v (prefix[0] = 0 <= v < prefix[n])
+__device__ inline int batch_find(const int* prefix, int n, int v)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int m = (lo + hi) >> 1;
+        if (prefix[m] <= v) lo = m;
+        else hi = m;
+    }
+    return lo;
+}
+
+int blocks2d(int w, int h) { return ((w + BX - 1) / BX) * ((h + BY - 1) / BY); }
+int rgba_blocks(const BatchFrame& f)
+{
+    const int W = f.front.A0.w, H = f.front.A0.h;
+    return f.rgba_vec ? ((W / 4 + 63) / 64) * ((H + 3) / 4) : blocks2d(W, H);
+}
+
+__global__ __launch_bounds__(256) void k_front_batch(const unsigned char* __restrict__ wl)
+{
+    const BatchHeader* hd = reinterpret_cast<const BatchHeader*>(wl);
+    const BatchLayout lay(hd->cap);
+    const BatchFrame* fr = reinterpret_cast<const BatchFrame*>(wl + lay.frames);
+    const int* pre = reinterpret_cast<const int*>(wl + lay.front);
+    const int n = hd->n_frames, total = pre[n];
+    for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const int f = batch_find(pre, n, vb);
+        const FrontArgs& a = fr[f].front;
+        const int local = vb - pre[f], gx = (a.A0.w + BX - 1) / BX;
+        const int x = (local % gx) * BX + (int)(threadIdx.x % BX), y = (local / gx) * BY + (int)(threadIdx.x / BX);
+        if (x < a.A0.w && y < a.A0.h) front_px(a, x, y);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_up2_batch(const unsigned char* __restrict__ wl)
+{
+    const BatchHeader* hd = reinterpret_cast<const BatchHeader*>(wl);
+    const BatchLayout lay(hd->cap);
+    const BatchFrame* fr = reinterpret_cast<const BatchFrame*>(wl + lay.frames);
+    const int* pre = reinterpret_cast<const int*>(wl + lay.up);
+    const int n = hd->n_frames, total = pre[n];
+    for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const int f = batch_find(pre, n, vb);
+        const BatchLevel& l0 = fr[f].lev[0];
+        const BatchLevel& l1 = fr[f].lev[1];
+        const View in = planar(l1.O[fr[f].up_idx], l1.w, l1.h, 3), out = planar(l0.O[0], l0.w, l0.h, 3);
+        const int local = vb - pre[f], gx = (l1.w + BX - 1) / BX;
+        const int x = (local % gx) * BX + (int)(threadIdx.x % BX), y = (local / gx) * BY + (int)(threadIdx.x / BX);
+        if (x < l1.w && y < l1.h) up2_px<true>(in, out, x, y);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rgba_batch(const unsigned char* __restrict__ wl)
+{
+    const BatchHeader* hd = reinterpret_cast<const BatchHeader*>(wl);
+    const BatchLayout lay(hd->cap);
+    const BatchFrame* fr = reinterpret_cast<const BatchFrame*>(wl + lay.frames);
+    const int* pre = reinterpret_cast<const int*>(wl + lay.rgba);
+    const int n = hd->n_frames, total = pre[n];
+    for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        const int f = batch_find(pre, n, vb);
+        const BatchLevel& l0 = fr[f].lev[0];
+        const View res = planar(l0.O[fr[f].out_idx], l0.w, l0.h, 3);
+        const int local = vb - pre[f];
+        if (fr[f].rgba_vec) {
+            const int gx = (l0.w / 4 + 63) / 64;
+            const int x0 = 4 * ((local % gx) * 64 + (int)(threadIdx.x % 64)), y = (local / gx) * 4 + (int)(threadIdx.x / 64);
+            if (x0 < l0.w && y < l0.h) to_rgba4_px(res, fr[f].out_rgba, x0, y);
+        } else {
+            const int gx = (l0.w + BX - 1) / BX;
+            const int x = (local % gx) * BX + (int)(threadIdx.x % BX), y = (local / gx) * BY + (int)(threadIdx.x / BX);
+            if (x < l0.w && y < l0.h) to_rgba_px(res, fr[f].out_rgba, x, y);
+        }
+    }
+}
+
+// Launch `launch` of level slot `slot` (0: level 0, 1: level 1) for every
+// frame of the batch: persistent blocks walk the tiles, XCD-aware (block b
+// runs on XCD b % 8 and takes tiles from that XCD's contiguous share).
+template <int R, bool FMA>
+__global__ __launch_bounds__(SB_TW * SB_STRIPS) void k_consist_batch(const unsigned char* __restrict__ wl, int slot,
+                                                                     int launch)
+{
+    __shared__ __attribute__((aligned(16))) float sO[2][SB_STRIPS][SB_TW + 1][sb_rs<R>()];
+    const BatchHeader* hd = reinterpret_cast<const BatchHeader*>(wl);
+    const BatchLayout lay(hd->cap);
+    const BatchFrame* fr = reinterpret_cast<const BatchFrame*>(wl + lay.frames);
+    const int* pre = reinterpret_cast<const int*>(wl + lay.tile_table(slot, launch));
+    const int n = ufirst(hd->n_frames), total = ufirst(pre[n]);
+    const int per_xcd = (total + 7) / 8;
+    const int xcd = blockIdx.x % 8, nk = gridDim.x / 8;
+    for (int i = blockIdx.x / 8; i < per_xcd; i += nk) {
+        const int t = xcd * per_xcd + i;
+        if (t >= total) break;
+        const int f = ufirst(batch_find(pre, n, t));
+        const BatchLevel& l = fr[f].lev[slot];
+        const int iters = ufirst(l.iters);
+        const int first = launch * BATCH_HALO;
+        const int sweeps = min(BATCH_HALO, iters - first);
+        const int in = ufirst(l.start) ^ (launch & 1), ui = launch & 1;
+        __syncthreads();   // the previous tile's last readers of sO are done
+        consist_tile<R, SB_STRIPS, FMA>(ufirst(l.L), ufirst(l.A), ufirst(l.Wt), ufirst(l.O[in]), ufirst(l.O[in ^ 1]),
+                                        ufirst(l.U[ui]), ufirst(l.U[ui ^ 1]), ufirst(l.w), ufirst(l.h), sweeps, first,
+                                        ufirst(fr[f].step), ufirst(fr[f].mom), BATCH_HALO, ufirst(l.tiles_x),
+                                        ufirst(l.tiles_y), first + sweeps < iters ? 1 : 0, t - ufirst(pre[f]), sO);
+    }
+}
+
+}  // namespace
+
+size_t batch_worklist_bytes(int cap) { return BatchLayout(cap).total; }
+
+void batch_tiles(int w, int h, int fma, int* tiles_x, int* tiles_y)
+{
+    *tiles_x = tile_count(w, SB_TW, BATCH_HALO);
+    *tiles_y = tile_count(h, SB_STRIPS * (fma ? SB_R_FMA : SB_R), BATCH_HALO);
+}
+
+void batch_build(unsigned char* host, int cap, const BatchFrame* frames, int n, int fma)
+{
+    (void)fma;
+    const BatchLayout lay(cap);
+    BatchHeader* hd = reinterpret_cast<BatchHeader*>(host);
+    hd->n_frames = n;
+    hd->cap = cap;
+    hd->n_launch[0] = hd->n_launch[1] = 0;
+    BatchFrame* fr = reinterpret_cast<BatchFrame*>(host + lay.frames);
+    int* front = reinterpret_cast<int*>(host + lay.front);
+    int* up = reinterpret_cast<int*>(host + lay.up);
+    int* rgba = reinterpret_cast<int*>(host + lay.rgba);
+    front[0] = up[0] = rgba[0] = 0;
+    for (int f = 0; f < n; ++f) {
+        fr[f] = frames[f];
+        front[f + 1] = front[f] + blocks2d(frames[f].front.A0.w, frames[f].front.A0.h);
+        up[f + 1] = up[f] + (frames[f].levels == 2 ? blocks2d(frames[f].lev[1].w, frames[f].lev[1].h) : 0);
+        rgba[f + 1] = rgba[f] + (frames[f].out_rgba ? rgba_blocks(frames[f]) : 0);
+        for (int j = 0; j < frames[f].levels; ++j)
+            hd->n_launch[j] = std::max(hd->n_launch[j], (frames[f].lev[j].iters + BATCH_HALO - 1) / BATCH_HALO);
+    }
+    for (int slot = 0; slot < 2; ++slot)
+        for (int k = 0; k < hd->n_launch[slot]; ++k) {
+            int* t = reinterpret_cast<int*>(host + lay.tile_table(slot, k));
+            t[0] = 0;
+            for (int f = 0; f < n; ++f) {
+                const BatchLevel& l = frames[f].lev[slot];
+                const bool on = slot < frames[f].levels && k * BATCH_HALO < l.iters;
+                t[f + 1] = t[f] + (on ? 3 * l.tiles_x * l.tiles_y : 0);
+            }
+        }
+}
+
+int launch_batch(const unsigned char* dev, const int n_launch[2], int fma, int grid, hipStream_t s)
+{
+    grid = std::max(8, grid / 8 * 8);
+    hipLaunchKernelGGL(k_front_batch, dim3(8 * grid), dim3(256), 0, s, dev);
+    for (int k = 0; k < n_launch[1]; ++k) {
+        if (fma) hipLaunchKernelGGL((k_consist_batch<SB_R_FMA, true>), dim3(grid), dim3(SB_TW * SB_STRIPS), 0, s, dev, 1, k);
+        else hipLaunchKernelGGL((k_consist_batch<SB_R, false>), dim3(grid), dim3(SB_TW * SB_STRIPS), 0, s, dev, 1, k);
+    }
+    hipLaunchKernelGGL(k_up2_batch, dim3(4 * grid), dim3(256), 0, s, dev);
+    for (int k = 0; k < n_launch[0]; ++k) {
+        if (fma) hipLaunchKernelGGL((k_consist_batch<SB_R_FMA, true>), dim3(grid), dim3(SB_TW * SB_STRIPS), 0, s, dev, 0, k);
+        else hipLaunchKernelGGL((k_consist_batch<SB_R, false>), dim3(grid), dim3(SB_TW * SB_STRIPS), 0, s, dev, 0, k);
+    }
+    hipLaunchKernelGGL(k_rgba_batch, dim3(4 * grid), dim3(256), 0, s, dev);
+    return check_launch("batched frames");
 }
 
 }  // namespace vsc

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <new>
+#include <string>
 #include <vector>
 
 using namespace vsc;
@@ -438,6 +439,96 @@ int frame_staged(vsc_stream* s, const vsc_frame_in* in, const vsc_params& p, uin
 }
 
 }  // namespace
+
+namespace vsc {
+
+// The fused frame of frame_fused as a batch item (the common forms: one level,
+// or two levels whose level 1 is the exact decimation of level 0).  The front
+// pass writes every level's constants; a single level's initial solution is P
+// written straight into the O buffer that does not hold `last` (the front pass
+// reads `last` while it runs).
+int stream_batch_frame(vsc_stream* s, const vsc_frame_in* in, const vsc_params* params, uint8_t* out_rgba, int fma,
+                       hipStream_t prep, BatchFrame* bf)
+{
+    if (!s || !bf) return set_error(VSC_E_INVALID, "vsc_stab_frame: null stream");
+    int rc = check_frame_in(s, in);
+    if (rc) return rc;
+    vsc_params p;
+    if (params) p = *params;
+    else vsc_default_params(&p);
+    if ((rc = check_params(s, p))) return rc;
+    const int L = p.pyramid_levels;
+    const int W = s->W, H = s->H;
+    if (L > 2 || (L == 2 && !batch_exact_2x(W, H)) || p.num_iter > BATCH_MAX_LAUNCH * BATCH_HALO)
+        return set_error(VSC_E_UNSUPPORTED, "batched frame: needs 1 level, or 2 with even frame sizes, and at most " +
+                                                std::to_string(BATCH_MAX_LAUNCH * BATCH_HALO) + " sweeps");
+    View fwd, bwd;
+    if ((rc = prepare_flows(s, in, &fwd, &bwd, prep))) return rc;
+    *bf = BatchFrame{};
+    FrontArgs& a = bf->front;
+    a.oPrev = Rgba{in->orig_prev, W, H};
+    a.oCur = Rgba{in->orig_cur, W, H};
+    a.oNext = Rgba{in->orig_next, W, H};
+    a.pPrev = Rgba{in->proc_prev, W, H};
+    a.pCur = Rgba{in->proc_cur, W, H};
+    a.pNext = Rgba{in->proc_next, W, H};
+    a.last = pl(s->O[0][s->last_idx], W, H);
+    a.fwd = fwd;
+    a.bwd = bwd;
+    a.L0 = pl(s->L[0], W, H);
+    a.A0 = pl(s->A[0], W, H);
+    a.W0 = pl(s->Wt[0], W, H);
+    a.decimate = L == 2 ? 1 : 0;
+    a.write_p = L == 1 ? 1 : 0;
+    const int start0 = L == 1 ? (s->last_idx ^ 1) : 0;
+    a.P0 = pl(s->O[0][start0], W, H);   // L == 1: P is the initial solution
+    if (L == 2) {
+        const int w1 = s->lw[1], h1 = s->lh[1];
+        a.L1 = pl(s->L[1], w1, h1);
+        a.A1 = pl(s->A[1], w1, h1);
+        a.W1 = pl(s->Wt[1], w1, h1);
+        a.O1 = pl(s->O[1][0], w1, h1);
+        a.P1 = a.O1;
+    } else {
+        a.L1 = a.A1 = a.W1 = a.O1 = a.P1 = a.A0;
+    }
+    a.alpha = p.alpha;
+    a.beta = p.beta;
+    a.gamma = p.gamma;
+    a.fma = fma ? 1 : 0;
+    a.step = p.step_size;
+    for (const View* v : {&a.fwd, &a.bwd, &a.last, &a.L0, &a.A0, &a.W0, &a.P0, &a.L1, &a.A1, &a.W1, &a.O1, &a.P1})
+        if (!fits32(*v)) return set_error(VSC_E_INVALID, "batched frame: image too large for 32-bit offsets");
+    bf->levels = L;
+    int fin[2] = {0, 0};
+    for (int j = 0; j < L; ++j) {
+        BatchLevel& l = bf->lev[j];
+        l.L = s->L[j];
+        l.A = s->A[j];
+        l.Wt = s->Wt[j];
+        for (int k = 0; k < 2; ++k) {
+            l.O[k] = s->O[j][k];
+            l.U[k] = s->U[j][k];
+        }
+        l.w = s->lw[j];
+        l.h = s->lh[j];
+        l.iters = p.num_iter / (j + 1);
+        l.start = j == 0 ? start0 : 0;
+        batch_tiles(l.w, l.h, fma, &l.tiles_x, &l.tiles_y);
+        const int launches = (l.iters + BATCH_HALO - 1) / BATCH_HALO;
+        fin[j] = l.start ^ (launches & 1);
+    }
+    bf->up_idx = fin[1];
+    bf->out_idx = fin[0];
+    bf->rgba_vec = (W % 4) == 0 && reinterpret_cast<uintptr_t>(out_rgba) % 16 == 0 ? 1 : 0;
+    bf->step = p.step_size;
+    bf->mom = p.mom_fac;
+    bf->out_rgba = out_rgba;
+    s->last_idx = fin[0];
+    return VSC_OK;
+}
+
+}  // namespace vsc
 
 extern "C" int vsc_stab_frame(vsc_stream* s, const vsc_frame_in* in, const vsc_params* params, uint8_t* out_rgba,
                               float* out_float, int mode, vsc_hip_stream stream)

@@ -59,7 +59,7 @@ inline bool fits32(const View& v) { return (long long)(v.ch - 1) * v.cs + ((long
 inline View hwc(const vsc_image& im) {
     return View{im.data, im.width, im.height, im.channels, 1, (long long)im.channels};
 }
-inline View planar(float* d, int w, int h, int ch) {
+__host__ __device__ inline View planar(float* d, int w, int h, int ch) {
     return View{d, w, h, ch, (long long)w * h, 1};
 }
 
@@ -256,6 +256,62 @@ struct FrontArgs {
 };
 int launch_front_pass(const FrontArgs& a, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// Batched frames (the multi-stream server): the fused frame of vsc_stab_frame
+// (front pass, 1- or 2-level pyramid solve, RGBA8 out) for many streams at
+// once, in persistent launches that walk a work list in device memory, so the
+// launch sequence is the same for every batch and can be replayed from one
+// captured graph.  Every solver launch of a batch runs BATCH_HALO sweeps per
+// frame (fewer at a level's end); the results equal the per-stream path's,
+// whose launch plan differs (the blocked solver is exact Jacobi for any plan).
+// ---------------------------------------------------------------------------
+constexpr int BATCH_HALO = 19;
+// a 2-level batch item needs level 1 to be level 0's exact decimation and the
+// upsampling back to be the exact 2x resample (even sizes, and the resampler's
+// x*w/(2w) exact in float: 2*w*w < 2^24)
+inline bool batch_exact_2x(int W, int H)
+{
+    const long long w = W / 2, h = H / 2;
+    return (W % 2) == 0 && (H % 2) == 0 && 2 * w * w < (1LL << 24) && 2 * h * h < (1LL << 24);
+}
+constexpr int BATCH_MAX_LAUNCH = 16;   // solver launches per level: num_iter <= 16 * 19
+struct BatchLevel {
+    const float* L;       // lapP (reference order) or k (FMA form)
+    const float* A;       // adapted processed frame (reference order)
+    const float* Wt;      // w (reference order) or q (FMA form)
+    float* O[2];
+    float* U[2];
+    int w, h, iters;
+    int start;            // O[start] holds the initial solution
+    int tiles_x, tiles_y; // tiles per channel at BATCH_HALO
+};
+struct BatchFrame {
+    FrontArgs front;
+    BatchLevel lev[2];    // level 0, level 1 (levels == 2)
+    int levels;
+    int up_idx;           // level-1 buffer holding its result
+    int out_idx;          // level-0 buffer holding the result
+    int rgba_vec;         // RGBA8 output with 4 pixels per thread
+    float step, mom;
+    uint8_t* out_rgba;
+};
+// work list: header, frames [cap], then int tables of cap+1 entries each:
+// front blocks, up-sample blocks, RGBA8 blocks, and the solver tiles of each
+// (level, launch)
+struct BatchHeader {
+    int n_frames;
+    int n_launch[2];      // solver launches at level 0 and level 1 (max over the frames)
+    int cap;
+};
+size_t batch_worklist_bytes(int cap);
+// host-side fill of a work list (in host memory) from n frames; returns the
+// solver launches per level in n_launch
+void batch_build(unsigned char* host, int cap, const BatchFrame* frames, int n, int fma);
+// the batch's launches on s: work list at `dev` (device copy of batch_build's
+// output), n_launch solver launches per level, `grid` blocks per persistent launch
+int launch_batch(const unsigned char* dev, const int n_launch[2], int fma, int grid, hipStream_t s);
+// tiles per channel of a level at BATCH_HALO
+void batch_tiles(int w, int h, int fma, int* tiles_x, int* tiles_y);
 
 // ---------------------------------------------------------------------------
 // 16-bit storage formats of the flow network (FP32 arithmetic in registers)
@@ -328,10 +384,27 @@ int upsampler_run_flows(vsc_upsampler* u, int n, const float* const flow_lo[2], 
 
 // host-state step of a pipeline frame replayed from a graph (pipeline.hip)
 int pipeline_advance(vsc_pipeline* p, long long frame);
+// Batched frames of a pipeline (the multi-stream server): whether the frame
+// can join a batch (checks of vsc_pipeline_frame, frame-size flows, the
+// batchable pyramid forms); the flows part of vsc_pipeline_frame alone on s
+// (host state restored on failure); and the frame's batch item after its
+// flows part was enqueued (host only; advances the stream's `last`).
+int pipeline_batch_check(vsc_pipeline* p, const vsc_params* params);
+int pipeline_flows_part(vsc_pipeline* p, const vsc_window* w, long long frame, hipStream_t s);
+int pipeline_batch_item(vsc_pipeline* p, const vsc_window* w, long long frame, const vsc_params* params,
+                        uint8_t* out_rgba, BatchFrame* bf);
 // parameter and state checks of vsc_stab_frame / vsc_pipeline_frame, made
 // before any state change (NULL params = defaults)
 int stream_params_check(const vsc_stream* s, const vsc_params* params);
 int stream_frame_check(const vsc_stream* s, const vsc_params* params);
+// One frame of vsc_stab_frame as a batch item: the host-side part of the
+// fused frame (flow upsampling, when the flows arrive downscaled, is launched
+// on `prep`).  VSC_E_UNSUPPORTED when the frame needs the per-stream path
+// (more than 2 levels, odd sizes at 2 levels, more sweeps than a batch runs).
+// Advances the stream's `last` as the frame would.
+int stream_batch_frame(vsc_stream* s, const vsc_frame_in* in, const vsc_params* params, uint8_t* out_rgba, int fma,
+                       hipStream_t prep, BatchFrame* out);
+
 // reduced-precision (BF16 / F16 / FP8 e4m3) correlation on MFMA, corr_mfma.hip
 size_t corr_mfma_workspace_bytes(int N, int C, int H, int W, int in_dtype);
 int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
