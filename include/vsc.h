@@ -112,6 +112,19 @@ int vsc_correlation_forward(const void* f1, const void* f2, void* out, int N, in
  * Errors: as vsc_correlation_forward, and VSC_E_INVALID when the workspace is
  * NULL or smaller than vsc_correlation_workspace_size().                    */
 size_t vsc_correlation_workspace_size(int N, int C, int H, int W, int in_dtype);
+/* No counterpart in the reference: the Correlation op for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results and errors as
+ * vsc_correlation_forward; the reduced-precision types use a static device
+ * scratch of VSC_CORRELATION_STATIC_SCRATCH bytes per GPU (held until the
+ * process ends) instead of an allocation: operands needing more
+ * (vsc_correlation_workspace_size) are refused with VSC_E_INVALID before any
+ * GPU work.  Calls that overlap on one GPU (two host threads or two streams)
+ * race on that scratch; serialise them.  Nothing is allocated or freed.    */
+#define VSC_CORRELATION_STATIC_SCRATCH (32u << 20)
+int vsc_correlation_forward_noalloc(const void* f1, const void* f2, void* out, int N, int C, int H, int W,
+                                    int max_displacement, int legacy, int in_dtype, const float* scale,
+                                    vsc_hip_stream stream);
 int vsc_correlation_forward_ws(const void* f1, const void* f2, void* out, int N, int C, int H, int W,
                                int max_displacement, int legacy, int in_dtype, const float* scale, void* workspace,
                                size_t workspace_bytes, vsc_hip_stream stream);

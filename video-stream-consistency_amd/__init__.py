@@ -60,6 +60,7 @@ EXPORTED = (
     "vsc_server_tick", "vsc_server_poll", "vsc_server_drain", "vsc_server_read", "vsc_flo_read", "vsc_flo_write",
     "vsc_flow_dir_path", "vsc_flow_dir_read", "vsc_stabilize_flow_dir", "vsc_correlation_workspace_size",
     "vsc_correlation_forward_ws", "vsc_quantize_fp8_pair_workspace_size", "vsc_quantize_fp8_pair_ws",
+    "vsc_correlation_forward_noalloc",
 )
 
 
@@ -150,6 +151,7 @@ def lib():
         L.vsc_correlation_workspace_size.argtypes = [i, i, i, i, i]
         L.vsc_correlation_workspace_size.restype = ctypes.c_size_t
         L.vsc_correlation_forward_ws.argtypes = [vp, vp, vp, i, i, i, i, i, i, i, vp, vp, ctypes.c_size_t, vp]
+        L.vsc_correlation_forward_noalloc.argtypes = [vp, vp, vp, i, i, i, i, i, i, i, vp, vp]
         L.vsc_convert_f32.argtypes = [vp, vp, ctypes.c_size_t, i, vp]
         L.vsc_warp_forward.argtypes = [vp, vp, vp, i, i, i, i, i, vp]
         L.vsc_stab_warp.argtypes = [img, img, img, vp]

@@ -3,6 +3,7 @@
 // shape contract of src/ort_custom_ops.
 #include "vsc_internal.h"
 
+#include <mutex>
 #include <string>
 
 namespace vsc {
@@ -300,6 +301,58 @@ extern "C" int vsc_correlation_forward(const void* f1, const void* f2, void* out
 {
     return correlation_forward(f1, f2, out, N, C, H, W, max_displacement, legacy, in_dtype, chan_scale, nullptr,
                                stream);
+}
+
+// ---------------------------------------------------------------------------
+// The Correlation op without any allocation: the channels-last operand copies
+// of the reduced-precision types live in a file-scope device array (one copy
+// per GPU, held until the process ends) instead of a stream-ordered
+// allocation.  Its address on each device is looked up once, with that
+// device current; host code uses that address, never the array's name.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[VSC_CORRELATION_STATIC_SCRATCH];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device
+int static_scratch(void** out)
+{
+    int dev = 0;
+    if (check_hip(hipGetDevice(&dev), "vsc_correlation_forward_noalloc")) return VSC_E_HIP;
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(VSC_E_UNSUPPORTED, "vsc_correlation_forward_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "vsc_correlation_forward_noalloc scratch"))
+            return VSC_E_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return VSC_OK;
+}
+
+}  // namespace
+
+extern "C" int vsc_correlation_forward_noalloc(const void* f1, const void* f2, void* out, int N, int C, int H, int W,
+                                               int max_displacement, int legacy, int in_dtype, const float* chan_scale,
+                                               vsc_hip_stream stream)
+{
+    const size_t need = (N < 1 || C < 1 || H < 1 || W < 1) ? 0 : corr_mfma_workspace_bytes(N, C, H, W, in_dtype);
+    if (need > VSC_CORRELATION_STATIC_SCRATCH)
+        return set_error(VSC_E_INVALID, "vsc_correlation_forward_noalloc: the operands need " + std::to_string(need) +
+                                            " bytes of scratch, more than the static " +
+                                            std::to_string((size_t)VSC_CORRELATION_STATIC_SCRATCH));
+    void* ws = nullptr;
+    if (need) {
+        int rc = static_scratch(&ws);
+        if (rc) return rc;
+    }
+    return correlation_forward(f1, f2, out, N, C, H, W, max_displacement, legacy, in_dtype, chan_scale, ws, stream);
 }
 
 extern "C" size_t vsc_correlation_workspace_size(int N, int C, int H, int W, int in_dtype)
