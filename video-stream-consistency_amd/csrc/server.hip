@@ -14,7 +14,8 @@
 // pyramid solve, RGBA8 out) runs as one captured graph on the server's tick
 // stream: a copy of the tick's work list into device memory followed by
 // persistent launches that walk it, one launch per solver step for all
-// streams together.  Unbatched (or for a window the batch cannot take: more
+// streams together (consecutive batches alternate between two tick streams,
+// so one may start while the previous runs).  Unbatched (or for a window the batch cannot take: more
 // than 2 pyramid levels, odd sizes at 2 levels), a stream's whole frame runs
 // on its pool stream, replayed from per-stream graphs.
 //
@@ -85,17 +86,23 @@ struct vsc_server {
     std::vector<hipStream_t> pool;
     size_t used = 0;
     int rr = 0;
-    // batched ticks
-    hipStream_t ts = nullptr;                 // the tick stream
-    unsigned char* wl_dev = nullptr;          // device work list
-    unsigned char* wl_host[2] = {};           // pinned host work lists, used alternately
-    hipEvent_t ev_wl[2] = {};                 // the copy that last read wl_host[i]
-    bool wl_used[2] = {false, false};
-    int wl_next = 0;
+    // batched ticks: batches go round robin to tick streams, each with its
+    // device work list and two pinned host work lists used alternately
+    struct TickLane {
+        hipStream_t ts = nullptr;
+        unsigned char* wl_dev = nullptr;
+        unsigned char* wl_host[2] = {};
+        hipEvent_t ev_wl[2] = {};             // the copy that last read wl_host[i]
+        bool wl_used[2] = {false, false};
+        int wl_next = 0;
+    };
+    std::vector<TickLane> lanes;
+    int lane_next = 0;
     int grid = 256;                           // blocks of a persistent launch (one per CU)
     std::vector<vsc::BatchFrame> items;
     std::vector<int> item_stream;
-    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> batch_graphs;  // (host list, frames, launches)
+    // (lane, host list, frames, launches at level 0, level 1)
+    std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> batch_graphs;
 };
 
 namespace vsc {
@@ -198,13 +205,15 @@ int frame_alone(vsc_server* sv, Stream& st, long long c)
     return check_hip(hipEventRecord(st.ev_done[c % R_OUT], st.hs), "vsc_server_tick");
 }
 
-// The flow work of window c on the stream's pool stream, and its batch item.
-int frame_flows_batched(vsc_server* sv, Stream& st, long long c, BatchFrame* item)
+// The flow work of window c on the stream's pool stream, and its batch item
+// for the tick stream ts.
+int frame_flows_batched(vsc_server* sv, Stream& st, long long c, hipStream_t ts, BatchFrame* item)
 {
     const vsc_window w = window_of(st, c);
     int rc;
-    // the flows of window c overwrite the buffers window c-2 stabilized from
-    if ((rc = wait_window(st, c - 2, st.hs))) return rc;
+    // the flows of window c overwrite the buffers window c-2 stabilized from;
+    // the batch continues from window c-1 (perhaps on another tick stream)
+    if ((rc = wait_window(st, c - 2, st.hs)) || (rc = wait_window(st, c - 1, ts))) return rc;
     const int ph = (int)(c % N_PHASES);
     auto enqueue = [&]() { return pipeline_flows_part(st.pipe, &w, c, st.hs); };
     if (sv->cfg.use_graphs && c >= 3 && st.gf[ph]) {
@@ -217,45 +226,46 @@ int frame_flows_batched(vsc_server* sv, Stream& st, long long c, BatchFrame* ite
         return rc;
     }
     if (check_hip(hipEventRecord(st.ev_flow, st.hs), "vsc_server_tick") ||
-        check_hip(hipStreamWaitEvent(sv->ts, st.ev_flow, 0), "vsc_server_tick"))
+        check_hip(hipStreamWaitEvent(ts, st.ev_flow, 0), "vsc_server_tick"))
         return VSC_E_HIP;
     return pipeline_batch_item(st.pipe, &w, c, &st.params, st.out[c % R_OUT], item);
 }
 
-// The stabilization of this round's batch items on the tick stream.
-int run_batch(vsc_server* sv)
+// The stabilization of this round's batch items on tick lane `li`.
+int run_batch(vsc_server* sv, int li)
 {
+    vsc_server::TickLane& lane = sv->lanes[li];
     const int n = (int)sv->items.size();
     const int fma = sv->cfg.form.solver ? 1 : 0;
-    const int b = sv->wl_next;
-    sv->wl_next ^= 1;
+    const int b = lane.wl_next;
+    lane.wl_next ^= 1;
     // the host list is refilled only after its previous copy has run
-    if (sv->wl_used[b] && check_hip(hipEventSynchronize(sv->ev_wl[b]), "vsc_server_tick")) return VSC_E_HIP;
-    batch_build(sv->wl_host[b], n, sv->items.data(), n, fma);
-    const BatchHeader* hd = reinterpret_cast<const BatchHeader*>(sv->wl_host[b]);
+    if (lane.wl_used[b] && check_hip(hipEventSynchronize(lane.ev_wl[b]), "vsc_server_tick")) return VSC_E_HIP;
+    batch_build(lane.wl_host[b], n, sv->items.data(), n, fma);
+    const BatchHeader* hd = reinterpret_cast<const BatchHeader*>(lane.wl_host[b]);
     const int nl[2] = {hd->n_launch[0], hd->n_launch[1]};
     const size_t bytes = batch_worklist_bytes(n);
     auto enqueue = [&]() {
-        if (check_hip(hipMemcpyAsync(sv->wl_dev, sv->wl_host[b], bytes, hipMemcpyHostToDevice, sv->ts),
+        if (check_hip(hipMemcpyAsync(lane.wl_dev, lane.wl_host[b], bytes, hipMemcpyHostToDevice, lane.ts),
                       "vsc_server_tick work list"))
             return VSC_E_HIP;
-        return launch_batch(sv->wl_dev, nl, fma, sv->grid, sv->ts);
+        return launch_batch(lane.wl_dev, nl, fma, sv->grid, lane.ts);
     };
     int rc;
     if (sv->cfg.use_graphs) {
-        const auto key = std::make_tuple(b, n, nl[0], nl[1]);
+        const auto key = std::make_tuple(li, b, n, nl[0], nl[1]);
         auto it = sv->batch_graphs.find(key);
         if (it == sv->batch_graphs.end()) {
             hipGraphExec_t ge = nullptr;
-            if ((rc = capture(sv->ts, &ge, enqueue))) return rc;
+            if ((rc = capture(lane.ts, &ge, enqueue))) return rc;
             it = sv->batch_graphs.emplace(key, ge).first;
         }
-        if (check_hip(hipGraphLaunch(it->second, sv->ts), "vsc_server_tick batch launch")) return VSC_E_HIP;
+        if (check_hip(hipGraphLaunch(it->second, lane.ts), "vsc_server_tick batch launch")) return VSC_E_HIP;
     } else if ((rc = enqueue())) {
         return rc;
     }
-    if (check_hip(hipEventRecord(sv->ev_wl[b], sv->ts), "vsc_server_tick")) return VSC_E_HIP;
-    sv->wl_used[b] = true;
+    if (check_hip(hipEventRecord(lane.ev_wl[b], lane.ts), "vsc_server_tick")) return VSC_E_HIP;
+    lane.wl_used[b] = true;
     return VSC_OK;
 }
 
@@ -323,12 +333,19 @@ extern "C" int vsc_server_create(const vsc_server_cfg* cfg, const float* upsampl
             return fail("device query failed");
         sv->grid = cus >= 8 ? cus / 8 * 8 : 8;
         const size_t wl = batch_worklist_bytes(cfg->max_streams);
-        if (hipStreamCreateWithFlags(&sv->ts, hipStreamNonBlocking) != hipSuccess) return fail("stream creation failed");
-        if (hipMalloc(&sv->wl_dev, wl) != hipSuccess) return fail("work list allocation failed");
-        for (int i = 0; i < 2; ++i)
-            if (hipHostMalloc(&sv->wl_host[i], wl, hipHostMallocDefault) != hipSuccess ||
-                hipEventCreateWithFlags(&sv->ev_wl[i], hipEventDisableTiming) != hipSuccess)
-                return fail("work list allocation failed");
+        // two tick streams: a batch may start while the previous one runs
+        // when their streams differ (a stream's frames stay in order through
+        // the wait on its previous window)
+        sv->lanes.resize(2);
+        for (vsc_server::TickLane& lane : sv->lanes) {
+            if (hipStreamCreateWithFlags(&lane.ts, hipStreamNonBlocking) != hipSuccess)
+                return fail("stream creation failed");
+            if (hipMalloc(&lane.wl_dev, wl) != hipSuccess) return fail("work list allocation failed");
+            for (int i = 0; i < 2; ++i)
+                if (hipHostMalloc(&lane.wl_host[i], wl, hipHostMallocDefault) != hipSuccess ||
+                    hipEventCreateWithFlags(&lane.ev_wl[i], hipEventDisableTiming) != hipSuccess)
+                    return fail("work list allocation failed");
+        }
         sv->items.reserve(cfg->max_streams);
         sv->item_stream.reserve(cfg->max_streams);
     }
@@ -340,16 +357,19 @@ extern "C" int vsc_server_destroy(vsc_server* sv)
 {
     if (!sv) return VSC_OK;
     for (hipStream_t h : sv->pool) (void)hipStreamSynchronize(h);
-    if (sv->ts) (void)hipStreamSynchronize(sv->ts);
+    for (vsc_server::TickLane& lane : sv->lanes)
+        if (lane.ts) (void)hipStreamSynchronize(lane.ts);
     for (Stream& st : sv->s)
         if (st.open) release(st);
     for (auto& kv : sv->batch_graphs) (void)hipGraphExecDestroy(kv.second);
-    for (int i = 0; i < 2; ++i) {
-        if (sv->wl_host[i]) (void)hipHostFree(sv->wl_host[i]);
-        if (sv->ev_wl[i]) (void)hipEventDestroy(sv->ev_wl[i]);
+    for (vsc_server::TickLane& lane : sv->lanes) {
+        for (int i = 0; i < 2; ++i) {
+            if (lane.wl_host[i]) (void)hipHostFree(lane.wl_host[i]);
+            if (lane.ev_wl[i]) (void)hipEventDestroy(lane.ev_wl[i]);
+        }
+        if (lane.wl_dev) (void)hipFree(lane.wl_dev);
+        if (lane.ts) (void)hipStreamDestroy(lane.ts);
     }
-    if (sv->wl_dev) (void)hipFree(sv->wl_dev);
-    if (sv->ts) (void)hipStreamDestroy(sv->ts);
     for (hipStream_t h : sv->pool) (void)hipStreamDestroy(h);
     delete sv;
     return VSC_OK;
@@ -423,7 +443,8 @@ extern "C" int vsc_server_close(vsc_server* sv, int id)
     Stream* st = get(sv, id);
     if (!st) return set_error(VSC_E_INVALID, "vsc_server_close: no such stream");
     (void)hipStreamSynchronize(st->hs);
-    if (sv->ts) (void)hipStreamSynchronize(sv->ts);
+    for (vsc_server::TickLane& lane : sv->lanes)
+        if (lane.ts) (void)hipStreamSynchronize(lane.ts);
     sv->used -= st->bytes;
     release(*st);
     return VSC_OK;
@@ -490,6 +511,8 @@ extern "C" int vsc_server_tick(vsc_server* sv, int* launched_out)
         more = false;
         sv->items.clear();
         sv->item_stream.clear();
+        const int li = sv->lanes.empty() ? 0 : sv->lane_next;
+        hipStream_t ts = sv->lanes.empty() ? nullptr : sv->lanes[li].ts;
         for (int id = 0; id < (int)sv->s.size(); ++id) {
             Stream& st = sv->s[id];
             if (failed[id] || !has_window(st)) continue;
@@ -497,7 +520,7 @@ extern "C" int vsc_server_tick(vsc_server* sv, int* launched_out)
             int rc;
             if (sv->cfg.batch && pipeline_batch_check(st.pipe, &st.params) == VSC_OK) {
                 sv->items.emplace_back();
-                rc = frame_flows_batched(sv, st, c, &sv->items.back());
+                rc = frame_flows_batched(sv, st, c, ts, &sv->items.back());
                 if (rc) {
                     sv->items.pop_back();
                 } else {
@@ -516,12 +539,13 @@ extern "C" int vsc_server_tick(vsc_server* sv, int* launched_out)
             }
         }
         if (!sv->items.empty()) {
-            const int rc = run_batch(sv);
+            const int rc = run_batch(sv, li);
             if (rc) return rc;   // the device is in an unknown state: report at once
+            sv->lane_next = (li + 1) % (int)sv->lanes.size();
             for (int id : sv->item_stream) {
                 Stream& st = sv->s[id];
                 const long long c = st.done_to + 1;
-                if (check_hip(hipEventRecord(st.ev_done[c % R_OUT], sv->ts), "vsc_server_tick")) return VSC_E_HIP;
+                if (check_hip(hipEventRecord(st.ev_done[c % R_OUT], ts), "vsc_server_tick")) return VSC_E_HIP;
                 launched(st, c);
                 ++n;
             }
@@ -566,7 +590,8 @@ extern "C" int vsc_server_drain(vsc_server* sv)
     if (!sv) return set_error(VSC_E_INVALID, "vsc_server_drain: NULL server");
     for (hipStream_t h : sv->pool)
         if (check_hip(hipStreamSynchronize(h), "vsc_server_drain")) return VSC_E_HIP;
-    if (sv->ts && check_hip(hipStreamSynchronize(sv->ts), "vsc_server_drain")) return VSC_E_HIP;
+    for (vsc_server::TickLane& lane : sv->lanes)
+        if (check_hip(hipStreamSynchronize(lane.ts), "vsc_server_drain")) return VSC_E_HIP;
     return VSC_OK;
 }
 
