@@ -458,6 +458,14 @@ def bench_pipeline(args):
     el_p, _ = run(plain, plain_steps, max(3, args.warmup // 4), 1)
     plain.close()
     fps_plain = plain_steps / el_p
+    # the chosen form's flow path with an FP32 network (FP32 convolutions and cost
+    # volume at FLOWDOWNSCALE 2, learned upsampler): the speed-up without 16-bit arithmetic
+    half32 = vsc.Pipeline(W, H, flow_dtype="f32", downscale=2, upsample="learned", solver="fma", seed=1,
+                          flow_head_scale=FLOW_HEAD_SCALE, overlap=True)
+    half32.set_last_rgba(P[2])
+    el_h, _ = run(half32, plain_steps, max(3, args.warmup // 4), 1)
+    half32.close()
+    fps_half32 = plain_steps / el_h
     # the plain form with the solver in the reference op order (bit-exact with the oracle)
     plain_x = vsc.Pipeline(W, H, form="plain", seed=1, flow_head_scale=FLOW_HEAD_SCALE, solver="exact",
                            overlap=True)
@@ -496,6 +504,11 @@ def bench_pipeline(args):
                        "ms_per_step": round(1000.0 * el_p / plain_steps, 4),
                        "form": "FP32 PWCNet-light at full resolution, FP32 stabilization (FMA-form solver)"},
         "speedup_vs_plain": round(fps / fps_plain, 3),
+        "fp32_network_half_resolution_form": {
+            "value": round(fps_half32, 2), "unit": "frames/s", "steps": plain_steps,
+            "form": "FP32 PWCNet-light at FLOWDOWNSCALE=2 (FP32 convolutions and cost volume), learned FP16 "
+                    "upsampler, FP32 stabilization (FMA-form solver)",
+            "speedup_vs_plain": round(fps_half32 / fps_plain, 3)},
         "plain_form_exact_solver": {"value": round(fps_exact, 2), "unit": "frames/s", "steps": plain_steps,
                                     "ms_per_step": round(1000.0 * el_x / plain_steps, 4),
                                     "form": "the plain form with the solver in the reference op order "

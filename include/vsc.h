@@ -104,9 +104,13 @@ int vsc_correlation_forward(const void* f1, const void* f2, void* out, int N, in
                             vsc_hip_stream stream);
 /* The same op with a caller-owned workspace (no allocation in the call; the
  * reference allocates and frees its channels-last temporaries on every call,
- * correlation_cuda.cu:365-370,440-441).  BF16 / F16 / FP8E4M3 inputs are
- * copied channels-last into the workspace (two tensors of N*H*W*ceil32(C)
- * elements); F32 and F64 need none (size 0, workspace may be NULL).
+ * correlation_cuda.cu:365-370,440-441).  BF16 / F16 / FP8E4M3 inputs whose
+ * width is a multiple of 4 and whose pointers are 8-byte (FP8: 4-byte)
+ * aligned are read in their NCHW layout by the MFMA kernel and need no
+ * workspace (NULL is accepted); other reduced-precision inputs are copied
+ * channels-last into the workspace (two tensors of N*H*W*ceil32(C)
+ * elements), whose size vsc_correlation_workspace_size gives from the shape
+ * alone.  F32 and F64 need none (size 0, workspace may be NULL).
  * vsc_correlation_forward above allocates that workspace stream-ordered
  * (hipMallocAsync / hipFreeAsync) when the type needs it.
  * Errors: as vsc_correlation_forward, and VSC_E_INVALID when the workspace is
@@ -115,9 +119,10 @@ size_t vsc_correlation_workspace_size(int N, int C, int H, int W, int in_dtype);
 /* No counterpart in the reference: the Correlation op for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  Same arguments, results and errors as
- * vsc_correlation_forward; the reduced-precision types use a static device
- * scratch of VSC_CORRELATION_STATIC_SCRATCH bytes per GPU (held until the
- * process ends) instead of an allocation: operands needing more
+ * vsc_correlation_forward; reduced-precision inputs that need channels-last
+ * copies (width not a multiple of 4, or unaligned pointers) use a static
+ * device scratch of VSC_CORRELATION_STATIC_SCRATCH bytes per GPU (held until
+ * the process ends) instead of an allocation: operands needing more
  * (vsc_correlation_workspace_size) are refused with VSC_E_INVALID before any
  * GPU work.  Calls that overlap on one GPU (two host threads or two streams)
  * race on that scratch; serialise them.  Nothing is allocated or freed.    */

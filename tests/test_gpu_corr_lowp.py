@@ -64,7 +64,8 @@ def test_corr_16bit_levels(vsc, orc, cuda, kind, C, H, W):
 
 
 @pytest.mark.parametrize("md", [0, 1, 3, 4])
-@pytest.mark.parametrize("shape", [(2, 5, 7, 9), (1, 40, 1, 33), (1, 33, 21, 1), (1, 70, 13, 50)])
+@pytest.mark.parametrize("shape", [(2, 5, 7, 9), (1, 40, 1, 33), (1, 33, 21, 1), (1, 70, 13, 50),
+                                   (2, 5, 7, 12), (1, 70, 13, 52), (1, 33, 3, 4)])
 def test_corr_bf16_shapes_and_legacy(vsc, orc, cuda, md, shape):
     rng = np.random.default_rng(md + sum(shape))
     f1 = rng.standard_normal(shape).astype(np.float32)
@@ -159,3 +160,40 @@ def test_corr_workspace_form_equals_allocating_form(vsc, cuda, kind):
             got = vsc.correlation(ca, cb, 4, 0, workspace=ws)
         torch.cuda.synchronize()
         assert torch.equal(ref, got), (kind, C, H, W)
+
+
+@pytest.mark.parametrize("kind", ["bf16", "f16", "fp8"])
+@pytest.mark.parametrize("shape", [(1, 32, 272, 480), (2, 70, 13, 52), (1, 196, 17, 28)])
+def test_corr_nchw_staging_equals_channels_last_copy(vsc, cuda, kind, shape):
+    """Widths that are multiples of 4 with aligned operands are read in their
+    NCHW layout by the MFMA kernel; misaligned operands take the channels-last
+    copy.  Both stage the same LDS tiles, so the outputs are the same bits
+    (normal and legacy)."""
+    import torch
+    rng = np.random.default_rng(sum(shape))
+    f1 = _t(rng.standard_normal(shape).astype(np.float32), cuda)
+    f2 = _t(rng.standard_normal(shape).astype(np.float32), cuda)
+    sc = None
+    if kind == "fp8":
+        a, b, sc = vsc.quantize_fp8_pair(f1, f2)
+    else:
+        a, b = vsc.convert(f1, kind), vsc.convert(f2, kind)
+
+    def misaligned(x):
+        # the same values one element past an aligned boundary (pointer not 4/8-byte aligned)
+        buf = torch.empty(x.numel() + 1, dtype=x.dtype, device=cuda)
+        v = buf[1:].view(x.shape)
+        v.copy_(x)
+        return v
+
+    ma, mb = misaligned(a), misaligned(b)
+    assert ma.data_ptr() % 4 != 0
+    for legacy in (0, 1):
+        if kind == "fp8":
+            direct = vsc.correlation_fp8(a, b, sc, 4, legacy)
+            copy = vsc.correlation_fp8(ma, mb, sc, 4, legacy)
+        else:
+            direct = vsc.correlation(a, b, 4, legacy)
+            copy = vsc.correlation(ma, mb, 4, legacy)
+        torch.cuda.synchronize()
+        assert torch.equal(direct, copy), (kind, shape, legacy)

@@ -485,7 +485,8 @@ def _dtype_code(t):
 
 
 def correlation_workspace_bytes(shape, dtype_code):
-    """Device bytes vsc_correlation_forward_ws needs for NCHW `shape`."""
+    """Device bytes vsc_correlation_forward_ws needs for NCHW `shape` (an upper
+    bound: operands read in their NCHW layout need none)."""
     N, C, H, W = shape
     return int(lib().vsc_correlation_workspace_size(N, C, H, W, int(dtype_code)))
 

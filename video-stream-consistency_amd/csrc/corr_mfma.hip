@@ -59,7 +59,31 @@ __device__ inline f32x4 cm_mfma(const void* a, const void* b, f32x4 acc)
     }
 }
 
-template <int KIND, int MD, bool LEGACY>
+// One 8-byte LDS vector (GC = 8/EB channels of one pixel) for each of 4
+// consecutive pixels, from GC channel-plane loads of those 4 pixels (NCHW):
+// a register transpose with byte permutes.  raw[k] holds pixels 0..3 of
+// channel k: 2 x 32 bits (16-bit elements) or 32 bits (FP8).
+__device__ inline uint2 nchw_px16(const uint2 (&raw)[4], int j)
+{
+    const uint32_t sel = (j & 1) ? 0x07060302u : 0x05040100u;
+    const bool hi = j >= 2;
+    return make_uint2(__builtin_amdgcn_perm(hi ? raw[1].y : raw[1].x, hi ? raw[0].y : raw[0].x, sel),
+                      __builtin_amdgcn_perm(hi ? raw[3].y : raw[3].x, hi ? raw[2].y : raw[2].x, sel));
+}
+__device__ inline uint2 nchw_px8(const uint32_t (&raw)[8], int j)
+{
+    const uint32_t sel = (uint32_t)j | ((uint32_t)(4 + j) << 8);
+    const uint32_t t01 = __builtin_amdgcn_perm(raw[1], raw[0], sel), t23 = __builtin_amdgcn_perm(raw[3], raw[2], sel);
+    const uint32_t t45 = __builtin_amdgcn_perm(raw[5], raw[4], sel), t67 = __builtin_amdgcn_perm(raw[7], raw[6], sel);
+    return make_uint2(__builtin_amdgcn_perm(t23, t01, 0x05040100u), __builtin_amdgcn_perm(t67, t45, 0x05040100u));
+}
+
+// NCHW: the operands are read in their NCHW layout (width a multiple of 4,
+// 8-byte (16-bit) or 4-byte (FP8) aligned): each staging unit loads 4 pixels
+// of GC channel planes and writes them channels-last into LDS, so no
+// conversion pass runs before the kernel.  Otherwise the operands are
+// channels-last copies [n][y][x][CP].
+template <int KIND, int MD, bool LEGACY, bool NCHW>
 __global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v, const void* __restrict__ f2v,
                                                    float* __restrict__ out, int C, int CP, int H, int W,
                                                    const float* scale)
@@ -101,33 +125,87 @@ __global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v,
 #pragma unroll
         for (int t = 0; t < 2; ++t) acc[d][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    constexpr int NA = CM_ROWS * CM_PX * VPP, NB = BR * BC * VPP, NU = NA + NB, RU = (NU + 511) / 512;
-    uint2 rg[RU];
-    auto fetch = [&](int c0) {
+    // NCHW staging units: (channel group of GC, row, group of 4 pixels)
+    constexpr int GC = 8 / EB, CG = CM_KC / GC;
+    constexpr int NA4 = CG * CM_ROWS * (CM_PX / 4), NB4 = CG * BR * (BC / 4), NU4 = NA4 + NB4;
+    constexpr int RU4 = (NU4 + 511) / 512;
+    using Raw = typename std::conditional<EB == 2, uint2, uint32_t>::type;
+    Raw raw[NCHW ? RU4 : 1][GC];
+    auto fetch4 = [&](int c0) {
+        const E* fa = f1 + (long long)n * C * HW;
+        const E* fb = f2 + (long long)n * C * HW;
 #pragma unroll
-        for (int k = 0; k < RU; ++k) {
+        for (int k = 0; k < RU4; ++k) {
             const int u = tid + 512 * k;
-            uint2 v = make_uint2(0u, 0u);
-            if (u < NA) {
-                const int p = u / VPP, cv = u % VPP, r = p / CM_PX, x = p % CM_PX;
-                const int gh = h0 + r, gw = w0 + x;
-                if (gh < H && gw < W)
-                    v = *reinterpret_cast<const uint2*>(a + ((long long)gh * W + gw) * CP + c0 + cv * (8 / EB));
-            } else if (u < NU) {
-                const int ub = u - NA, p = ub / VPP, cv = ub % VPP, r = p / BC, x = p % BC;
-                const int gh = h0 - CM_HALO + r, gw = w0 - CM_HALO + x;
-                if (gh >= 0 && gh < H && gw >= 0 && gw < W)
-                    v = *reinterpret_cast<const uint2*>(b + ((long long)gh * W + gw) * CP + c0 + cv * (8 / EB));
+            const bool inA = u < NA4;
+            const int ul = inA ? u : u - NA4;
+            const int rows = inA ? CM_ROWS : BR, groups = inA ? CM_PX / 4 : BC / 4;
+            const int g = ul % groups, r = (ul / groups) % rows, cg = ul / (groups * rows);
+            const int gh = inA ? h0 + r : h0 - CM_HALO + r;
+            const int gw = inA ? w0 + 4 * g : w0 - CM_HALO + 4 * g;
+            const bool ok = u < NU4 && gh >= 0 && gh < H && gw >= 0 && gw < W;
+            const E* src = (inA ? fa : fb) + (long long)gh * W + gw;
+#pragma unroll
+            for (int j = 0; j < GC; ++j) {
+                const int c = c0 + cg * GC + j;
+                Raw v{};
+                if (ok && c < C) v = *reinterpret_cast<const Raw*>(src + (long long)c * HW);
+                raw[k][j] = v;
             }
-            rg[k] = v;
+        }
+    };
+    auto stage4 = [&]() {
+#pragma unroll
+        for (int k = 0; k < RU4; ++k) {
+            const int u = tid + 512 * k;
+            if (u >= NU4) continue;
+            const bool inA = u < NA4;
+            const int ul = inA ? u : u - NA4;
+            const int rows = inA ? CM_ROWS : BR, groups = inA ? CM_PX / 4 : BC / 4;
+            const int g = ul % groups, r = (ul / groups) % rows, cg = ul / (groups * rows);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                uint2 v;
+                if constexpr (EB == 2) v = nchw_px16(raw[k], j);
+                else v = nchw_px8(raw[k], j);
+                E* dst = inA ? &sA[r][4 * g + j][cg * GC] : &sB[r][4 * g + j][cg * GC];
+                *reinterpret_cast<uint2*>(dst) = v;
+            }
+        }
+    };
+
+    constexpr int NA = CM_ROWS * CM_PX * VPP, NB = BR * BC * VPP, NU = NA + NB, RU = (NU + 511) / 512;
+    uint2 rg[NCHW ? 1 : RU];
+    auto fetch = [&](int c0) {
+        if constexpr (NCHW) {
+            fetch4(c0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < RU; ++k) {
+                const int u = tid + 512 * k;
+                uint2 v = make_uint2(0u, 0u);
+                if (u < NA) {
+                    const int p = u / VPP, cv = u % VPP, r = p / CM_PX, x = p % CM_PX;
+                    const int gh = h0 + r, gw = w0 + x;
+                    if (gh < H && gw < W)
+                        v = *reinterpret_cast<const uint2*>(a + ((long long)gh * W + gw) * CP + c0 + cv * (8 / EB));
+                } else if (u < NU) {
+                    const int ub = u - NA, p = ub / VPP, cv = ub % VPP, r = p / BC, x = p % BC;
+                    const int gh = h0 - CM_HALO + r, gw = w0 - CM_HALO + x;
+                    if (gh >= 0 && gh < H && gw >= 0 && gw < W)
+                        v = *reinterpret_cast<const uint2*>(b + ((long long)gh * W + gw) * CP + c0 + cv * (8 / EB));
+                }
+                rg[k] = v;
+            }
         }
     };
 
     fetch(0);
     for (int c0 = 0; c0 < CP; c0 += CM_KC) {
         __syncthreads();
+        if constexpr (NCHW) stage4();
 #pragma unroll
-        for (int k = 0; k < RU; ++k) {
+        for (int k = 0; k < (NCHW ? 0 : RU); ++k) {
             const int u = tid + 512 * k;
             if (u < NA) {
                 const int p = u / VPP, cv = u % VPP;
@@ -185,25 +263,29 @@ __global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v,
 
 template <int KIND, int MD>
 static void launch_mfma(const void* f1, const void* f2, float* out, int N, int C, int CP, int H, int W, bool legacy,
-                        const float* scale, hipStream_t s)
+                        bool nchw, const float* scale, hipStream_t s)
 {
     dim3 grid((W + CM_PX - 1) / CM_PX, (H + CM_ROWS - 1) / CM_ROWS, N);
-    if (legacy)
-        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, true>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
+    if (legacy && nchw)
+        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, true, true>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
+    else if (legacy)
+        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, true, false>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
+    else if (nchw)
+        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, false, true>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
     else
-        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, false>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
+        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, false, false>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
 }
 
 template <int KIND>
 static int dispatch_md(const void* f1, const void* f2, float* out, int N, int C, int CP, int H, int W, int md,
-                       bool legacy, const float* scale, hipStream_t s)
+                       bool legacy, bool nchw, const float* scale, hipStream_t s)
 {
     switch (md) {
-        case 0: launch_mfma<KIND, 0>(f1, f2, out, N, C, CP, H, W, legacy, scale, s); break;
-        case 1: launch_mfma<KIND, 1>(f1, f2, out, N, C, CP, H, W, legacy, scale, s); break;
-        case 2: launch_mfma<KIND, 2>(f1, f2, out, N, C, CP, H, W, legacy, scale, s); break;
-        case 3: launch_mfma<KIND, 3>(f1, f2, out, N, C, CP, H, W, legacy, scale, s); break;
-        case 4: launch_mfma<KIND, 4>(f1, f2, out, N, C, CP, H, W, legacy, scale, s); break;
+        case 0: launch_mfma<KIND, 0>(f1, f2, out, N, C, CP, H, W, legacy, nchw, scale, s); break;
+        case 1: launch_mfma<KIND, 1>(f1, f2, out, N, C, CP, H, W, legacy, nchw, scale, s); break;
+        case 2: launch_mfma<KIND, 2>(f1, f2, out, N, C, CP, H, W, legacy, nchw, scale, s); break;
+        case 3: launch_mfma<KIND, 3>(f1, f2, out, N, C, CP, H, W, legacy, nchw, scale, s); break;
+        case 4: launch_mfma<KIND, 4>(f1, f2, out, N, C, CP, H, W, legacy, nchw, scale, s); break;
         default:
             return set_error(VSC_E_UNSUPPORTED,
                              "vsc_correlation_forward: reduced-precision correlation supports max_displacement <= 4");
@@ -293,8 +375,17 @@ size_t corr_mfma_workspace_bytes(int N, int C, int H, int W, int in_dtype)
     return 2 * one;
 }
 
+// widths that are multiples of 4 with aligned operands: the kernel reads the
+// NCHW operands itself, with no channels-last copy and no workspace
+bool corr_mfma_reads_nchw(const void* f1, const void* f2, int W, int in_dtype)
+{
+    const uintptr_t align = in_dtype == VSC_FP8E4M3 ? 4 : 8;
+    return W % 4 == 0 && reinterpret_cast<uintptr_t>(f1) % align == 0 && reinterpret_cast<uintptr_t>(f2) % align == 0;
+}
+
 // workspace: corr_mfma_workspace_bytes() bytes of device memory, or NULL to
-// allocate it stream-ordered for this call
+// allocate it stream-ordered for this call (unused when the kernel reads the
+// NCHW operands directly)
 int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
                        int in_dtype, const float* chan_scale, void* workspace, hipStream_t s)
 {
@@ -306,6 +397,11 @@ int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C,
     const int eb = in_dtype == VSC_FP8E4M3 ? 1 : 2;
     const int CP = (C + CM_KC - 1) / CM_KC * CM_KC;
     const long long HW = (long long)H * W;
+    if (corr_mfma_reads_nchw(f1, f2, W, in_dtype)) {
+        if (in_dtype == VSC_BF16) return dispatch_md<CM_BF16>(f1, f2, out, N, C, CP, H, W, md, legacy, true, nullptr, s);
+        if (in_dtype == VSC_F16) return dispatch_md<CM_F16>(f1, f2, out, N, C, CP, H, W, md, legacy, true, nullptr, s);
+        return dispatch_md<CM_FP8>(f1, f2, out, N, C, CP, H, W, md, legacy, true, chan_scale, s);
+    }
     const size_t bytes = corr_mfma_workspace_bytes(N, C, H, W, in_dtype);
     void* ws = workspace;
     int rc;
@@ -321,9 +417,9 @@ int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C,
     }
     rc = check_launch("correlation to channels-last");
     if (!rc) {
-        if (in_dtype == VSC_BF16) rc = dispatch_md<CM_BF16>(t1, t2, out, N, C, CP, H, W, md, legacy, nullptr, s);
-        else if (in_dtype == VSC_F16) rc = dispatch_md<CM_F16>(t1, t2, out, N, C, CP, H, W, md, legacy, nullptr, s);
-        else rc = dispatch_md<CM_FP8>(t1, t2, out, N, C, CP, H, W, md, legacy, chan_scale, s);
+        if (in_dtype == VSC_BF16) rc = dispatch_md<CM_BF16>(t1, t2, out, N, C, CP, H, W, md, legacy, false, nullptr, s);
+        else if (in_dtype == VSC_F16) rc = dispatch_md<CM_F16>(t1, t2, out, N, C, CP, H, W, md, legacy, false, nullptr, s);
+        else rc = dispatch_md<CM_FP8>(t1, t2, out, N, C, CP, H, W, md, legacy, false, chan_scale, s);
     }
     if (workspace) return rc;
     const int rc2 = check_hip(hipFreeAsync(ws, s), "vsc_correlation_forward workspace free");

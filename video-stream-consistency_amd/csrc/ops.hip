@@ -342,7 +342,9 @@ extern "C" int vsc_correlation_forward_noalloc(const void* f1, const void* f2, v
                                                int max_displacement, int legacy, int in_dtype, const float* chan_scale,
                                                vsc_hip_stream stream)
 {
-    const size_t need = (N < 1 || C < 1 || H < 1 || W < 1) ? 0 : corr_mfma_workspace_bytes(N, C, H, W, in_dtype);
+    const size_t need = (N < 1 || C < 1 || H < 1 || W < 1 || corr_mfma_reads_nchw(f1, f2, W, in_dtype))
+                            ? 0
+                            : corr_mfma_workspace_bytes(N, C, H, W, in_dtype);
     if (need > VSC_CORRELATION_STATIC_SCRATCH)
         return set_error(VSC_E_INVALID, "vsc_correlation_forward_noalloc: the operands need " + std::to_string(need) +
                                             " bytes of scratch, more than the static " +
@@ -365,7 +367,8 @@ extern "C" int vsc_correlation_forward_ws(const void* f1, const void* f2, void* 
                                           int max_displacement, int legacy, int in_dtype, const float* chan_scale,
                                           void* workspace, size_t workspace_bytes, vsc_hip_stream stream)
 {
-    const size_t need = vsc_correlation_workspace_size(N, C, H, W, in_dtype);
+    const size_t need =
+        corr_mfma_reads_nchw(f1, f2, W, in_dtype) ? 0 : vsc_correlation_workspace_size(N, C, H, W, in_dtype);
     if (need && (!workspace || workspace_bytes < need))
         return set_error(VSC_E_INVALID, "vsc_correlation_forward_ws: workspace missing or smaller than "
                                         "vsc_correlation_workspace_size()");

@@ -407,6 +407,7 @@ int stream_batch_frame(vsc_stream* s, const vsc_frame_in* in, const vsc_params* 
 
 // reduced-precision (BF16 / F16 / FP8 e4m3) correlation on MFMA, corr_mfma.hip
 size_t corr_mfma_workspace_bytes(int N, int C, int H, int W, int in_dtype);
+bool corr_mfma_reads_nchw(const void* f1, const void* f2, int W, int in_dtype);
 int corr_mfma_dispatch(const void* f1, const void* f2, float* out, int N, int C, int H, int W, int md, int legacy,
                        int in_dtype, const float* chan_scale, void* workspace, hipStream_t s);
 
