@@ -101,8 +101,8 @@ struct vsc_server {
     int grid = 256;                           // blocks of a persistent launch (one per CU)
     std::vector<vsc::BatchFrame> items;
     std::vector<int> item_stream;
-    // (lane, host list, frames, launches at level 0, level 1)
-    std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> batch_graphs;
+    // (lane, host list, launches at level 0, level 1)
+    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> batch_graphs;
 };
 
 namespace vsc {
@@ -241,10 +241,14 @@ int run_batch(vsc_server* sv, int li)
     lane.wl_next ^= 1;
     // the host list is refilled only after its previous copy has run
     if (lane.wl_used[b] && check_hip(hipEventSynchronize(lane.ev_wl[b]), "vsc_server_tick")) return VSC_E_HIP;
-    batch_build(lane.wl_host[b], n, sv->items.data(), n, fma);
+    // the layout has room for every stream, so a captured graph depends only
+    // on the lane, the host buffer and the launch counts, not on how many
+    // streams this round batches
+    const int cap = sv->cfg.max_streams;
+    batch_build(lane.wl_host[b], cap, sv->items.data(), n, fma);
     const BatchHeader* hd = reinterpret_cast<const BatchHeader*>(lane.wl_host[b]);
     const int nl[2] = {hd->n_launch[0], hd->n_launch[1]};
-    const size_t bytes = batch_worklist_bytes(n);
+    const size_t bytes = batch_worklist_bytes(cap);
     auto enqueue = [&]() {
         if (check_hip(hipMemcpyAsync(lane.wl_dev, lane.wl_host[b], bytes, hipMemcpyHostToDevice, lane.ts),
                       "vsc_server_tick work list"))
@@ -253,7 +257,7 @@ int run_batch(vsc_server* sv, int li)
     };
     int rc;
     if (sv->cfg.use_graphs) {
-        const auto key = std::make_tuple(li, b, n, nl[0], nl[1]);
+        const auto key = std::make_tuple(li, b, nl[0], nl[1]);
         auto it = sv->batch_graphs.find(key);
         if (it == sv->batch_graphs.end()) {
             hipGraphExec_t ge = nullptr;

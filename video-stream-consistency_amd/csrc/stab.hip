@@ -286,11 +286,12 @@ int launch_consist_sweep(View P, View A, View Wt, View Oin, View Oout, View U, f
 //
 // Neighbour exchange goes through LDS, stored column-major: the R values of
 // (strip, column) are contiguous with a padded stride RS = R+4 (RS/4 odd, so a
-// 16-byte access of 8 consecutive lanes covers all 32 banks once), and each
-// sweep writes its strip with 16-byte stores, then reads the left and right
-// strips with 16-byte loads issued one 4-row chunk ahead of their use.  The
-// up/down neighbours come from the thread's own registers (LDS only at the
-// strip ends).  Two LDS planes alternate (one barrier per sweep).
+// 16-byte access of 8 consecutive lanes covers all 32 banks once).  A sweep
+// reads the left and right columns of one plane with 16-byte loads issued one
+// 4-row chunk ahead of their use and writes each chunk's new values into the
+// other plane with a 16-byte store as soon as they are computed; the planes
+// alternate, one barrier per sweep.  The up/down neighbours come from the
+// thread's own registers (LDS only at the strip ends).
 //
 // Arithmetic per pixel and sweep is that of k_consist_sweep; lapP (the
 // Laplacian of the processed frame, a constant of the level) is an input,
@@ -303,6 +304,9 @@ int launch_consist_sweep(View P, View A, View Wt, View Oin, View Oout, View U, f
 // image (never stored, never read by an image pixel) skip the arithmetic.
 // ---------------------------------------------------------------------------
 static constexpr int SB_TW = 128;
+static constexpr int SB_R = 28;      // rows per thread, reference op order (5 values per pixel)
+static constexpr int SB_R_FMA = 32;  // FMA form: 4 values per pixel in registers, taller strips
+static constexpr int SB_STRIPS = 4;
 
 // Tile placement along one axis of length L: tiles of T pixels whose outer
 // `h` pixels on a side facing another tile are halo.  When L >= T the first
@@ -354,8 +358,8 @@ enum StripEdge { SE_NONE = 0, SE_TOP = 1, SE_BOTTOM = 2, SE_ROWS = 3 };
 template <int R, bool MOM, int EDGE, bool REV>
 __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const float (&a)[R], const float (&w)[R],
                                             const float (&lp)[R], const float* colL, const float* colR,
-                                            float up_edge, float dn_edge, float step, float mom, int cntc, int rU,
-                                            int rD)
+                                            float* nxt, float up_edge, float dn_edge, float step, float mom, int cntc,
+                                            int rU, int rD)
 {
     constexpr int NK = R / 4;
     float4 l4 = *reinterpret_cast<const float4*>(colL + (REV ? R - 4 : 0));
@@ -397,6 +401,8 @@ __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const 
             u[r] = sg;
             prev = cur;
         }
+        // publish the chunk's new values for the next sweep
+        *reinterpret_cast<float4*>(nxt + 4 * k) = make_float4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
         l4 = ln;
         r4 = rn;
     }
@@ -409,7 +415,8 @@ __device__ __forceinline__ void sweep_strip(float (&o)[R], float (&u)[R], const 
 template <int R, bool MOM, int EDGE, bool REV>
 __device__ __forceinline__ void sweep_strip_fma(float (&o)[R], float (&u)[R], const float (&q)[R],
                                                 const float (&kk)[R], const float* colL, const float* colR,
-                                                float up_edge, float dn_edge, float step, float mom, int rU, int rD)
+                                                float* nxt, float up_edge, float dn_edge, float step, float mom, int rU,
+                                                int rD)
 {
     constexpr int NK = R / 4;
     float4 l4 = *reinterpret_cast<const float4*>(colL + (REV ? R - 4 : 0));
@@ -448,6 +455,8 @@ __device__ __forceinline__ void sweep_strip_fma(float (&o)[R], float (&u)[R], co
             u[r] = sg;
             prev = cur;
         }
+        // publish the chunk's new values for the next sweep
+        *reinterpret_cast<float4*>(nxt + 4 * k) = make_float4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
         l4 = ln;
         r4 = rn;
     }
@@ -546,34 +555,42 @@ __device__ __forceinline__ void consist_tile(
     const int rU0 = 1 - gys;
     const int rD0 = (H - 3) - gys;
 
-    // One sweep: publish this strip's O, then update it from the neighbours.
+    // One sweep: once every strip's values are in plane s&1, update this
+    // strip from its neighbours there, writing each 4-row chunk's new values
+    // into plane (s+1)&1 as soon as they are computed (the writes spread over
+    // the sweep instead of bunching before its barrier).
     auto sweep = [&](int s, auto mom_c, auto rev_c, auto edge_c) {
         constexpr bool MOM = decltype(mom_c)::value, REV = decltype(rev_c)::value;
         constexpr int EDGE = decltype(edge_c)::value;
         float (*buf)[TW + 1][RS] = sO[s & 1];
-        float* own = buf[strip][tx];
-#pragma unroll
-        for (int k = 0; k < R / 4; ++k)
-            *reinterpret_cast<float4*>(own + 4 * k) = make_float4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
         __syncthreads();
         const float up_edge = buf[stU][tx][R - 1];
         const float dn_edge = buf[stD][tx][0];
         const float* colL = buf[strip][txl];
         const float* colR = buf[strip][txr];
+        float* nxt = sO[(s + 1) & 1][strip][tx];
         int rU = rU0, rD = rD0;
         // opaque per sweep: keeps the compiler from hoisting 2*R row
         // predicates out of the sweep loop into scalar registers
         asm volatile("" : "+s"(rU), "+s"(rD));
         if constexpr (FMA)
-            sweep_strip_fma<R, MOM, EDGE, REV>(o, u, w, lp, colL, colR, up_edge, dn_edge, step, mom, rU, rD);
+            sweep_strip_fma<R, MOM, EDGE, REV>(o, u, w, lp, colL, colR, nxt, up_edge, dn_edge, step, mom, rU, rD);
         else
-            sweep_strip<R, MOM, EDGE, REV>(o, u, a, w, lp, colL, colR, up_edge, dn_edge, step, mom, cntc, rU, rD);
+            sweep_strip<R, MOM, EDGE, REV>(o, u, a, w, lp, colL, colR, nxt, up_edge, dn_edge, step, mom, cntc, rU,
+                                           rD);
     };
     // The level's first sweep has no momentum term; after it, ascending and
-    // descending sweeps alternate in pairs.
+    // descending sweeps alternate in pairs.  The initial values go to plane 0.
     auto run = [&](auto edge_c) {
         const std::true_type yes{};
         const std::false_type no{};
+        {
+            float* own = sO[0][strip][tx];
+#pragma unroll
+            for (int k = 0; k < R / 4; ++k)
+                *reinterpret_cast<float4*>(own + 4 * k) =
+                    make_float4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+        }
         int s = 0;
         if (first_sweep == 0) {
             sweep(0, no, no, edge_c);
@@ -679,10 +696,6 @@ int launch_fma_consts(float* L, const float* A, float* Wt, int W, int H, float s
                        H, step);
     return check_launch("fma_consts");
 }
-
-static constexpr int SB_R = 28;      // rows per thread, reference op order (5 values per pixel)
-static constexpr int SB_R_FMA = 32;  // FMA form: 4 values per pixel in registers, taller strips
-static constexpr int SB_STRIPS = 4;
 
 // Launch plan of one pyramid level: the halo (= the most sweeps per launch)
 // and the tile height, from a cost model of the kernel fitted on MI355X.  A
