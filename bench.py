@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=0, help="timed frames of the plain form (default steps/3)")
     ap.add_argument("--solver", choices=["fma", "exact"], default="fma",
                     help="stabilize workload: solver in FMA form or in the reference op order")
+    ap.add_argument("--flow-batch", type=int, default=1, choices=[1, 2, 3, 4],
+                    help="pipeline workload: frames whose flows one network decode batches (chosen form, "
+                         "frame-pipelined)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -407,18 +410,25 @@ def bench_pipeline(args):
         r = [(t - 1) % RING, t % RING, (t + 1) % RING]
         return (O[r[0]], O[r[1]], O[r[2]], P[r[0]], P[r[1]], P[r[2]])
 
-    def run(pipe, steps, warmup, t_start, prof=False):
-        for i in range(warmup):
-            pipe.frame(window(t_start + i), t_start + i, out_rgba=out)
+    def frames(pipe, t, n, B):
+        # frames t .. t+n-1: one call per frame, or per batch of B frames
+        if B == 1:
+            for i in range(n):
+                pipe.frame(window(t + i), t + i, out_rgba=out)
+            return
+        for i in range(0, n, B):
+            m = min(B, n - i)
+            pipe.frames([window(t + i + j) for j in range(m)], t + i, out_rgba=[out] * m)
+
+    def run(pipe, steps, warmup, t_start, prof=False, B=1):
+        frames(pipe, t_start, warmup, B)
         torch.cuda.synchronize()
         if prof:
             # the library brackets every solver and front-pass launch with HIP
             # events on the stream it launches on
             vsc.profile_begin(64 * steps)
         t0 = time.perf_counter()
-        for i in range(steps):
-            t = t_start + warmup + i
-            pipe.frame(window(t), t, out_rgba=out)
+        frames(pipe, t_start + warmup, steps, B)
         torch.cuda.synchronize()
         return time.perf_counter() - t0, t_start + warmup + steps
 
@@ -427,9 +437,10 @@ def bench_pipeline(args):
     # scatter every warp's gathers); the same network weights serve both forms
     # throughput: frame-pipelined pipelines (the flows of frame t+1 are computed
     # while frame t is stabilized; the timed region ends with a device sync)
-    chosen = vsc.Pipeline(W, H, form="chosen", seed=1, flow_head_scale=FLOW_HEAD_SCALE, overlap=True)
+    B = args.flow_batch
+    chosen = vsc.Pipeline(W, H, form="chosen", seed=1, flow_head_scale=FLOW_HEAD_SCALE, overlap=True, flow_batch=B)
     chosen.set_last_rgba(P[2])
-    el_c, _ = run(chosen, args.steps, args.warmup, 1, prof=True)
+    el_c, _ = run(chosen, args.steps, args.warmup, 1, prof=True, B=B)
     n_solver, ms_solver, by_solver = vsc.profile_end("consist_blocked")
     n_front, ms_front, by_front = vsc.profile_end("front_pass")
     gate, count = chosen.gate()
@@ -497,6 +508,7 @@ def bench_pipeline(args):
                            "fwd-bwd fallback, FP32 stabilization (FMA-form solver)",
                    "execution": "frame-pipelined on two HIP streams (flows of frame t+1 during the "
                                 "stabilization of frame t); every form measured the same way",
+                   "flow_batch": B,
                    "width": W, "height": H, "pyramid_levels": 2, "num_iter": 150},
         "chosen_serial": {"value": round(fps_serial, 2), "unit": "frames/s",
                           "form": "the chosen form on one stream (no frame pipelining)"},

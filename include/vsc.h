@@ -419,8 +419,9 @@ int vsc_flow_fb_gate(const float* fwd, const float* bwd, int width, int height, 
  * Per-stream frame pipeline: VideoStabilizer::doOneStep (videostabilizer.cpp:
  * 167-265) including retrieveOpticalFlow (:267-279) through the flow network,
  * FLOWDOWNSCALE (videostabilizer.cpp:74-94, flowmodel.cpp:126-133,158-161)
- * and the learned upsampler.  Owns a vsc_flownet (3 feature slots), the flow
- * buffers, an upsampler and a vsc_stream; a frame allocates nothing.        */
+ * and the learned upsampler.  Owns a vsc_flownet (flow_batch + 2 feature
+ * slots), the flow buffers, an upsampler and a vsc_stream; a frame allocates
+ * nothing.                                                                   */
 typedef struct vsc_pipeline vsc_pipeline;
 
 typedef struct {
@@ -439,6 +440,10 @@ typedef struct {
                                    outputs are then ordered on the caller's stream by the call for
                                    frame t+1 (or vsc_pipeline_sync), and the window frames of call t
                                    must stay unchanged until then.  Same results as 0. */
+    int flow_batch;             /* B, 1..4: the most frames whose flows vsc_pipeline_frames computes in
+                                   one batched network decode (the reference's batchSize: FlowModel::run
+                                   over B frame pairs every B frames, videostabilizer.cpp:65-71,177,
+                                   267-279).  Each flow has the same bits in any batch. */
 } vsc_pipeline_cfg;
 
 /* The k = 1 frame window (device RGBA8 frames, 4-byte aligned). */
@@ -466,6 +471,17 @@ int vsc_pipeline_flows(vsc_pipeline* p, const vsc_window* w, long long frame, fl
 /* One stabilized frame of window `frame` (flows, upsampling, vsc_stab_frame). */
 int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long long frame, const vsc_params* params,
                        uint8_t* out_rgba, float* out_float, vsc_hip_stream stream);
+/* n (1 .. flow_batch) consecutive stabilized frames: w[i] is the window of
+ * frame first + i (the reference's doOneStep calls for currentFrame .. +B-1,
+ * whose frame window holds 2k+B frames).  The flows of all n frames come from
+ * one batched decode of 2n pairs, then the frames are stabilized in order;
+ * the results equal n vsc_pipeline_frame calls.  out_rgba / out_float: n
+ * pointers each (the arrays, or their entries, may be NULL).  With overlap =
+ * 1 the outputs of the batch are ordered on `stream` by the next call (or
+ * vsc_pipeline_sync).  Errors: VSC_E_INVALID for n outside [1, flow_batch]
+ * or a NULL window, before any work. */
+int vsc_pipeline_frames(vsc_pipeline* p, const vsc_window* w, int n, long long first, const vsc_params* params,
+                        uint8_t* const* out_rgba, float* const* out_float, vsc_hip_stream stream);
 /* The last frame's fallback gate (1 learned, 0 bilinear) and inconsistent count. */
 int vsc_pipeline_gate(const vsc_pipeline* p, float* gate, uint32_t* count, vsc_hip_stream stream);
 /* Order `stream` after every frame submitted so far (needed only with overlap = 1). */

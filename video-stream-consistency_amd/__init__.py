@@ -54,7 +54,8 @@ EXPORTED = (
     "vsc_upsampler_set_params", "vsc_upsampler_workspace_size", "vsc_upsampler_run", "vsc_upsampler_train_step",
     "vsc_upsampler_loss_grad", "vsc_flow_fb_gate", "vsc_pipeline_default_cfg", "vsc_pipeline_create",
     "vsc_pipeline_destroy", "vsc_pipeline_footprint", "vsc_pipeline_set_upsampler_params", "vsc_pipeline_flownet",
-    "vsc_pipeline_stream", "vsc_pipeline_flows", "vsc_pipeline_frame", "vsc_pipeline_gate", "vsc_pipeline_sync",
+    "vsc_pipeline_stream", "vsc_pipeline_flows", "vsc_pipeline_frame", "vsc_pipeline_frames", "vsc_pipeline_gate",
+    "vsc_pipeline_sync",
     "vsc_server_default_cfg", "vsc_server_create", "vsc_server_destroy", "vsc_server_stream_footprint",
     "vsc_server_used_bytes", "vsc_server_open", "vsc_server_close", "vsc_server_set_params", "vsc_server_submit",
     "vsc_server_tick", "vsc_server_poll", "vsc_server_drain", "vsc_server_read", "vsc_flo_read", "vsc_flo_write",
@@ -90,7 +91,7 @@ class PipelineCfg(ctypes.Structure):
     _fields_ = [("flow_dtype", ctypes.c_int), ("downscale", ctypes.c_int), ("upsample", ctypes.c_int),
                 ("scale_vectors", ctypes.c_int), ("fb_max_inconsistent", ctypes.c_float),
                 ("ref_bwd_index", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("solver", ctypes.c_int),
-                ("overlap", ctypes.c_int)]
+                ("overlap", ctypes.c_int), ("flow_batch", ctypes.c_int)]
 
 
 class Window(ctypes.Structure):
@@ -216,6 +217,8 @@ def lib():
         L.vsc_pipeline_flows.argtypes = [vp, ctypes.POINTER(Window), ctypes.c_longlong, vp, vp, vp]
         L.vsc_pipeline_frame.argtypes = [vp, ctypes.POINTER(Window), ctypes.c_longlong, ctypes.POINTER(VscParams),
                                          vp, vp, vp]
+        L.vsc_pipeline_frames.argtypes = [vp, ctypes.POINTER(Window), i, ctypes.c_longlong, ctypes.POINTER(VscParams),
+                                          ctypes.POINTER(vp), ctypes.POINTER(vp), vp]
         L.vsc_pipeline_gate.argtypes = [vp, vp, vp, vp]
         L.vsc_pipeline_sync.argtypes = [vp, vp]
         L.vsc_server_default_cfg.argtypes = [ctypes.POINTER(ServerCfg)]
@@ -799,7 +802,10 @@ class Pipeline:
     overlap=True pipelines frames on two internal HIP streams (the flows of a
     frame while the previous one is stabilized): frame t's outputs are then
     ordered on torch's current stream by the call for frame t+1, or by sync(),
-    and the window tensors of a call must stay unchanged until then."""
+    and the window tensors of a call must stay unchanged until then.
+
+    flow_batch=B (1..4): frames() stabilizes up to B consecutive windows with
+    their flows from one batched network decode (the reference's batchSize)."""
 
     # both forms stabilize with the FMA-form solver (FP32, <= 1 LSB from the
     # reference op order); solver="exact" selects the reference order
@@ -810,7 +816,7 @@ class Pipeline:
 
     def __init__(self, width, height, form=None, flow_dtype="f32", downscale=1, upsample="bilinear",
                  scale_vectors=False, fb_max_inconsistent=0.25, ref_bwd_index=False, seed=0, upsampler_params=None,
-                 flow_head_scale=None, solver=None, overlap=False):
+                 flow_head_scale=None, solver=None, overlap=False, flow_batch=1):
         if form is not None:
             kw = self.FORMS[form]
             flow_dtype, downscale, upsample = kw["flow_dtype"], kw["downscale"], kw["upsample"]
@@ -826,6 +832,7 @@ class Pipeline:
         c.seed = seed
         c.solver = SOLVERS[solver or "exact"]
         c.overlap = int(bool(overlap))
+        c.flow_batch = int(flow_batch)
         self.cfg = c
         self.width, self.height = width, height
         h = ctypes.c_void_p()
@@ -883,6 +890,22 @@ class Pipeline:
         prm = (params or self.controlParameters).c()
         check(lib().vsc_pipeline_frame(self.h, ctypes.byref(self._window(window)), frame, ctypes.byref(prm),
                                        _ptr(out_rgba), None if out_float is None else _ptr(out_float), _stream()))
+        return out_rgba
+
+    def frames(self, windows, first, out_rgba=None, out_float=None, params=None):
+        """Stabilize len(windows) <= flow_batch consecutive windows (frames first,
+        first+1, ...) with one batched flow decode; out_rgba / out_float: lists
+        of tensors (or None).  Returns the RGBA8 outputs."""
+        torch = _torch()
+        n = len(windows)
+        if out_rgba is None:
+            out_rgba = [torch.empty((self.height, self.width, 4), dtype=torch.uint8, device="cuda") for _ in range(n)]
+        ws = (Window * n)(*(self._window(w) for w in windows))
+        ro = (ctypes.c_void_p * n)(*(None if t is None else t.data_ptr() for t in out_rgba))
+        fo = None if out_float is None else (ctypes.c_void_p * n)(*(None if t is None else t.data_ptr()
+                                                                    for t in out_float))
+        prm = (params or self.controlParameters).c()
+        check(lib().vsc_pipeline_frames(self.h, ws, n, first, ctypes.byref(prm), ro, fo, _stream()))
         return out_rgba
 
     def sync(self):

@@ -305,16 +305,16 @@ static void conv_go(const ConvArgs& a, hipStream_t s)
     // 128 blocks per item; 16-bit: under 32, where the partial-sum traffic
     // and the finalize launch cost more than idle CUs down to that size, as
     // measured on 1080p frames) and the caller provided room for the partial
-    // sums.  The split depends on one batch item's geometry only (room
-    // reserved for two items), so an item's result has the same bits alone
-    // and in a batch of two.
+    // sums.  The split depends on one batch item's geometry and the room per
+    // item only, so an item's result has the same bits alone and in a batch
+    // of any size.
     const long long item_blocks = (long long)b.tiles_x * b.tiles_y * ps * ps * b.ctiles;
     const long long item_stride = (long long)a.Ho * a.Wo * a.Cout;
     const long long stride = item_stride * a.N;
     int parts = 1;
-    if (a.partial && chunks > 1 && item_blocks < (EK == EK_F32 ? 128 : 32) && a.N <= 2) {
+    if (a.partial && chunks > 1 && item_blocks < (EK == EK_F32 ? 128 : 32)) {
         parts = (int)std::min<long long>(chunks, (256 + item_blocks - 1) / item_blocks);
-        parts = (int)std::min<long long>(parts, a.part_stride / std::max<long long>(2 * item_stride, 1));
+        parts = (int)std::min<long long>(parts, a.part_stride / std::max<long long>(item_stride, 1));
         if (parts < 2) parts = 1;
     }
     b.cpp = (chunks + parts - 1) / parts;
@@ -417,7 +417,7 @@ int launch_conv3x3(const ConvDesc& d, hipStream_t s)
     a.add = d.add.p; a.add_cs = d.add.cs; a.add_c0 = d.add.c0;
     a.gate = d.gate;
     a.partial = d.partial;
-    a.part_stride = (long long)d.partial_cap;   // capacity in floats; conv_go turns it into the part stride
+    a.part_stride = (long long)d.partial_cap;   // capacity per item in floats; conv_go turns it into the part stride
     a.Ho = Ho; a.Wo = Wo; a.Cout = Cout;
     a.pad = d.pad; a.pstep = d.dil; a.act = d.act;
     if (ek == EK_F32) conv_stride<EK_F32, EK_F32>(a, d.stride, s);

@@ -34,7 +34,7 @@ constexpr int kDecC = 456;                                  // dec channels: 448
 constexpr int kRefIn0 = 416, kRefInC = 40;
 constexpr int kRefOut[7] = {128, 128, 128, 96, 64, 32, 2};
 constexpr int kRefDil[7] = {1, 2, 4, 8, 16, 1, 1};
-constexpr size_t kPartialFloats = (size_t)4 << 20;         // 16 MB of FP32 partial sums
+constexpr size_t kPartialFloatsPerItem = (size_t)2 << 20;  // 8 MB of FP32 partial sums per batch item
 
 inline int rup(int x, int m) { return (x + m - 1) / m * m; }
 
@@ -71,19 +71,19 @@ struct Layer {
 
 struct vsc_flownet {
     int W, H, down, Wd, Hd, Wp, Hp;
-    int ek, eb, vec, slots;
+    int ek, eb, vec, slots, pairs;       // pairs: the most (first, second) pairs one decode batches
     std::vector<vsc::Layer> L;
     int ext0, up0, dec0, ref0;            // first layer index of each group
     void* img = nullptr;                  // [Hp][Wp][vec]
     void* tmp[2] = {nullptr, nullptr};    // extractor scratch (level-1 size)
     std::vector<void*> feat;              // [slot*4 + (level-3)], levels 3..6
-    void* cat0[7] = {};                   // per level [2][h][w][ccat]
-    void* dec[7] = {};                    // per level [2][h][w][kDecC]
-    void* warped = nullptr;               // [2][h3][w3][CV max]
-    float* upflow[7] = {};                // per level [2][h][w][2]
-    float* flow[7] = {};                  // per level [2][h][w][2]
-    float* refined = nullptr;             // [2][h3][w3][2]
-    void* rbuf[2] = {nullptr, nullptr};   // refiner ping-pong [2][h3][w3][128]
+    void* cat0[7] = {};                   // per level [pairs][h][w][ccat]
+    void* dec[7] = {};                    // per level [pairs][h][w][kDecC]
+    void* warped = nullptr;               // [pairs][h3][w3][CV max]
+    float* upflow[7] = {};                // per level [pairs][h][w][2]
+    float* flow[7] = {};                  // per level [pairs][h][w][2]
+    float* refined = nullptr;             // [pairs][h3][w3][2]
+    void* rbuf[2] = {nullptr, nullptr};   // refiner ping-pong [pairs][h3][w3][128]
     float* partial = nullptr;             // split-channel partial sums of the small layers
     std::vector<void*> owned;
     int last_pairs = 0;
@@ -208,19 +208,19 @@ Sizes sizes_of(const vsc_flownet* n)
     z.feat_slot = 0;
     for (int lvl = 3; lvl <= 6; ++lvl) z.feat_slot += (size_t)(n->Hp >> lvl) * (n->Wp >> lvl) * cv_of(n, lvl) * eb;
     for (int lvl = 3; lvl <= 6; ++lvl) {
-        const size_t px = (size_t)2 * (n->Hp >> lvl) * (n->Wp >> lvl);
+        const size_t px = (size_t)n->pairs * (n->Hp >> lvl) * (n->Wp >> lvl);
         z.cat0[lvl] = px * ccat_of(n, lvl) * eb;
         z.dec[lvl] = px * kDecC * eb;
         z.flow[lvl] = px * 2 * 4;
     }
-    const size_t px3 = (size_t)2 * (n->Hp >> 3) * (n->Wp >> 3);
+    const size_t px3 = (size_t)n->pairs * (n->Hp >> 3) * (n->Wp >> 3);
     z.warped = px3 * cv_of(n, 6) * eb;  // the widest level, sized at level 3's pixel count
     z.refined = px3 * 2 * 4;
     z.rbuf = px3 * 128 * eb;
     z.weights = 0;
     for (const Layer& l : n->L)
         z.weights += (l.k == 4 ? l.w.size() * 4 : (size_t)l.cout * 9 * l.cin_pad * eb) + l.b.size() * 4 + 512;
-    z.weights += kPartialFloats * 4;
+    z.weights += kPartialFloatsPerItem * n->pairs * 4;
     return z;
 }
 
@@ -265,7 +265,7 @@ int alloc_all(vsc_flownet* n)
     n->refined = static_cast<float*>(dalloc(n, z.refined, rc));
     n->rbuf[0] = dalloc(n, z.rbuf, rc);
     n->rbuf[1] = dalloc(n, z.rbuf, rc);
-    n->partial = static_cast<float*>(dalloc(n, kPartialFloats * 4, rc));
+    n->partial = static_cast<float*>(dalloc(n, kPartialFloatsPerItem * n->pairs * 4, rc));
     for (Layer& l : n->L) {
         l.dw = dalloc(n, l.k == 4 ? l.w.size() * 4 : (size_t)l.cout * 9 * l.cin_pad * n->eb, rc);
         l.db = static_cast<float*>(dalloc(n, l.b.size() * 4, rc));
@@ -286,7 +286,7 @@ int conv(vsc_flownet* n, const Layer& l, Nhwc in, Nhwc out, Nhwc add, int ek_out
     d.stride = l.stride; d.dil = l.dil; d.pad = l.dil; d.act = l.act;
     d.ek_in = n->ek; d.ek_out = ek_out;
     d.partial = n->partial;
-    d.partial_cap = kPartialFloats;
+    d.partial_cap = kPartialFloatsPerItem;
     return launch_conv3x3(d, s);
 }
 
@@ -323,8 +323,8 @@ int decode(vsc_flownet* n, int np, const int* first, const int* second, hipStrea
     for (int lvl = 6; lvl >= 3; --lvl) {
         const int h = n->Hp >> lvl, w = n->Wp >> lvl, C = kFeatC[lvl], CV = cv_of(n, lvl), cc = ccat_of(n, lvl);
         const long long px = (long long)h * w;
-        const void* f1[2] = {nullptr, nullptr};
-        const void* f2[2] = {nullptr, nullptr};
+        const void* f1[FLOWNET_MAX_PAIRS] = {};
+        const void* f2[FLOWNET_MAX_PAIRS] = {};
         for (int i = 0; i < np; ++i) { f1[i] = feat_ptr(n, first[i], lvl); f2[i] = feat_ptr(n, second[i], lvl); }
         void* cat = n->cat0[lvl];
         if (lvl == 6) {
@@ -335,7 +335,8 @@ int decode(vsc_flownet* n, int np, const int* first, const int* second, hipStrea
             const Layer& up = n->L[n->up0 + (5 - lvl)];
             const UpflowArgs ua{n->flow[lvl + 1], static_cast<const float*>(up.dw), up.db, n->upflow[lvl], cat, cc, 88 + CV};
             if ((rc = launch_warp_nhwc(ek, f2, CV, nullptr, 2, kDbl[lvl], n->warped, CV, CV, np, h, w, s, &ua))) return rc;
-            const void* wp[2] = {n->warped, static_cast<const char*>(n->warped) + px * CV * n->eb};
+            const void* wp[FLOWNET_MAX_PAIRS] = {};
+            for (int i = 0; i < np; ++i) wp[i] = static_cast<const char*>(n->warped) + i * px * CV * n->eb;
             if ((rc = launch_corr_nhwc(ek, f1, wp, CV, CV, CV, np, h, w, cat, cc, 88, s))) return rc;
         }
         // cat0 = [leaky(corr) | pad | f1 | upflow | pad]: the cost volume kernel wrote f1 at channel 88
@@ -381,7 +382,7 @@ static int flownet_dims(int W, int H, int down, int& Wd, int& Hd, int& Wp, int& 
     return VSC_OK;
 }
 
-static vsc_flownet* flownet_shell(int W, int H, int down, int dtype, int slots, int& rc)
+static vsc_flownet* flownet_shell(int W, int H, int down, int dtype, int slots, int pairs, int& rc)
 {
     int Wd, Hd, Wp, Hp;
     rc = flownet_dims(W, H, down, Wd, Hd, Wp, Hp);
@@ -394,18 +395,25 @@ static vsc_flownet* flownet_shell(int W, int H, int down, int dtype, int slots, 
         rc = set_error(VSC_E_INVALID, "vsc_flownet: slots must be in [2, 64]");
         return nullptr;
     }
+    if (pairs < 1 || pairs > FLOWNET_MAX_PAIRS) {
+        rc = set_error(VSC_E_INVALID, "vsc_flownet: pair capacity must be in [1, 8]");
+        return nullptr;
+    }
     vsc_flownet* n = new (std::nothrow) vsc_flownet();
     if (!n) { rc = set_error(VSC_E_NOMEM, "vsc_flownet: out of host memory"); return nullptr; }
     n->W = W; n->H = H; n->down = down; n->Wd = Wd; n->Hd = Hd; n->Wp = Wp; n->Hp = Hp;
     n->ek = elem_kind(dtype); n->eb = elem_bytes(n->ek); n->vec = n->ek == EK_F32 ? 4 : 8; n->slots = slots;
+    n->pairs = pairs;
     build_layers(n);
     return n;
 }
 
-extern "C" size_t vsc_flownet_footprint(int width, int height, int downscale, int dtype, int slots)
+namespace vsc {
+
+size_t flownet_footprint_pairs(int width, int height, int downscale, int dtype, int slots, int pairs)
 {
     int rc;
-    vsc_flownet* n = flownet_shell(width, height, downscale, dtype, slots, rc);
+    vsc_flownet* n = flownet_shell(width, height, downscale, dtype, slots, pairs, rc);
     if (!n) return 0;
     for (Layer& l : n->L) { l.w.resize((size_t)l.cin * l.cout * l.k * l.k); l.b.resize(l.cout); }
     const size_t f = footprint_of(n);
@@ -413,13 +421,13 @@ extern "C" size_t vsc_flownet_footprint(int width, int height, int downscale, in
     return f;
 }
 
-extern "C" int vsc_flownet_create(int width, int height, int downscale, int dtype, unsigned long long seed, int slots,
-                                  vsc_flownet** out)
+int flownet_create_pairs(int width, int height, int downscale, int dtype, unsigned long long seed, int slots,
+                         int pairs, vsc_flownet** out)
 {
     if (!out) return set_error(VSC_E_INVALID, "vsc_flownet_create: out is NULL");
     *out = nullptr;
     int rc;
-    vsc_flownet* n = flownet_shell(width, height, downscale, dtype, slots, rc);
+    vsc_flownet* n = flownet_shell(width, height, downscale, dtype, slots, pairs, rc);
     if (!n) return rc;
     init_params(n, seed);
     if ((rc = alloc_all(n)) || (rc = upload_params(n))) {
@@ -428,6 +436,19 @@ extern "C" int vsc_flownet_create(int width, int height, int downscale, int dtyp
     }
     *out = n;
     return VSC_OK;
+}
+
+}  // namespace vsc
+
+extern "C" size_t vsc_flownet_footprint(int width, int height, int downscale, int dtype, int slots)
+{
+    return flownet_footprint_pairs(width, height, downscale, dtype, slots, 2);
+}
+
+extern "C" int vsc_flownet_create(int width, int height, int downscale, int dtype, unsigned long long seed, int slots,
+                                  vsc_flownet** out)
+{
+    return flownet_create_pairs(width, height, downscale, dtype, seed, slots, 2, out);
 }
 
 extern "C" int vsc_flownet_destroy(vsc_flownet* n)
@@ -498,7 +519,8 @@ extern "C" int vsc_flownet_flow(vsc_flownet* n, int npairs, const int* first, co
                                 int out_channels, vsc_hip_stream stream)
 {
     if (!n || !first || !second || !out) return set_error(VSC_E_INVALID, "vsc_flownet_flow: NULL argument");
-    if (npairs < 1 || npairs > 2) return set_error(VSC_E_INVALID, "vsc_flownet_flow: npairs must be 1 or 2");
+    if (npairs < 1 || npairs > n->pairs)
+        return set_error(VSC_E_INVALID, "vsc_flownet_flow: npairs must be in [1, the network's pair capacity]");
     if (out_channels != 2 && out_channels != 3) return set_error(VSC_E_INVALID, "vsc_flownet_flow: out_channels 2|3");
     for (int i = 0; i < npairs; ++i) {
         if (first[i] < 0 || first[i] >= n->slots || second[i] < 0 || second[i] >= n->slots)
@@ -509,8 +531,7 @@ extern "C" int vsc_flownet_flow(vsc_flownet* n, int npairs, const int* first, co
     int rc = decode(n, npairs, first, second, s);
     if (rc) return rc;
     const int h = n->Hp >> 3, w = n->Wp >> 3;
-    float* const outs[2] = {out[0], npairs > 1 ? out[1] : nullptr};
-    if ((rc = launch_flow_final(n->refined, npairs, h, w, n->Hp, n->Wp, outs, n->Wd, n->Hd, out_channels, 0, s)))
+    if ((rc = launch_flow_final(n->refined, npairs, h, w, n->Hp, n->Wp, out, n->Wd, n->Hd, out_channels, 0, s)))
         return rc;
     n->last_pairs = npairs;
     return VSC_OK;

@@ -6,8 +6,13 @@
 
 namespace vsc {
 
-// f1_c0 >= 0: also copies f1 (C channels) into `out` at channel f1_c0
-int launch_corr_nhwc(int ek, const void* const f1[2], const void* const f2[2], int cs1, int cs2, int C, int N, int H,
+// the most (first, second) pairs one decode batches: the forward and backward
+// flows of up to 4 frames
+constexpr int FLOWNET_MAX_PAIRS = 8;
+
+// f1_c0 >= 0: also copies f1 (C channels) into `out` at channel f1_c0;
+// f1, f2: N <= FLOWNET_MAX_PAIRS items
+int launch_corr_nhwc(int ek, const void* const* f1, const void* const* f2, int cs1, int cs2, int C, int N, int H,
                      int W, void* out, int ocs, int f1_c0, hipStream_t s);
 // upflow fused into the warp (moduleUpflow, pwcnet.py:472, then Backward)
 struct UpflowArgs {
@@ -18,12 +23,12 @@ struct UpflowArgs {
     void* cat; int ccs, cc0;   // its copy in the concatenation buffer (element kind)
 };
 // up: nullptr, or the warp flow is computed from up->in (flow/fcs unused)
-int launch_warp_nhwc(int ek, const void* const src[2], int scs, const float* flow, int fcs, float scale, void* out,
+int launch_warp_nhwc(int ek, const void* const* src, int scs, const float* flow, int fcs, float scale, void* out,
                      int ocs, int C, int N, int H, int W, hipStream_t s, const UpflowArgs* up = nullptr);
 int launch_preprocess(int ek, const uint8_t* rgba, int W, int H, int Wd, int Hd, int Wp, int Hp, void* out, int ocs,
                       hipStream_t s);
-// fl: N (1 or 2) level-3 flows [N][h][w][2]; out[i]: the i-th flow at Wd x Hd
-int launch_flow_final(const float* fl, int N, int h, int w, int Hp, int Wp, float* const out[2], int Wd, int Hd,
+// fl: N (1 .. FLOWNET_MAX_PAIRS) level-3 flows [N][h][w][2]; out[i]: the i-th flow at Wd x Hd
+int launch_flow_final(const float* fl, int N, int h, int w, int Hp, int Wp, float* const* out, int Wd, int Hd,
                       int out_ch, int swap_scale, hipStream_t s);
 
 }  // namespace vsc

@@ -30,7 +30,8 @@ template <> struct CorrK<EK_F32> { static constexpr int KC = 16; };
 template <> struct CorrK<EK_BF16> { static constexpr int KC = 32; };
 
 struct CorrArgs {
-    const void* f1[2]; const void* f2[2];   // per batch item, channel 0 of pixel 0
+    const void* f1[FLOWNET_MAX_PAIRS];       // per batch item, channel 0 of pixel 0
+    const void* f2[FLOWNET_MAX_PAIRS];
     int cs1, cs2, C, H, W;
     void* out; int ocs;                      // out: channel 0 of pixel 0 of batch 0
     int f1_c0;                               // >= 0: f1's C channels are also copied to out at this channel
@@ -167,9 +168,10 @@ __global__ __launch_bounds__(512) void k_corr_nhwc(CorrArgs a)
     }
 }
 
-int launch_corr_nhwc(int ek, const void* const f1[2], const void* const f2[2], int cs1, int cs2, int C, int N,
+int launch_corr_nhwc(int ek, const void* const* f1, const void* const* f2, int cs1, int cs2, int C, int N,
                      int H, int W, void* out, int ocs, int f1_c0, hipStream_t s)
 {
+    if (N < 1 || N > FLOWNET_MAX_PAIRS) return set_error(VSC_E_INVALID, "corr nhwc: batch of 1 to 8 items");
     CorrArgs a{};
     for (int i = 0; i < N; ++i) { a.f1[i] = f1[i]; a.f2[i] = f2[i]; }
     a.cs1 = cs1; a.cs2 = cs2; a.C = C; a.H = H; a.W = W; a.out = out; a.ocs = ocs;
@@ -216,7 +218,8 @@ __device__ inline void upflow_px(const float* __restrict__ in, int Hi, int Wi, c
 // rule of the op kernel), zero otherwise.  Thread = (pixel, 16-byte group).
 // ---------------------------------------------------------------------------
 struct WarpArgs {
-    const void* src[2]; int scs;            // f2 features of each batch item
+    const void* src[FLOWNET_MAX_PAIRS];     // f2 features of each batch item
+    int scs;
     const float* flow; int fcs;             // FP32 flow, channels (x, y) at flow + pix*fcs
     float scale;
     void* out; int ocs;                     // warped features
@@ -307,9 +310,10 @@ __global__ __launch_bounds__(256) void k_warp_nhwc(WarpArgs a)
     }
 }
 
-int launch_warp_nhwc(int ek, const void* const src[2], int scs, const float* flow, int fcs, float scale, void* out,
+int launch_warp_nhwc(int ek, const void* const* src, int scs, const float* flow, int fcs, float scale, void* out,
                      int ocs, int C, int N, int H, int W, hipStream_t s, const UpflowArgs* up)
 {
+    if (N < 1 || N > FLOWNET_MAX_PAIRS) return set_error(VSC_E_INVALID, "warp nhwc: batch of 1 to 8 items");
     WarpArgs a{};
     for (int i = 0; i < N; ++i) a.src[i] = src[i];
     a.scs = scs; a.flow = flow; a.fcs = fcs; a.scale = scale; a.out = out; a.ocs = ocs;
@@ -387,7 +391,7 @@ int launch_preprocess(int ek, const uint8_t* rgba, int W, int H, int Wd, int Hd,
 // channel 0 scaled by Hd/Hp and channel 1 by Wd/Wp (the wrapper's order),
 // written HWC with out_ch (2 or 3; the third is 0) channels.
 // ---------------------------------------------------------------------------
-struct OutPair { float* p[2]; };
+struct OutPair { float* p[FLOWNET_MAX_PAIRS]; };
 
 // blockIdx.z: the pair (flow item) of the batch
 __global__ void k_flow_final(const float* __restrict__ flows, int h, int w, int Hp, int Wp, OutPair outs, int Wd,
@@ -424,10 +428,10 @@ __global__ void k_flow_final(const float* __restrict__ flows, int h, int w, int 
     if (out_ch == 3) o[2] = 0.0f;
 }
 
-int launch_flow_final(const float* fl, int N, int h, int w, int Hp, int Wp, float* const out[2], int Wd, int Hd,
+int launch_flow_final(const float* fl, int N, int h, int w, int Hp, int Wp, float* const* out, int Wd, int Hd,
                       int out_ch, int swap_scale, hipStream_t s)
 {
-    if (N < 1 || N > 2) return set_error(VSC_E_INVALID, "flow final: one or two flows");
+    if (N < 1 || N > FLOWNET_MAX_PAIRS) return set_error(VSC_E_INVALID, "flow final: 1 to 8 flows");
     OutPair o{};
     for (int i = 0; i < N; ++i) o.p[i] = out[i];
     dim3 grid((Wd + 255) / 256, Hd, N);

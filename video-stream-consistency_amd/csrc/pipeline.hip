@@ -10,33 +10,38 @@
 //      learned upsampler gated per frame by the forward-backward check,
 //   4. the fused stabilization of vsc_stab_frame.
 // All buffers are allocated at creation; a frame allocates nothing.  The
-// flows of frame t (and its fallback gate) live in buffer set t & 1, so that
-// with overlap the flows of frame t+1 (internal stream sf: features, decode,
-// consistency gate) are computed while frame t is upsampled and stabilized
-// (internal stream ss).
+// flows of frame t (and its fallback gate) live in buffer set t mod 2B (B =
+// cfg.flow_batch), so that with overlap the flows of the next frame or batch
+// (internal stream sf: features, decode, consistency gate) are computed while
+// the current one is upsampled and stabilized (internal stream ss).  A batch
+// of n <= B frames (vsc_pipeline_frames) decodes the forward and backward
+// flows of all its frames in one network batch of 2n pairs.
 #include "vsc_internal.h"
 
 #include <cmath>
 #include <new>
 
+constexpr int PIPE_MAX_BATCH = 4, PIPE_MAX_SETS = 2 * PIPE_MAX_BATCH, PIPE_MAX_SLOTS = PIPE_MAX_BATCH + 2;
+
 struct vsc_pipeline {
     int W, H, Wd, Hd;
     vsc_pipeline_cfg cfg;
+    int sets = 2, slots = 3;                       // flow buffer sets (2B), feature slots (B + 2)
     vsc_flownet* net = nullptr;
     vsc_upsampler* ups = nullptr;
     vsc_stream* stab = nullptr;
-    float* lo[2][2] = {};                          // [frame & 1][fwd, bwd] flows at the flow resolution (HWC 2)
-    float* hi[2][2] = {};                          // [frame & 1][fwd, bwd] at frame size when upsampling here
-    uint32_t* count = nullptr;                     // [frame & 1] inconsistent-pixel counts (64 B apart)
-    float* gate = nullptr;                         // [frame & 1] fallback gates (64 B apart)
-    long long slot_frame[3] = {-1, -1, -1};        // frame index held by each feature slot
+    float* lo[PIPE_MAX_SETS][2] = {};              // [frame mod sets][fwd, bwd] flows at the flow resolution (HWC 2)
+    float* hi[PIPE_MAX_SETS][2] = {};              // [frame mod sets][fwd, bwd] at frame size when upsampling here
+    uint32_t* count = nullptr;                     // [set] inconsistent-pixel counts (64 B apart)
+    float* gate = nullptr;                         // [set] fallback gates (64 B apart)
+    long long slot_frame[PIPE_MAX_SLOTS] = {-1, -1, -1, -1, -1, -1};   // frame index held by each feature slot
     long long prev_fwd_frame = -2;                 // last frame whose flows were computed
     // overlap: flows on sf, stabilization on ss; ev_s[k] = the stabilization
     // that last read buffer set k, ev_f = the last flow work
     hipStream_t sf = nullptr, ss = nullptr;
-    hipEvent_t ev_in = nullptr, ev_f = nullptr, ev_s[2] = {nullptr, nullptr};
-    bool f_rec = false, s_rec[2] = {false, false};
-    int last_parity = 0;                           // buffer set of the last frame
+    hipEvent_t ev_in = nullptr, ev_f = nullptr, ev_s[PIPE_MAX_SETS] = {};
+    bool f_rec = false, s_rec[PIPE_MAX_SETS] = {};
+    int last_set = 0;                              // buffer set of the last frame
 };
 
 namespace vsc {
@@ -67,7 +72,7 @@ int dalloc(void** p, size_t bytes)
 bool learned(const vsc_pipeline_cfg& c) { return c.downscale > 1 && c.upsample == 1; }
 bool upsample_here(const vsc_pipeline_cfg& c) { return c.downscale > 1 && (c.upsample == 1 || c.scale_vectors); }
 
-int parity(long long frame) { return (int)(((frame % 2) + 2) % 2); }
+int pmod(long long v, int m) { return (int)(((v % m) + m) % m); }
 
 bool capturing(hipStream_t s)
 {
@@ -80,7 +85,7 @@ int join_all(vsc_pipeline* p, hipStream_t s)
 {
     if (!p->cfg.overlap) return VSC_OK;
     if (p->f_rec && check_hip(hipStreamWaitEvent(s, p->ev_f, 0), "vsc_pipeline join")) return VSC_E_HIP;
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < p->sets; ++k)
         if (p->s_rec[k] && check_hip(hipStreamWaitEvent(s, p->ev_s[k], 0), "vsc_pipeline join")) return VSC_E_HIP;
     return VSC_OK;
 }
@@ -102,6 +107,7 @@ extern "C" void vsc_pipeline_default_cfg(vsc_pipeline_cfg* c)
     c->seed = 0;
     c->solver = 0;
     c->overlap = 0;
+    c->flow_batch = 1;
 }
 
 static int cfg_ok(int W, int H, const vsc_pipeline_cfg* c)
@@ -114,6 +120,8 @@ static int cfg_ok(int W, int H, const vsc_pipeline_cfg* c)
     if (c->upsample != 0 && c->upsample != 1) return set_error(VSC_E_INVALID, "vsc_pipeline: upsample must be 0 or 1");
     if (c->solver != 0 && c->solver != 1) return set_error(VSC_E_INVALID, "vsc_pipeline: solver must be 0 or 1");
     if (c->overlap != 0 && c->overlap != 1) return set_error(VSC_E_INVALID, "vsc_pipeline: overlap must be 0 or 1");
+    if (c->flow_batch < 1 || c->flow_batch > PIPE_MAX_BATCH)
+        return set_error(VSC_E_INVALID, "vsc_pipeline: flow_batch must be in [1, 4]");
     return VSC_OK;
 }
 
@@ -121,11 +129,12 @@ extern "C" size_t vsc_pipeline_footprint(int width, int height, const vsc_pipeli
 {
     if (cfg_ok(width, height, cfg)) return 0;
     const int Wd = (int)std::lround((double)width / cfg->downscale), Hd = (int)std::lround((double)height / cfg->downscale);
-    size_t t = vsc_flownet_footprint(width, height, cfg->downscale, cfg->flow_dtype, 3);
+    const int B = cfg->flow_batch;
+    size_t t = flownet_footprint_pairs(width, height, cfg->downscale, cfg->flow_dtype, B + 2, 2 * B);
     const bool here = upsample_here(*cfg);
     t += vsc_stream_footprint(width, height, here ? width : Wd, here ? height : Hd);
-    t += 4 * (size_t)Wd * Hd * 2 * 4;
-    if (here) t += 4 * (size_t)width * height * 2 * 4;
+    t += 2 * 2 * B * (size_t)Wd * Hd * 2 * 4;
+    if (here) t += 2 * 2 * B * (size_t)width * height * 2 * 4;
     if (learned(*cfg)) t += 4 * 3202 * 4 + 512;
     return t + 64;
 }
@@ -139,7 +148,10 @@ extern "C" int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg
     vsc_pipeline* p = new (std::nothrow) vsc_pipeline();
     if (!p) return set_error(VSC_E_NOMEM, "vsc_pipeline_create: host memory");
     p->W = width; p->H = height; p->cfg = *cfg;
-    if ((rc = vsc_flownet_create(width, height, cfg->downscale, cfg->flow_dtype, cfg->seed, 3, &p->net)) ||
+    p->sets = 2 * cfg->flow_batch;
+    p->slots = cfg->flow_batch + 2;
+    if ((rc = flownet_create_pairs(width, height, cfg->downscale, cfg->flow_dtype, cfg->seed, p->slots, 2 * cfg->flow_batch,
+                                   &p->net)) ||
         (rc = vsc_flownet_flow_size(p->net, &p->Wd, &p->Hd))) {
         vsc_pipeline_destroy(p);
         return rc;
@@ -150,24 +162,26 @@ extern "C" int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg
         return rc;
     }
     const size_t lo_b = (size_t)p->Wd * p->Hd * 2 * 4, hi_b = (size_t)width * height * 2 * 4;
-    for (int i = 0; i < 4 && !rc; ++i) rc = dalloc(reinterpret_cast<void**>(&p->lo[i / 2][i % 2]), lo_b);
+    for (int i = 0; i < 2 * p->sets && !rc; ++i) rc = dalloc(reinterpret_cast<void**>(&p->lo[i / 2][i % 2]), lo_b);
     if (here)
-        for (int i = 0; i < 4 && !rc; ++i) rc = dalloc(reinterpret_cast<void**>(&p->hi[i / 2][i % 2]), hi_b);
+        for (int i = 0; i < 2 * p->sets && !rc; ++i) rc = dalloc(reinterpret_cast<void**>(&p->hi[i / 2][i % 2]), hi_b);
     if (!rc && cfg->overlap) {
         // default-priority streams; events without timing
         if (check_hip(hipStreamCreateWithFlags(&p->sf, hipStreamNonBlocking), "vsc_pipeline_create") ||
             check_hip(hipStreamCreateWithFlags(&p->ss, hipStreamNonBlocking), "vsc_pipeline_create") ||
             check_hip(hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming), "vsc_pipeline_create") ||
-            check_hip(hipEventCreateWithFlags(&p->ev_f, hipEventDisableTiming), "vsc_pipeline_create") ||
-            check_hip(hipEventCreateWithFlags(&p->ev_s[0], hipEventDisableTiming), "vsc_pipeline_create") ||
-            check_hip(hipEventCreateWithFlags(&p->ev_s[1], hipEventDisableTiming), "vsc_pipeline_create"))
+            check_hip(hipEventCreateWithFlags(&p->ev_f, hipEventDisableTiming), "vsc_pipeline_create"))
             rc = VSC_E_HIP;
+        for (int k = 0; k < p->sets && !rc; ++k)
+            if (check_hip(hipEventCreateWithFlags(&p->ev_s[k], hipEventDisableTiming), "vsc_pipeline_create"))
+                rc = VSC_E_HIP;
     }
-    if (!rc) rc = dalloc(reinterpret_cast<void**>(&p->count), 256);
-    if (!rc) rc = dalloc(reinterpret_cast<void**>(&p->gate), 256);
+    const size_t gb = 64 * PIPE_MAX_SETS;
+    if (!rc) rc = dalloc(reinterpret_cast<void**>(&p->count), gb);
+    if (!rc) rc = dalloc(reinterpret_cast<void**>(&p->gate), gb);
     // forms without the learned upsampler report gate 0 (bilinear) and count 0
-    if (!rc && (check_hip(hipMemset(p->count, 0, 256), "vsc_pipeline_create") ||
-                check_hip(hipMemset(p->gate, 0, 256), "vsc_pipeline_create")))
+    if (!rc && (check_hip(hipMemset(p->count, 0, gb), "vsc_pipeline_create") ||
+                check_hip(hipMemset(p->gate, 0, gb), "vsc_pipeline_create")))
         rc = VSC_E_HIP;
     if (!rc && learned(*cfg)) {
         rc = vsc_upsampler_create(cfg->seed, &p->ups);
@@ -188,10 +202,16 @@ extern "C" int vsc_pipeline_destroy(vsc_pipeline* p)
     if (p->net) vsc_flownet_destroy(p->net);
     if (p->ups) vsc_upsampler_destroy(p->ups);
     if (p->stab) vsc_stream_destroy(p->stab);
-    void* bufs[] = {p->lo[0][0], p->lo[0][1], p->lo[1][0], p->lo[1][1], p->hi[0][0], p->hi[0][1],
-                    p->hi[1][0], p->hi[1][1], p->count, p->gate};
-    for (void* b : bufs) (void)hipFree(b);
-    for (hipEvent_t e : {p->ev_in, p->ev_f, p->ev_s[0], p->ev_s[1]})
+    for (int k = 0; k < PIPE_MAX_SETS; ++k)
+        for (int d = 0; d < 2; ++d) {
+            if (p->lo[k][d]) (void)hipFree(p->lo[k][d]);
+            if (p->hi[k][d]) (void)hipFree(p->hi[k][d]);
+        }
+    for (void* b : {static_cast<void*>(p->count), static_cast<void*>(p->gate)})
+        if (b) (void)hipFree(b);
+    for (hipEvent_t e : {p->ev_in, p->ev_f})
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : p->ev_s)
         if (e) (void)hipEventDestroy(e);
     for (hipStream_t h : {p->sf, p->ss})
         if (h) (void)hipStreamDestroy(h);
@@ -222,7 +242,7 @@ extern "C" int vsc_pipeline_stream(vsc_pipeline* p, vsc_stream** s)
 
 static int ensure_features(vsc_pipeline* p, const uint8_t* rgba, long long frame, hipStream_t s)
 {
-    const int slot = (int)(((frame % 3) + 3) % 3);
+    const int slot = pmod(frame, p->slots);
     if (p->slot_frame[slot] == frame) return VSC_OK;
     int rc = vsc_flownet_extract(p->net, rgba, slot, reinterpret_cast<vsc_hip_stream>(s));
     if (rc) return rc;
@@ -230,19 +250,40 @@ static int ensure_features(vsc_pipeline* p, const uint8_t* rgba, long long frame
     return VSC_OK;
 }
 
+// the forward and backward pairs of window `frame` (feature slots of frames
+// frame-1 .. frame+1, extracted before)
+static void frame_pairs(const vsc_pipeline* p, long long frame, int* first, int* second)
+{
+    const int sp = pmod(frame - 1, p->slots), sc = pmod(frame, p->slots), sn = pmod(frame + 1, p->slots);
+    // forward cur -> next; backward cur -> prev (or next -> cur as videostabilizer.cpp:271-272)
+    first[0] = sc;
+    second[0] = sn;
+    first[1] = p->cfg.ref_bwd_index ? sn : sc;
+    second[1] = p->cfg.ref_bwd_index ? sc : sp;
+}
+
+// the network part for the n consecutive windows of frames first .. first+n-1:
+// features of frames first-1 .. first+n (each extracted once), then one
+// batched decode of their 2n flow pairs into outs (fwd, bwd per frame)
+static int flows_batch_on(vsc_pipeline* p, const vsc_window* w, int n, long long first, float* const* outs,
+                          hipStream_t s)
+{
+    int rc;
+    for (int i = 0; i < n; ++i)
+        if ((rc = ensure_features(p, w[i].orig_prev, first + i - 1, s)) ||
+            (rc = ensure_features(p, w[i].orig_cur, first + i, s)) ||
+            (rc = ensure_features(p, w[i].orig_next, first + i + 1, s)))
+            return rc;
+    int a[2 * PIPE_MAX_BATCH], b[2 * PIPE_MAX_BATCH];
+    for (int i = 0; i < n; ++i) frame_pairs(p, first + i, a + 2 * i, b + 2 * i);
+    return vsc_flownet_flow(p->net, 2 * n, a, b, outs, 2, reinterpret_cast<vsc_hip_stream>(s));
+}
+
 static int flows_on(vsc_pipeline* p, const vsc_window* w, long long frame, float* fwd_out, float* bwd_out,
                     hipStream_t s)
 {
-    int rc;
-    if ((rc = ensure_features(p, w->orig_prev, frame - 1, s)) || (rc = ensure_features(p, w->orig_cur, frame, s)) ||
-        (rc = ensure_features(p, w->orig_next, frame + 1, s)))
-        return rc;
-    const int sp = (int)(((frame - 1) % 3 + 3) % 3), sc = (int)((frame % 3 + 3) % 3), sn = (int)(((frame + 1) % 3 + 3) % 3);
-    // forward cur -> next; backward cur -> prev (or next -> cur as videostabilizer.cpp:271-272)
-    const int first[2] = {sc, p->cfg.ref_bwd_index ? sn : sc};
-    const int second[2] = {sn, p->cfg.ref_bwd_index ? sc : sp};
     float* outs[2] = {fwd_out, bwd_out};
-    return vsc_flownet_flow(p->net, 2, first, second, outs, 2, reinterpret_cast<vsc_hip_stream>(s));
+    return flows_batch_on(p, w, 1, frame, outs, s);
 }
 
 extern "C" int vsc_pipeline_flows(vsc_pipeline* p, const vsc_window* w, long long frame, float* fwd_out,
@@ -253,7 +294,7 @@ extern "C" int vsc_pipeline_flows(vsc_pipeline* p, const vsc_window* w, long lon
     hipStream_t s = as_stream(stream);
     int rc = join_all(p, s);   // the flow network's buffers may still be in use by frame work
     if (rc) return rc;
-    const int k = parity(frame);
+    const int k = pmod(frame, p->sets);
     return flows_on(p, w, frame, fwd_out ? fwd_out : p->lo[k][0], bwd_out ? bwd_out : p->lo[k][1], s);
 }
 
@@ -263,31 +304,27 @@ namespace {
 // enqueueing (or capturing) fails part-way, so a later call recomputes what
 // the failed one did not finish.
 struct HostState {
-    long long slot_frame[3];
+    long long slot_frame[PIPE_MAX_SLOTS];
     long long prev_fwd;
-    int parity;
-    explicit HostState(const vsc_pipeline* p)
-        : slot_frame{p->slot_frame[0], p->slot_frame[1], p->slot_frame[2]},
-          prev_fwd(p->prev_fwd_frame), parity(p->last_parity) {}
+    int set;
+    explicit HostState(const vsc_pipeline* p) : prev_fwd(p->prev_fwd_frame), set(p->last_set)
+    {
+        for (int i = 0; i < PIPE_MAX_SLOTS; ++i) slot_frame[i] = p->slot_frame[i];
+    }
     void restore(vsc_pipeline* p) const
     {
-        for (int i = 0; i < 3; ++i) p->slot_frame[i] = slot_frame[i];
+        for (int i = 0; i < PIPE_MAX_SLOTS; ++i) p->slot_frame[i] = slot_frame[i];
         p->prev_fwd_frame = prev_fwd;
-        p->last_parity = parity;
+        p->last_set = set;
     }
 };
 
-// The flows of window `frame` at frame size: features, the batched
-// forward/backward decode and the consistency gate on fs, the upsampling on
-// ss (ov: the flows' completion is ordered before ss by ev_f).  *ff, *fb
-// receive the frame-size (or, without upsampling here, flow-size) flows.
-int frame_flows(vsc_pipeline* p, const vsc_window* w, long long frame, hipStream_t fs, hipStream_t ss, bool ov,
-                const float** ff, const float** fb)
+// The fallback gate of window `frame` on fs (its flows are in set frame mod
+// sets; the previous frame's forward flow, when computed, in its own set).
+int gate_on(vsc_pipeline* p, long long frame, hipStream_t fs)
 {
-    const int f = p->cfg.downscale;
-    const int k = parity(frame);
+    const int k = pmod(frame, p->sets);
     int rc;
-    if ((rc = flows_on(p, w, frame, p->lo[k][0], p->lo[k][1], fs))) return rc;
     const vsc_hip_stream fsv = reinterpret_cast<vsc_hip_stream>(fs);
     // previous forward flow for the consistency check of this frame's backward flow
     const bool have_prev = p->prev_fwd_frame == frame - 1;
@@ -304,21 +341,25 @@ int frame_flows(vsc_pipeline* p, const vsc_window* w, long long frame, hipStream
                 return rc;
         } else {
             // bwd: cur -> prev against the previous frame's fwd: prev -> cur
-            if ((rc = vsc_flow_fb_gate(p->lo[k][1], p->lo[1 - k][0], p->Wd, p->Hd, 2, p->cfg.fb_max_inconsistent,
-                                       count, gate, fsv)))
+            if ((rc = vsc_flow_fb_gate(p->lo[k][1], p->lo[pmod(frame - 1, p->sets)][0], p->Wd, p->Hd, 2,
+                                       p->cfg.fb_max_inconsistent, count, gate, fsv)))
                 return rc;
         }
     }
     p->prev_fwd_frame = frame;
-    p->last_parity = k;
-    if (ov) {
-        if (check_hip(hipEventRecord(p->ev_f, fs), "vsc_pipeline_frame") ||
-            check_hip(hipStreamWaitEvent(ss, p->ev_f, 0), "vsc_pipeline_frame"))
-            return VSC_E_HIP;
-        p->f_rec = true;
-    }
-    // upsampling to the frame, on the stabilization stream (it reads only this
-    // frame's buffer set)
+    p->last_set = k;
+    return VSC_OK;
+}
+
+// Upsampling of window `frame`'s flows to the frame, on ss (it reads only this
+// frame's buffer set).  *ff, *fb receive the frame-size (or, without
+// upsampling here, flow-size) flows.
+int upsample_on(vsc_pipeline* p, const vsc_window* w, long long frame, hipStream_t ss, const float** ff,
+                const float** fb)
+{
+    const int f = p->cfg.downscale;
+    const int k = pmod(frame, p->sets);
+    int rc;
     *ff = p->lo[k][0];
     *fb = p->lo[k][1];
     if (learned(p->cfg)) {
@@ -326,7 +367,8 @@ int frame_flows(vsc_pipeline* p, const vsc_window* w, long long frame, hipStream
         const float sc = (float)f;
         const float* const lo[2] = {p->lo[k][0], p->lo[k][1]};
         float* const hi[2] = {p->hi[k][0], p->hi[k][1]};
-        if ((rc = upsampler_run_flows(p->ups, 2, lo, p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, hi, 2, gate, ss)))
+        if ((rc = upsampler_run_flows(p->ups, 2, lo, p->Wd, p->Hd, 2, w->orig_cur, p->W, p->H, sc, hi, 2,
+                                      p->gate + 16 * k, ss)))
             return rc;
         *ff = p->hi[k][0];
         *fb = p->hi[k][1];
@@ -343,10 +385,33 @@ int frame_flows(vsc_pipeline* p, const vsc_window* w, long long frame, hipStream
     return VSC_OK;
 }
 
+// flows computed on fs are complete before ss continues (overlap)
+int flows_done(vsc_pipeline* p, hipStream_t fs, hipStream_t ss)
+{
+    if (check_hip(hipEventRecord(p->ev_f, fs), "vsc_pipeline_frame") ||
+        check_hip(hipStreamWaitEvent(ss, p->ev_f, 0), "vsc_pipeline_frame"))
+        return VSC_E_HIP;
+    p->f_rec = true;
+    return VSC_OK;
+}
+
+// The flows of window `frame` at frame size: features, the batched
+// forward/backward decode and the consistency gate on fs, the upsampling on
+// ss (ov: the flows' completion is ordered before ss by ev_f).
+int frame_flows(vsc_pipeline* p, const vsc_window* w, long long frame, hipStream_t fs, hipStream_t ss, bool ov,
+                const float** ff, const float** fb)
+{
+    const int k = pmod(frame, p->sets);
+    int rc;
+    if ((rc = flows_on(p, w, frame, p->lo[k][0], p->lo[k][1], fs)) || (rc = gate_on(p, frame, fs))) return rc;
+    if (ov && (rc = flows_done(p, fs, ss))) return rc;
+    return upsample_on(p, w, frame, ss, ff, fb);
+}
+
 // the flows frame_flows leaves for window `frame` (no launch)
 void frame_flow_ptrs(const vsc_pipeline* p, long long frame, const float** ff, const float** fb)
 {
-    const int k = parity(frame);
+    const int k = pmod(frame, p->sets);
     const bool hi = learned(p->cfg) || upsample_here(p->cfg);
     *ff = hi ? p->hi[k][0] : p->lo[k][0];
     *fb = hi ? p->hi[k][1] : p->lo[k][1];
@@ -357,8 +422,19 @@ void frame_flow_ptrs(const vsc_pipeline* p, long long frame, const float** ff, c
 extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long long frame, const vsc_params* params,
                                   uint8_t* out_rgba, float* out_float, vsc_hip_stream stream)
 {
-    if (!p || !w || !w->orig_prev || !w->orig_cur || !w->orig_next)
-        return set_error(VSC_E_INVALID, "vsc_pipeline_frame: NULL argument");
+    return vsc_pipeline_frames(p, w, 1, frame, params, &out_rgba, &out_float, stream);
+}
+
+extern "C" int vsc_pipeline_frames(vsc_pipeline* p, const vsc_window* w, int n, long long first,
+                                   const vsc_params* params, uint8_t* const* out_rgba, float* const* out_float,
+                                   vsc_hip_stream stream)
+{
+    if (!p || !w) return set_error(VSC_E_INVALID, "vsc_pipeline_frame: NULL argument");
+    if (n < 1 || n > p->cfg.flow_batch)
+        return set_error(VSC_E_INVALID, "vsc_pipeline_frames: n must be in [1, flow_batch]");
+    for (int i = 0; i < n; ++i)
+        if (!w[i].orig_prev || !w[i].orig_cur || !w[i].orig_next)
+            return set_error(VSC_E_INVALID, "vsc_pipeline_frame: NULL argument");
     // every check before any state change: a refused frame leaves the
     // pipeline as it was
     int rc = stream_frame_check(p->stab, params);
@@ -366,31 +442,51 @@ extern "C" int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long lon
     hipStream_t cs = as_stream(stream);
     const bool ov = p->cfg.overlap && !capturing(cs);
     hipStream_t fs = ov ? p->sf : cs, ss = ov ? p->ss : cs;
-    const int k = parity(frame);
     const HostState saved(p);
     auto fail = [&](int code) {
         saved.restore(p);
         return code;
     };
     if (ov) {
-        // flows wait for the caller's inputs and for the stabilization that
-        // last read buffer set k (frame - 2)
+        // flows wait for the caller's inputs and for the stabilizations that
+        // last read the batch's buffer sets (frames first - 2B ..)
         if (check_hip(hipEventRecord(p->ev_in, cs), "vsc_pipeline_frame") ||
-            check_hip(hipStreamWaitEvent(fs, p->ev_in, 0), "vsc_pipeline_frame") ||
-            (p->s_rec[k] && check_hip(hipStreamWaitEvent(fs, p->ev_s[k], 0), "vsc_pipeline_frame")))
+            check_hip(hipStreamWaitEvent(fs, p->ev_in, 0), "vsc_pipeline_frame"))
             return fail(VSC_E_HIP);
+        for (int i = 0; i < n; ++i) {
+            const int k = pmod(first + i, p->sets);
+            if (p->s_rec[k] && check_hip(hipStreamWaitEvent(fs, p->ev_s[k], 0), "vsc_pipeline_frame"))
+                return fail(VSC_E_HIP);
+        }
     }
-    const float *ff = nullptr, *fb = nullptr;
-    if ((rc = frame_flows(p, w, frame, fs, ss, ov, &ff, &fb))) return fail(rc);
-    vsc_frame_in in{w->orig_prev, w->orig_cur, w->orig_next, w->proc_prev, w->proc_cur, w->proc_next, ff, fb, 2};
-    rc = vsc_stab_frame(p->stab, &in, params, out_rgba, out_float, p->cfg.solver ? VSC_FRAME_FMA : VSC_FRAME_FUSED,
-                        reinterpret_cast<vsc_hip_stream>(ss));
-    if (rc) return fail(rc);
+    float* outs[2 * PIPE_MAX_BATCH];
+    for (int i = 0; i < n; ++i) {
+        const int k = pmod(first + i, p->sets);
+        outs[2 * i] = p->lo[k][0];
+        outs[2 * i + 1] = p->lo[k][1];
+    }
+    if ((rc = flows_batch_on(p, w, n, first, outs, fs))) return fail(rc);
+    for (int i = 0; i < n; ++i)
+        if ((rc = gate_on(p, first + i, fs))) return fail(rc);
+    if (ov && (rc = flows_done(p, fs, ss))) return fail(rc);
+    for (int i = 0; i < n; ++i) {
+        const long long frame = first + i;
+        const float *ff = nullptr, *fb = nullptr;
+        if ((rc = upsample_on(p, &w[i], frame, ss, &ff, &fb))) return fail(rc);
+        vsc_frame_in in{w[i].orig_prev, w[i].orig_cur, w[i].orig_next, w[i].proc_prev, w[i].proc_cur,
+                        w[i].proc_next, ff, fb, 2};
+        rc = vsc_stab_frame(p->stab, &in, params, out_rgba ? out_rgba[i] : nullptr, out_float ? out_float[i] : nullptr,
+                            p->cfg.solver ? VSC_FRAME_FMA : VSC_FRAME_FUSED, reinterpret_cast<vsc_hip_stream>(ss));
+        if (rc) return fail(rc);
+        if (!ov) continue;
+        const int k = pmod(frame, p->sets);
+        if (check_hip(hipEventRecord(p->ev_s[k], ss), "vsc_pipeline_frame")) return fail(VSC_E_HIP);
+        p->s_rec[k] = true;
+    }
     if (!ov) return VSC_OK;
-    if (check_hip(hipEventRecord(p->ev_s[k], ss), "vsc_pipeline_frame")) return fail(VSC_E_HIP);
-    p->s_rec[k] = true;
-    // the caller's stream is ordered after the previous frame (frame - 1)
-    if (p->s_rec[1 - k] && check_hip(hipStreamWaitEvent(cs, p->ev_s[1 - k], 0), "vsc_pipeline_frame"))
+    // the caller's stream is ordered after the frame before this call's
+    const int kp = pmod(first - 1, p->sets);
+    if (p->s_rec[kp] && check_hip(hipStreamWaitEvent(cs, p->ev_s[kp], 0), "vsc_pipeline_frame"))
         return fail(VSC_E_HIP);
     return VSC_OK;
 }
@@ -448,7 +544,7 @@ extern "C" int vsc_pipeline_gate(const vsc_pipeline* p, float* gate_dev, uint32_
     hipStream_t s = as_stream(stream);
     if (p->cfg.overlap && p->f_rec && check_hip(hipStreamWaitEvent(s, p->ev_f, 0), "vsc_pipeline_gate"))
         return VSC_E_HIP;
-    const int k = p->last_parity;
+    const int k = p->last_set;
     if (gate_dev &&
         check_hip(hipMemcpyAsync(gate_dev, p->gate + 16 * k, 4, hipMemcpyDeviceToDevice, s), "vsc_pipeline_gate"))
         return VSC_E_HIP;
@@ -466,12 +562,12 @@ namespace vsc {
 // captured graph.  Returns VSC_E_INVALID when the pipeline is not in that state.
 int pipeline_advance(vsc_pipeline* p, long long frame)
 {
-    const int sp = (int)(((frame - 1) % 3 + 3) % 3), sc = (int)((frame % 3 + 3) % 3), sn = (int)(((frame + 1) % 3 + 3) % 3);
+    const int sp = pmod(frame - 1, p->slots), sc = pmod(frame, p->slots), sn = pmod(frame + 1, p->slots);
     if (p->prev_fwd_frame != frame - 1 || p->slot_frame[sp] != frame - 1 || p->slot_frame[sc] != frame)
         return set_error(VSC_E_INVALID, "pipeline_advance: not in steady state");
     p->slot_frame[sn] = frame + 1;
     p->prev_fwd_frame = frame;
-    p->last_parity = parity(frame);
+    p->last_set = pmod(frame, p->sets);
     return VSC_OK;
 }
 

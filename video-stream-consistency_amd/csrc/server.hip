@@ -313,6 +313,9 @@ extern "C" int vsc_server_create(const vsc_server_cfg* cfg, const float* upsampl
     if (cfg->form.overlap != 0)
         return set_error(VSC_E_INVALID, "vsc_server_create: form.overlap must be 0 (the server orders each stream's "
                                         "frames on its own HIP stream)");
+    if (cfg->form.flow_batch != 1)
+        return set_error(VSC_E_INVALID, "vsc_server_create: form.flow_batch must be 1 (a stream's window advances "
+                                        "one frame per tick; the tick batches across streams instead)");
     if (cfg->batch != 0 && cfg->batch != 1) return set_error(VSC_E_INVALID, "vsc_server_create: batch must be 0 or 1");
     if (cfg->form.downscale > 1 && cfg->form.upsample == 1 && !upsampler_params)
         return set_error(VSC_E_INVALID, "vsc_server_create: the learned upsampler needs its parameters");

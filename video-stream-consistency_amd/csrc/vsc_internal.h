@@ -372,9 +372,15 @@ struct ConvDesc {
     int stride, dil, pad, act;
     int ek_in, ek_out;
     float* partial;      // optional FP32 scratch: small layers split their channel loop into it
-    size_t partial_cap;  // scratch capacity in floats
+    size_t partial_cap;  // scratch capacity in floats per batch item (the scratch holds in.n items' worth)
 };
 int launch_conv3x3(const ConvDesc& d, hipStream_t s);
+
+// the flow network with room for `pairs` (1 .. 8) (first, second) pairs per
+// decode (vsc_flownet_create: 2)
+size_t flownet_footprint_pairs(int width, int height, int downscale, int dtype, int slots, int pairs);
+int flownet_create_pairs(int width, int height, int downscale, int dtype, unsigned long long seed, int slots,
+                         int pairs, vsc_flownet** out);
 
 // the fused learned upsampler on n (1 or 2) flows of one frame in one launch
 // (upsampler.hip; vsc_upsampler_run is the one-flow case)
