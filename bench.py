@@ -68,9 +68,9 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=0, help="timed frames of the plain form (default steps/3)")
     ap.add_argument("--solver", choices=["fma", "exact"], default="fma",
                     help="stabilize workload: solver in FMA form or in the reference op order")
-    ap.add_argument("--flow-batch", type=int, default=1, choices=[1, 2, 3, 4],
-                    help="pipeline workload: frames whose flows one network decode batches (chosen form, "
-                         "frame-pipelined)")
+    ap.add_argument("--flow-batch", type=int, default=2, choices=[1, 2, 3, 4],
+                    help="pipeline workload: frames whose flows one network decode batches (every "
+                         "frame-pipelined form; the reference's batchSize)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -437,6 +437,8 @@ def bench_pipeline(args):
     # scatter every warp's gathers); the same network weights serve both forms
     # throughput: frame-pipelined pipelines (the flows of frame t+1 are computed
     # while frame t is stabilized; the timed region ends with a device sync)
+    # every frame-pipelined form runs with the same flow batch (B frames' flows
+    # per network decode, the reference's batchSize; outputs identical for any B)
     B = args.flow_batch
     chosen = vsc.Pipeline(W, H, form="chosen", seed=1, flow_head_scale=FLOW_HEAD_SCALE, overlap=True, flow_batch=B)
     chosen.set_last_rgba(P[2])
@@ -446,6 +448,14 @@ def bench_pipeline(args):
     gate, count = chosen.gate()
     chosen.close()
     fps = args.steps / el_c
+    # the same with one frame's flows per decode (the reference's default batch size)
+    fps_b1 = None
+    if B != 1:
+        one = vsc.Pipeline(W, H, form="chosen", seed=1, flow_head_scale=FLOW_HEAD_SCALE, overlap=True)
+        one.set_last_rgba(P[2])
+        el_1, _ = run(one, args.steps, args.warmup, 1)
+        one.close()
+        fps_b1 = args.steps / el_1
     # serial pipeline (one stream): throughput without frame pipelining and the
     # latency of one frame (HIP events on its stream)
     chosen = vsc.Pipeline(W, H, form="chosen", seed=1, flow_head_scale=FLOW_HEAD_SCALE)
@@ -464,24 +474,24 @@ def bench_pipeline(args):
     chosen.close()
 
     plain_steps = args.plain_steps or max(10, args.steps // 3)
-    plain = vsc.Pipeline(W, H, form="plain", seed=1, flow_head_scale=FLOW_HEAD_SCALE, overlap=True)
+    plain = vsc.Pipeline(W, H, form="plain", seed=1, flow_head_scale=FLOW_HEAD_SCALE, overlap=True, flow_batch=B)
     plain.set_last_rgba(P[2])
-    el_p, _ = run(plain, plain_steps, max(3, args.warmup // 4), 1)
+    el_p, _ = run(plain, plain_steps, max(3, args.warmup // 4), 1, B=B)
     plain.close()
     fps_plain = plain_steps / el_p
     # the chosen form's flow path with an FP32 network (FP32 convolutions and cost
     # volume at FLOWDOWNSCALE 2, learned upsampler): the speed-up without 16-bit arithmetic
     half32 = vsc.Pipeline(W, H, flow_dtype="f32", downscale=2, upsample="learned", solver="fma", seed=1,
-                          flow_head_scale=FLOW_HEAD_SCALE, overlap=True)
+                          flow_head_scale=FLOW_HEAD_SCALE, overlap=True, flow_batch=B)
     half32.set_last_rgba(P[2])
-    el_h, _ = run(half32, plain_steps, max(3, args.warmup // 4), 1)
+    el_h, _ = run(half32, plain_steps, max(3, args.warmup // 4), 1, B=B)
     half32.close()
     fps_half32 = plain_steps / el_h
     # the plain form with the solver in the reference op order (bit-exact with the oracle)
     plain_x = vsc.Pipeline(W, H, form="plain", seed=1, flow_head_scale=FLOW_HEAD_SCALE, solver="exact",
-                           overlap=True)
+                           overlap=True, flow_batch=B)
     plain_x.set_last_rgba(P[2])
-    el_x, _ = run(plain_x, plain_steps, max(3, args.warmup // 4), 1)
+    el_x, _ = run(plain_x, plain_steps, max(3, args.warmup // 4), 1, B=B)
     plain_x.close()
     fps_exact = plain_steps / el_x
 
@@ -506,12 +516,16 @@ def bench_pipeline(args):
                                "stabilization (2-level pyramid, 75 + 150 blocked sweeps), RGBA8 out",
                    "form": "chosen: BF16 PWCNet-light at FLOWDOWNSCALE=2, learned FP16 upsampler with the "
                            "fwd-bwd fallback, FP32 stabilization (FMA-form solver)",
-                   "execution": "frame-pipelined on two HIP streams (flows of frame t+1 during the "
-                                "stabilization of frame t); every form measured the same way",
+                   "execution": "frame-pipelined on two HIP streams (the flows of the next batch of "
+                                "flow_batch frames during the stabilization of the current one); every "
+                                "form measured the same way",
                    "flow_batch": B,
                    "width": W, "height": H, "pyramid_levels": 2, "num_iter": 150},
+        "chosen_flow_batch_1": None if fps_b1 is None else {
+            "value": round(fps_b1, 2), "unit": "frames/s",
+            "form": "the chosen form, frame-pipelined, one frame's flows per network decode"},
         "chosen_serial": {"value": round(fps_serial, 2), "unit": "frames/s",
-                          "form": "the chosen form on one stream (no frame pipelining)"},
+                          "form": "the chosen form on one stream (no frame pipelining, one frame per call)"},
         "plain_form": {"value": round(fps_plain, 2), "unit": "frames/s", "steps": plain_steps,
                        "ms_per_step": round(1000.0 * el_p / plain_steps, 4),
                        "form": "FP32 PWCNet-light at full resolution, FP32 stabilization (FMA-form solver)"},

@@ -72,11 +72,12 @@ struct Layer {
 struct vsc_flownet {
     int W, H, down, Wd, Hd, Wp, Hp;
     int ek, eb, vec, slots, pairs;       // pairs: the most (first, second) pairs one decode batches
+    int frames;                          // the most frames one extraction batches
     std::vector<vsc::Layer> L;
     int ext0, up0, dec0, ref0;            // first layer index of each group
-    void* img = nullptr;                  // [Hp][Wp][vec]
-    void* tmp[2] = {nullptr, nullptr};    // extractor scratch (level-1 size)
-    std::vector<void*> feat;              // [slot*4 + (level-3)], levels 3..6
+    void* img = nullptr;                  // [frames][Hp][Wp][vec]
+    void* tmp[2] = {nullptr, nullptr};    // extractor scratch, [frames][level-1 size]
+    void* feat[7] = {};                   // levels 3..6: [slots][h][w][CV], slot-major
     void* cat0[7] = {};                   // per level [pairs][h][w][ccat]
     void* dec[7] = {};                    // per level [pairs][h][w][kDecC]
     void* warped = nullptr;               // [pairs][h3][w3][CV max]
@@ -203,8 +204,8 @@ Sizes sizes_of(const vsc_flownet* n)
 {
     Sizes z{};
     const size_t eb = n->eb;
-    z.img = (size_t)n->Hp * n->Wp * n->vec * eb;
-    z.tmp = (size_t)(n->Hp / 2) * (n->Wp / 2) * rup(16, n->vec) * eb;
+    z.img = (size_t)n->frames * n->Hp * n->Wp * n->vec * eb;
+    z.tmp = (size_t)n->frames * (n->Hp / 2) * (n->Wp / 2) * rup(16, n->vec) * eb;
     z.feat_slot = 0;
     for (int lvl = 3; lvl <= 6; ++lvl) z.feat_slot += (size_t)(n->Hp >> lvl) * (n->Wp >> lvl) * cv_of(n, lvl) * eb;
     for (int lvl = 3; lvl <= 6; ++lvl) {
@@ -252,9 +253,8 @@ int alloc_all(vsc_flownet* n)
     n->img = dalloc(n, z.img, rc);
     n->tmp[0] = dalloc(n, z.tmp, rc);
     n->tmp[1] = dalloc(n, z.tmp, rc);
-    for (int s = 0; s < n->slots; ++s)
-        for (int lvl = 3; lvl <= 6; ++lvl)
-            n->feat.push_back(dalloc(n, (size_t)(n->Hp >> lvl) * (n->Wp >> lvl) * cv_of(n, lvl) * n->eb, rc));
+    for (int lvl = 3; lvl <= 6; ++lvl)
+        n->feat[lvl] = dalloc(n, (size_t)n->slots * (n->Hp >> lvl) * (n->Wp >> lvl) * cv_of(n, lvl) * n->eb, rc);
     for (int lvl = 3; lvl <= 6; ++lvl) {
         n->cat0[lvl] = dalloc(n, z.cat0[lvl], rc);
         n->dec[lvl] = dalloc(n, z.dec[lvl], rc);
@@ -273,7 +273,10 @@ int alloc_all(vsc_flownet* n)
     return rc;
 }
 
-void* feat_ptr(vsc_flownet* n, int slot, int lvl) { return n->feat[(size_t)slot * 4 + (lvl - 3)]; }
+void* feat_ptr(vsc_flownet* n, int slot, int lvl)
+{
+    return static_cast<char*>(n->feat[lvl]) + (size_t)slot * (n->Hp >> lvl) * (n->Wp >> lvl) * cv_of(n, lvl) * n->eb;
+}
 
 int conv(vsc_flownet* n, const Layer& l, Nhwc in, Nhwc out, Nhwc add, int ek_out, hipStream_t s, Nhwc out2 = Nhwc{})
 {
@@ -292,11 +295,17 @@ int conv(vsc_flownet* n, const Layer& l, Nhwc in, Nhwc out, Nhwc add, int ek_out
 
 Nhwc T(void* p, int N, int h, int w, int c, int cs, int c0) { return Nhwc{p, N, h, w, c, cs, c0}; }
 
-int extract(vsc_flownet* n, const uint8_t* rgba, int slot, hipStream_t s)
+// features of m frames into slots slot .. slot+m-1 (one batch through every
+// extractor layer; each frame's features have the same bits as alone)
+int extract(vsc_flownet* n, const uint8_t* const* rgba, int m, int slot, hipStream_t s)
 {
-    int rc = launch_preprocess(n->ek, rgba, n->W, n->H, n->Wd, n->Hd, n->Wp, n->Hp, n->img, n->vec, s);
-    if (rc) return rc;
-    Nhwc x = T(n->img, 1, n->Hp, n->Wp, n->vec, n->vec, 0);
+    int rc;
+    const size_t img_item = (size_t)n->Hp * n->Wp * n->vec * n->eb;
+    for (int i = 0; i < m; ++i)
+        if ((rc = launch_preprocess(n->ek, rgba[i], n->W, n->H, n->Wd, n->Hd, n->Wp, n->Hp,
+                                    static_cast<char*>(n->img) + i * img_item, n->vec, s)))
+            return rc;
+    Nhwc x = T(n->img, m, n->Hp, n->Wp, n->vec, n->vec, 0);
     for (int k = 1; k <= 6; ++k) {
         const int h = n->Hp >> k, w = n->Wp >> k, cv = cv_of(n, k);
         const Layer* l = &n->L[n->ext0 + 3 * (k - 1)];
@@ -308,10 +317,10 @@ int extract(vsc_flownet* n, const uint8_t* rgba, int slot, hipStream_t s)
         // outputs carry the layer's channel count; as inputs the slices include
         // the zero padding channels up to the 16-byte multiple
         const int co = kFeatC[k];
-        if ((rc = conv(n, l[0], x, T(a, 1, h, w, co, cv, 0), Nhwc{}, n->ek, s))) return rc;
-        if ((rc = conv(n, l[1], T(a, 1, h, w, cv, cv, 0), T(b, 1, h, w, co, cv, 0), Nhwc{}, n->ek, s))) return rc;
-        if ((rc = conv(n, l[2], T(b, 1, h, w, cv, cv, 0), T(c, 1, h, w, co, cv, 0), Nhwc{}, n->ek, s))) return rc;
-        x = T(c, 1, h, w, cv, cv, 0);
+        if ((rc = conv(n, l[0], x, T(a, m, h, w, co, cv, 0), Nhwc{}, n->ek, s))) return rc;
+        if ((rc = conv(n, l[1], T(a, m, h, w, cv, cv, 0), T(b, m, h, w, co, cv, 0), Nhwc{}, n->ek, s))) return rc;
+        if ((rc = conv(n, l[2], T(b, m, h, w, cv, cv, 0), T(c, m, h, w, co, cv, 0), Nhwc{}, n->ek, s))) return rc;
+        x = T(c, m, h, w, cv, cv, 0);
     }
     return VSC_OK;
 }
@@ -384,6 +393,7 @@ static int flownet_dims(int W, int H, int down, int& Wd, int& Hd, int& Wp, int& 
 
 static vsc_flownet* flownet_shell(int W, int H, int down, int dtype, int slots, int pairs, int& rc)
 {
+    const int frames = (pairs + 1) / 2;
     int Wd, Hd, Wp, Hp;
     rc = flownet_dims(W, H, down, Wd, Hd, Wp, Hp);
     if (rc) return nullptr;
@@ -404,6 +414,7 @@ static vsc_flownet* flownet_shell(int W, int H, int down, int dtype, int slots, 
     n->W = W; n->H = H; n->down = down; n->Wd = Wd; n->Hd = Hd; n->Wp = Wp; n->Hp = Hp;
     n->ek = elem_kind(dtype); n->eb = elem_bytes(n->ek); n->vec = n->ek == EK_F32 ? 4 : 8; n->slots = slots;
     n->pairs = pairs;
+    n->frames = frames;
     build_layers(n);
     return n;
 }
@@ -508,11 +519,23 @@ extern "C" int vsc_flownet_scale_flow_heads(vsc_flownet* n, float factor)
     return upload_params(n);
 }
 
-extern "C" int vsc_flownet_extract(vsc_flownet* n, const uint8_t* rgba, int slot, vsc_hip_stream stream)
+namespace vsc {
+
+int flownet_extract_frames(vsc_flownet* n, const uint8_t* const* rgba, int m, int slot, hipStream_t s)
 {
     if (!n || !rgba) return set_error(VSC_E_INVALID, "vsc_flownet_extract: NULL argument");
-    if (slot < 0 || slot >= n->slots) return set_error(VSC_E_INVALID, "vsc_flownet_extract: slot out of range");
-    return extract(n, rgba, slot, as_stream(stream));
+    if (m < 1 || m > n->frames) return set_error(VSC_E_INVALID, "vsc_flownet_extract: batch beyond the network's");
+    if (slot < 0 || slot + m > n->slots) return set_error(VSC_E_INVALID, "vsc_flownet_extract: slot out of range");
+    for (int i = 0; i < m; ++i)
+        if (!rgba[i]) return set_error(VSC_E_INVALID, "vsc_flownet_extract: NULL argument");
+    return extract(n, rgba, m, slot, s);
+}
+
+}  // namespace vsc
+
+extern "C" int vsc_flownet_extract(vsc_flownet* n, const uint8_t* rgba, int slot, vsc_hip_stream stream)
+{
+    return flownet_extract_frames(n, &rgba, 1, slot, as_stream(stream));
 }
 
 extern "C" int vsc_flownet_flow(vsc_flownet* n, int npairs, const int* first, const int* second, float* const* out,

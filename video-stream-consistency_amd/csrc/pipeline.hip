@@ -18,6 +18,7 @@
 // flows of all its frames in one network batch of 2n pairs.
 #include "vsc_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <new>
 
@@ -240,16 +241,6 @@ extern "C" int vsc_pipeline_stream(vsc_pipeline* p, vsc_stream** s)
     return VSC_OK;
 }
 
-static int ensure_features(vsc_pipeline* p, const uint8_t* rgba, long long frame, hipStream_t s)
-{
-    const int slot = pmod(frame, p->slots);
-    if (p->slot_frame[slot] == frame) return VSC_OK;
-    int rc = vsc_flownet_extract(p->net, rgba, slot, reinterpret_cast<vsc_hip_stream>(s));
-    if (rc) return rc;
-    p->slot_frame[slot] = frame;
-    return VSC_OK;
-}
-
 // the forward and backward pairs of window `frame` (feature slots of frames
 // frame-1 .. frame+1, extracted before)
 static void frame_pairs(const vsc_pipeline* p, long long frame, int* first, int* second)
@@ -269,11 +260,25 @@ static int flows_batch_on(vsc_pipeline* p, const vsc_window* w, int n, long long
                           hipStream_t s)
 {
     int rc;
-    for (int i = 0; i < n; ++i)
-        if ((rc = ensure_features(p, w[i].orig_prev, first + i - 1, s)) ||
-            (rc = ensure_features(p, w[i].orig_cur, first + i, s)) ||
-            (rc = ensure_features(p, w[i].orig_next, first + i + 1, s)))
-            return rc;
+    // frames first-1 .. first+n: the ones not cached are extracted in batches
+    // of consecutive frames whose slots do not wrap
+    const uint8_t* rgba[PIPE_MAX_BATCH + 2];
+    rgba[0] = w[0].orig_prev;
+    for (int i = 0; i < n; ++i) rgba[i + 1] = w[i].orig_cur;
+    rgba[n + 1] = w[n - 1].orig_next;
+    for (int j = 0; j < n + 2;) {
+        const long long f = first - 1 + j;
+        if (p->slot_frame[pmod(f, p->slots)] == f) {
+            ++j;
+            continue;
+        }
+        int m = 1;
+        while (j + m < n + 2 && pmod(f + m, p->slots) != 0 && p->slot_frame[pmod(f + m, p->slots)] != f + m) ++m;
+        m = std::min(m, p->cfg.flow_batch);
+        if ((rc = flownet_extract_frames(p->net, rgba + j, m, pmod(f, p->slots), s))) return rc;
+        for (int i = 0; i < m; ++i) p->slot_frame[pmod(f + i, p->slots)] = f + i;
+        j += m;
+    }
     int a[2 * PIPE_MAX_BATCH], b[2 * PIPE_MAX_BATCH];
     for (int i = 0; i < n; ++i) frame_pairs(p, first + i, a + 2 * i, b + 2 * i);
     return vsc_flownet_flow(p->net, 2 * n, a, b, outs, 2, reinterpret_cast<vsc_hip_stream>(s));

@@ -381,6 +381,8 @@ int launch_conv3x3(const ConvDesc& d, hipStream_t s);
 size_t flownet_footprint_pairs(int width, int height, int downscale, int dtype, int slots, int pairs);
 int flownet_create_pairs(int width, int height, int downscale, int dtype, unsigned long long seed, int slots,
                          int pairs, vsc_flownet** out);
+// features of m (<= ceil(pairs / 2)) frames into slots slot .. slot+m-1, one batch
+int flownet_extract_frames(vsc_flownet* n, const uint8_t* const* rgba, int m, int slot, hipStream_t s);
 
 // the fused learned upsampler on n (1 or 2) flows of one frame in one launch
 // (upsampler.hip; vsc_upsampler_run is the one-flow case)
