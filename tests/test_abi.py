@@ -99,6 +99,33 @@ def test_default_cfgs_fill_every_field(vsc):
     assert L.vsc_pipeline_footprint(640, 360, ctypes.byref(c)) > 0
 
 
+def test_pipeline_flow_batch_config(vsc):
+    """flow_batch defaults to 1; 1..4 are accepted with a footprint that grows
+    with the batch (more feature slots, flow buffer sets and decode room);
+    other values make the configuration invalid; the server takes only 1."""
+    L = vsc.lib()
+    c = vsc.PipelineCfg()
+    L.vsc_pipeline_default_cfg(ctypes.byref(c))
+    assert c.flow_batch == 1
+    c.flow_dtype, c.downscale, c.upsample = vsc.VSC_BF16, 2, 1
+    sizes = []
+    for b in (1, 2, 3, 4):
+        c.flow_batch = b
+        sizes.append(L.vsc_pipeline_footprint(1920, 1080, ctypes.byref(c)))
+    assert all(x > 0 for x in sizes) and sizes == sorted(sizes) and len(set(sizes)) == 4
+    for b in (0, 5):
+        c.flow_batch = b
+        assert L.vsc_pipeline_footprint(1920, 1080, ctypes.byref(c)) == 0
+    s = vsc.ServerCfg()
+    L.vsc_server_default_cfg(ctypes.byref(s))
+    assert s.form.flow_batch == 1
+    s.form.flow_batch = 2
+    h = ctypes.c_void_p()
+    ups = (ctypes.c_float * L.vsc_upsampler_param_count())()
+    assert L.vsc_server_create(ctypes.byref(s), ups, ctypes.byref(h)) == -1
+    assert b"flow_batch" in L.vsc_last_error()
+
+
 def test_server_rejects_overlap_form(vsc):
     L = vsc.lib()
     c = vsc.ServerCfg()
