@@ -473,7 +473,9 @@ int vsc_pipeline_frame(vsc_pipeline* p, const vsc_window* w, long long frame, co
                        uint8_t* out_rgba, float* out_float, vsc_hip_stream stream);
 /* n (1 .. flow_batch) consecutive stabilized frames: w[i] is the window of
  * frame first + i (the reference's doOneStep calls for currentFrame .. +B-1,
- * whose frame window holds 2k+B frames).  The flows of all n frames come from
+ * whose frame window holds 2k+B frames).  Consecutive windows share frames;
+ * the network reads each original frame once, keyed by its index (prev of
+ * w[0], cur of every window, next of w[n-1]), like vsc_pipeline_flows.  The flows of all n frames come from
  * one batched decode of 2n pairs, then the frames are stabilized in order;
  * the results equal n vsc_pipeline_frame calls.  out_rgba / out_float: n
  * pointers each (the arrays, or their entries, may be NULL).  With overlap =
