@@ -487,6 +487,14 @@ def bench_pipeline(args):
     el_h, _ = run(half32, plain_steps, max(3, args.warmup // 4), 1, B=B)
     half32.close()
     fps_half32 = plain_steps / el_h
+    # the north star's BF16-correlation form: the same FP32 network with the cost
+    # volume's products on the BF16 MFMA, learned upsampler
+    bfc = vsc.Pipeline(W, H, flow_dtype="f32", downscale=2, upsample="learned", solver="fma", seed=1,
+                       flow_head_scale=FLOW_HEAD_SCALE, overlap=True, flow_batch=B, corr_dtype="bf16")
+    bfc.set_last_rgba(P[2])
+    el_b, _ = run(bfc, plain_steps, max(3, args.warmup // 4), 1, B=B)
+    bfc.close()
+    fps_bfc = plain_steps / el_b
     # the plain form with the solver in the reference op order (bit-exact with the oracle)
     plain_x = vsc.Pipeline(W, H, form="plain", seed=1, flow_head_scale=FLOW_HEAD_SCALE, solver="exact",
                            overlap=True, flow_batch=B)
@@ -535,6 +543,12 @@ def bench_pipeline(args):
             "form": "FP32 PWCNet-light at FLOWDOWNSCALE=2 (FP32 convolutions and cost volume), learned FP16 "
                     "upsampler, FP32 stabilization (FMA-form solver)",
             "speedup_vs_plain": round(fps_half32 / fps_plain, 3)},
+        "bf16_correlation_form": {
+            "value": round(fps_bfc, 2), "unit": "frames/s", "steps": plain_steps,
+            "form": "FP32 PWCNet-light at FLOWDOWNSCALE=2 with the cost volume's products on the BF16 MFMA "
+                    "(FP32 accumulation; every other layer FP32), learned FP16 upsampler, FP32 stabilization "
+                    "(FMA-form solver)",
+            "speedup_vs_plain": round(fps_bfc / fps_plain, 3)},
         "plain_form_exact_solver": {"value": round(fps_exact, 2), "unit": "frames/s", "steps": plain_steps,
                                     "ms_per_step": round(1000.0 * el_x / plain_steps, 4),
                                     "form": "the plain form with the solver in the reference op order "

@@ -338,6 +338,13 @@ int vsc_flownet_destroy(vsc_flownet* net);
 size_t vsc_flownet_footprint(int frame_width, int frame_height, int downscale, int dtype, int slots);
 /* flow resolution (round(W/f), round(H/f)) */
 int vsc_flownet_flow_size(const vsc_flownet* net, int* width, int* height);
+/* Element type of the cost volume's products (no counterpart in the
+ * reference, whose ONNX graph runs in FP32): VSC_BF16 in an FP32 network
+ * rounds the features to BF16 as the cost volume stages them and multiplies
+ * on the BF16 MFMA with FP32 accumulation (the north star's BF16-correlation
+ * form); every other layer stays FP32.  VSC_F32 (default) keeps FP32
+ * products.  A BF16 network accepts only VSC_BF16.  Errors: VSC_E_DTYPE. */
+int vsc_flownet_set_corr_dtype(vsc_flownet* net, int dtype);
 /* parameters as float32 in layer order (extractor 18 convs, upflow
  * ConvTranspose2d of levels 5, 4, 3, decoder convs of levels 6, 5, 4, 3 (6 each),
  * refiner 7 convs), each weight in torch layout then its bias. */
@@ -444,6 +451,8 @@ typedef struct {
                                    one batched network decode (the reference's batchSize: FlowModel::run
                                    over B frame pairs every B frames, videostabilizer.cpp:65-71,177,
                                    267-279).  Each flow has the same bits in any batch. */
+    int corr_dtype;             /* cost volume products: VSC_F32 (default) or VSC_BF16 (in an FP32 network:
+                                   the BF16-correlation form, vsc_flownet_set_corr_dtype) */
 } vsc_pipeline_cfg;
 
 /* The k = 1 frame window (device RGBA8 frames, 4-byte aligned). */

@@ -9,7 +9,9 @@ the largest magnitude of the reference tensor.
 
 Tolerances: FP32 storage 2e-5 (float32 through ~30 conv layers; measured
 errors ~3e-6, printed); BF16 storage 4e-2 (bfloat16 activations, 2^-9 relative
-rounding per layer).
+rounding per layer); FP32 storage with the cost volume's products on the BF16
+MFMA 4e-2 (the features rounded to bfloat16 before the dot products of each
+level, everything else FP32).
 """
 import numpy as np
 import pytest
@@ -18,7 +20,7 @@ from oracle import pwcnet_light as pw
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"f32": 2e-5, "bf16": 4e-2}
+TOL = {"f32": 2e-5, "bf16": 4e-2, "f32+bf16corr": 4e-2}
 
 
 def _frames(vsc, W, H, seed=3):
@@ -32,7 +34,7 @@ def _rel(dev, ref):
     return float(np.max(np.abs(np.asarray(dev, np.float64) - ref)) / (np.max(np.abs(ref)) + 1e-3))
 
 
-@pytest.mark.parametrize("kind", ["f32", "bf16"])
+@pytest.mark.parametrize("kind", ["f32", "bf16", "f32+bf16corr"])
 @pytest.mark.parametrize("W,H,down", [(192, 128, 1), (200, 136, 2), (320, 256, 1)])
 def test_flownet_levels(vsc, cuda, kind, W, H, down):
     import torch
@@ -68,6 +70,31 @@ def test_flownet_levels(vsc, cuda, kind, W, H, down):
               f"  |flow|max={np.abs(fin).max():.2f}")
         worst = max(worst, max(errs.values()))
     assert worst <= TOL[kind]
+
+
+def test_flownet_bf16_cost_volume_between_fp32_and_bf16(vsc, cuda):
+    """The BF16 cost volume in an FP32 network changes the flows (its products
+    are BF16), by less than the all-BF16 network does; a BF16 network refuses
+    FP32 cost-volume products."""
+    import torch
+    W, H = 320, 256
+    orig = _frames(vsc, W, H)
+    fr = torch.from_numpy(orig).to(cuda)
+    flows = {}
+    for kind in ("f32", "f32+bf16corr", "bf16"):
+        net = vsc.FlowNet(W, H, 1, kind, seed=11, slots=3)
+        net.extract(fr[0], 0)
+        net.extract(fr[1], 1)
+        flows[kind] = net.flow([(0, 1)])[0].cpu().numpy().astype(np.float64)
+        if kind == "bf16":
+            assert vsc.lib().vsc_flownet_set_corr_dtype(net.h, vsc.VSC_F32) == vsc.VSC_E_DTYPE
+        net.close()
+    ref = flows["f32"]
+    scale = np.abs(ref).max() + 1e-3
+    e_mix = np.abs(flows["f32+bf16corr"] - ref).max() / scale
+    e_bf = np.abs(flows["bf16"] - ref).max() / scale
+    print(f"vs FP32 network: BF16 cost volume {e_mix:.2e}, BF16 network {e_bf:.2e}")
+    assert 0.0 < e_mix < e_bf
 
 
 @pytest.mark.parametrize("kind", ["f32", "bf16"])

@@ -49,7 +49,7 @@ EXPORTED = (
     "vsc_stab_consist_out_planar", "vsc_conv3x3_nhwc", "vsc_flownet_create", "vsc_flownet_destroy",
     "vsc_flownet_footprint", "vsc_flownet_flow_size", "vsc_flownet_param_count", "vsc_flownet_get_params",
     "vsc_flownet_set_params", "vsc_flownet_extract", "vsc_flownet_flow", "vsc_flownet_intermediate",
-    "vsc_flownet_scale_flow_heads",
+    "vsc_flownet_scale_flow_heads", "vsc_flownet_set_corr_dtype",
     "vsc_upsampler_create", "vsc_upsampler_destroy", "vsc_upsampler_param_count", "vsc_upsampler_get_params",
     "vsc_upsampler_set_params", "vsc_upsampler_workspace_size", "vsc_upsampler_run", "vsc_upsampler_train_step",
     "vsc_upsampler_loss_grad", "vsc_flow_fb_gate", "vsc_pipeline_default_cfg", "vsc_pipeline_create",
@@ -91,7 +91,7 @@ class PipelineCfg(ctypes.Structure):
     _fields_ = [("flow_dtype", ctypes.c_int), ("downscale", ctypes.c_int), ("upsample", ctypes.c_int),
                 ("scale_vectors", ctypes.c_int), ("fb_max_inconsistent", ctypes.c_float),
                 ("ref_bwd_index", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("solver", ctypes.c_int),
-                ("overlap", ctypes.c_int), ("flow_batch", ctypes.c_int)]
+                ("overlap", ctypes.c_int), ("flow_batch", ctypes.c_int), ("corr_dtype", ctypes.c_int)]
 
 
 class Window(ctypes.Structure):
@@ -193,6 +193,7 @@ def lib():
         L.vsc_flownet_flow.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(vp), i, vp]
         L.vsc_flownet_intermediate.argtypes = [vp, i, i, i, vp, vp]
         L.vsc_flownet_scale_flow_heads.argtypes = [vp, f]
+        L.vsc_flownet_set_corr_dtype.argtypes = [vp, i]
         L.vsc_upsampler_create.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(vp)]
         L.vsc_upsampler_destroy.argtypes = [vp]
         L.vsc_upsampler_param_count.argtypes = []
@@ -313,13 +314,17 @@ class FlowNet:
 
     Frames are device RGBA8 tensors [H, W, 4]; extract() caches a frame's
     features in a slot, flow() runs the decoders for one or two (first,
-    second) slot pairs and returns HWC float flows at flow_size."""
+    second) slot pairs and returns HWC float flows at flow_size.  dtype
+    "f32+bf16corr": an FP32 network whose cost volume multiplies on the BF16
+    MFMA."""
 
     def __init__(self, width, height, downscale=1, dtype="f32", seed=0, slots=4):
-        self.dtype = {"f32": VSC_F32, "bf16": VSC_BF16}[dtype]
+        self.dtype = {"f32": VSC_F32, "bf16": VSC_BF16, "f32+bf16corr": VSC_F32}[dtype]
         h = ctypes.c_void_p()
         check(lib().vsc_flownet_create(width, height, downscale, self.dtype, seed, slots, ctypes.byref(h)))
         self.h = h
+        if dtype == "f32+bf16corr":
+            check(lib().vsc_flownet_set_corr_dtype(self.h, VSC_BF16))
         w_, h_ = ctypes.c_int(), ctypes.c_int()
         check(lib().vsc_flownet_flow_size(self.h, ctypes.byref(w_), ctypes.byref(h_)))
         self.flow_size = (w_.value, h_.value)
@@ -816,7 +821,7 @@ class Pipeline:
 
     def __init__(self, width, height, form=None, flow_dtype="f32", downscale=1, upsample="bilinear",
                  scale_vectors=False, fb_max_inconsistent=0.25, ref_bwd_index=False, seed=0, upsampler_params=None,
-                 flow_head_scale=None, solver=None, overlap=False, flow_batch=1):
+                 flow_head_scale=None, solver=None, overlap=False, flow_batch=1, corr_dtype=None):
         if form is not None:
             kw = self.FORMS[form]
             flow_dtype, downscale, upsample = kw["flow_dtype"], kw["downscale"], kw["upsample"]
@@ -833,6 +838,7 @@ class Pipeline:
         c.solver = SOLVERS[solver or "exact"]
         c.overlap = int(bool(overlap))
         c.flow_batch = int(flow_batch)
+        c.corr_dtype = {None: VSC_F32, "f32": VSC_F32, "bf16": VSC_BF16}[corr_dtype]
         self.cfg = c
         self.width, self.height = width, height
         h = ctypes.c_void_p()

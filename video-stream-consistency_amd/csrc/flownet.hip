@@ -73,6 +73,7 @@ struct vsc_flownet {
     int W, H, down, Wd, Hd, Wp, Hp;
     int ek, eb, vec, slots, pairs;       // pairs: the most (first, second) pairs one decode batches
     int frames;                          // the most frames one extraction batches
+    bool corr_bf16 = false;              // FP32 network: cost volume products on the BF16 MFMA
     std::vector<vsc::Layer> L;
     int ext0, up0, dec0, ref0;            // first layer index of each group
     void* img = nullptr;                  // [frames][Hp][Wp][vec]
@@ -338,7 +339,7 @@ int decode(vsc_flownet* n, int np, const int* first, const int* second, hipStrea
         void* cat = n->cat0[lvl];
         if (lvl == 6) {
             // level 6 has no upflow: its channels of cat0 stay as zeroed at create
-            if ((rc = launch_corr_nhwc(ek, f1, f2, CV, CV, CV, np, h, w, cat, cc, 88, s))) return rc;
+            if ((rc = launch_corr_nhwc(ek, f1, f2, CV, CV, CV, np, h, w, cat, cc, 88, s, n->corr_bf16))) return rc;
         } else {
             // upflow of the coarser level's flow, computed inside the warp of f2 by it
             const Layer& up = n->L[n->up0 + (5 - lvl)];
@@ -346,7 +347,7 @@ int decode(vsc_flownet* n, int np, const int* first, const int* second, hipStrea
             if ((rc = launch_warp_nhwc(ek, f2, CV, nullptr, 2, kDbl[lvl], n->warped, CV, CV, np, h, w, s, &ua))) return rc;
             const void* wp[FLOWNET_MAX_PAIRS] = {};
             for (int i = 0; i < np; ++i) wp[i] = static_cast<const char*>(n->warped) + i * px * CV * n->eb;
-            if ((rc = launch_corr_nhwc(ek, f1, wp, CV, CV, CV, np, h, w, cat, cc, 88, s))) return rc;
+            if ((rc = launch_corr_nhwc(ek, f1, wp, CV, CV, CV, np, h, w, cat, cc, 88, s, n->corr_bf16))) return rc;
         }
         // cat0 = [leaky(corr) | pad | f1 | upflow | pad]: the cost volume kernel wrote f1 at channel 88
         const Layer* l = &n->L[n->dec0 + 6 * (6 - lvl)];
@@ -532,6 +533,17 @@ int flownet_extract_frames(vsc_flownet* n, const uint8_t* const* rgba, int m, in
 }
 
 }  // namespace vsc
+
+extern "C" int vsc_flownet_set_corr_dtype(vsc_flownet* n, int dtype)
+{
+    if (!n) return set_error(VSC_E_INVALID, "vsc_flownet_set_corr_dtype: NULL network");
+    if (dtype != VSC_F32 && dtype != VSC_BF16)
+        return set_error(VSC_E_DTYPE, "vsc_flownet_set_corr_dtype: dtype must be VSC_F32 or VSC_BF16");
+    if (n->ek == EK_BF16 && dtype != VSC_BF16)
+        return set_error(VSC_E_DTYPE, "vsc_flownet_set_corr_dtype: a BF16 network has a BF16 cost volume");
+    n->corr_bf16 = n->ek == EK_F32 && dtype == VSC_BF16;
+    return VSC_OK;
+}
 
 extern "C" int vsc_flownet_extract(vsc_flownet* n, const uint8_t* rgba, int slot, vsc_hip_stream stream)
 {

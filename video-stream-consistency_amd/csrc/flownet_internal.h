@@ -11,9 +11,10 @@ namespace vsc {
 constexpr int FLOWNET_MAX_PAIRS = 8;
 
 // f1_c0 >= 0: also copies f1 (C channels) into `out` at channel f1_c0;
-// f1, f2: N <= FLOWNET_MAX_PAIRS items
+// f1, f2: N <= FLOWNET_MAX_PAIRS items; bf16_products (FP32 storage): the
+// products on the BF16 MFMA (features rounded to BF16 as staged)
 int launch_corr_nhwc(int ek, const void* const* f1, const void* const* f2, int cs1, int cs2, int C, int N, int H,
-                     int W, void* out, int ocs, int f1_c0, hipStream_t s);
+                     int W, void* out, int ocs, int f1_c0, hipStream_t s, bool bf16_products = false);
 // upflow fused into the warp (moduleUpflow, pwcnet.py:472, then Backward)
 struct UpflowArgs {
     const float* in;           // coarser flow, [N][H/2][W/2][2] FP32

@@ -51,22 +51,29 @@ __device__ inline fo_f32x4 corr_mma(const fo_u32x4& a, const fo_u32x4& b, fo_f32
     }
 }
 
-template <int EK>
+// BFMM (FP32 storage only): the FP32 features are rounded to BF16 as they are
+// staged and multiplied on the BF16 MFMA (FP32 accumulation): the BF16 cost
+// volume of an otherwise FP32 network.
+template <int EK, bool BFMM>
 __global__ __launch_bounds__(512) void k_corr_nhwc(CorrArgs a)
 {
+    static_assert(!BFMM || EK == EK_F32, "BF16 products of FP32 storage only");
     using E = Elem<EK>;
     using T = typename E::T;
-    constexpr int VEC = E::VEC, KC = CorrK<EK>::KC, KV = KC / VEC, ROW = KC + VEC;
+    constexpr int MK = BFMM ? EK_BF16 : EK;             // element kind of the LDS operands and the MFMA
+    using M = typename Elem<MK>::T;
+    constexpr int VEC = E::VEC, KC = CorrK<MK>::KC, KV = KC / VEC;
+    constexpr int MVEC = Elem<MK>::VEC, ROW = KC + MVEC;
     constexpr int RR = 4, RP = 32, HALO = 4, BR = RR + 2 * HALO, BP = RP + 2 * HALO;
     constexpr int NA = RR * RP * KV, NB = BR * BP * KV, NU = NA + NB, RU = (NU + 511) / 512;
     constexpr int OPX = 88 / VEC;                       // output vectors per pixel (81 + 7 zeros)
     constexpr int SROW = 89;                            // stage floats per pixel (odd: fewer conflicts)
-    constexpr int OPER_BYTES = (RR * RP + BR * BP) * ROW * (int)sizeof(T);
+    constexpr int OPER_BYTES = (RR * RP + BR * BP) * ROW * (int)sizeof(M);
     constexpr int STAGE_BYTES = RR * RP * SROW * 4;
     constexpr int LDS = OPER_BYTES > STAGE_BYTES ? OPER_BYTES : STAGE_BYTES;
     __shared__ __attribute__((aligned(16))) unsigned char lds[LDS];
-    T* sA = reinterpret_cast<T*>(lds);
-    T* sB = sA + RR * RP * ROW;
+    M* sA = reinterpret_cast<M*>(lds);
+    M* sB = sA + RR * RP * ROW;
     float* stage = reinterpret_cast<float*>(lds);
 
     const int n = blockIdx.z, h0 = blockIdx.y * RR, w0 = blockIdx.x * RP;
@@ -114,8 +121,18 @@ __global__ __launch_bounds__(512) void k_corr_nhwc(CorrArgs a)
 #pragma unroll
         for (int k = 0; k < RU; ++k) {
             const int u = tid + 512 * k;
-            if (u < NA) *reinterpret_cast<fo_u32x4*>(&sA[(u / KV) * ROW + (u % KV) * VEC]) = rg[k];
-            else if (u < NU) *reinterpret_cast<fo_u32x4*>(&sB[((u - NA) / KV) * ROW + ((u - NA) % KV) * VEC]) = rg[k];
+            if (u >= NU) continue;
+            M* dst = u < NA ? &sA[(u / KV) * ROW + (u % KV) * VEC] : &sB[((u - NA) / KV) * ROW + ((u - NA) % KV) * VEC];
+            if constexpr (BFMM) {
+                // 4 FP32 channels -> 4 BF16 (round to nearest even)
+                const fo_f32x4 f = __builtin_bit_cast(fo_f32x4, rg[k]);
+                using B = Elem<EK_BF16>;
+                *reinterpret_cast<uint2*>(dst) =
+                    make_uint2((uint32_t)B::st(f[0]) | ((uint32_t)B::st(f[1]) << 16),
+                               (uint32_t)B::st(f[2]) | ((uint32_t)B::st(f[3]) << 16));
+            } else {
+                *reinterpret_cast<fo_u32x4*>(dst) = rg[k];
+            }
         }
         __syncthreads();
         if (c0 + KC < a.C) fetch(c0 + KC);
@@ -126,7 +143,7 @@ __global__ __launch_bounds__(512) void k_corr_nhwc(CorrArgs a)
             for (int t = 0; t < 2; ++t) {
                 const fo_u32x4 fb = *reinterpret_cast<const fo_u32x4*>(
                     &sB[((qr + d) * BP + 16 * seg + 8 * t + l16) * ROW + lg * (KC / 4)]);
-                acc[d][t] = corr_mma<EK>(fa, fb, acc[d][t]);
+                acc[d][t] = corr_mma<MK>(fa, fb, acc[d][t]);
             }
         }
     }
@@ -169,7 +186,7 @@ __global__ __launch_bounds__(512) void k_corr_nhwc(CorrArgs a)
 }
 
 int launch_corr_nhwc(int ek, const void* const* f1, const void* const* f2, int cs1, int cs2, int C, int N,
-                     int H, int W, void* out, int ocs, int f1_c0, hipStream_t s)
+                     int H, int W, void* out, int ocs, int f1_c0, hipStream_t s, bool bf16_products)
 {
     if (N < 1 || N > FLOWNET_MAX_PAIRS) return set_error(VSC_E_INVALID, "corr nhwc: batch of 1 to 8 items");
     CorrArgs a{};
@@ -178,8 +195,9 @@ int launch_corr_nhwc(int ek, const void* const* f1, const void* const* f2, int c
     a.f1_c0 = f1_c0;
     if (f1_c0 >= 0 && f1_c0 + C > ocs) return set_error(VSC_E_INVALID, "corr nhwc: f1 copy outside the pixel");
     dim3 grid((W + 31) / 32, (H + 3) / 4, N);
-    if (ek == EK_BF16) hipLaunchKernelGGL(k_corr_nhwc<EK_BF16>, grid, dim3(512), 0, s, a);
-    else hipLaunchKernelGGL(k_corr_nhwc<EK_F32>, grid, dim3(512), 0, s, a);
+    if (ek == EK_BF16) hipLaunchKernelGGL((k_corr_nhwc<EK_BF16, false>), grid, dim3(512), 0, s, a);
+    else if (bf16_products) hipLaunchKernelGGL((k_corr_nhwc<EK_F32, true>), grid, dim3(512), 0, s, a);
+    else hipLaunchKernelGGL((k_corr_nhwc<EK_F32, false>), grid, dim3(512), 0, s, a);
     return check_launch("corr nhwc");
 }
 

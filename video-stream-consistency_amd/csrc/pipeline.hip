@@ -109,6 +109,7 @@ extern "C" void vsc_pipeline_default_cfg(vsc_pipeline_cfg* c)
     c->solver = 0;
     c->overlap = 0;
     c->flow_batch = 1;
+    c->corr_dtype = VSC_F32;
 }
 
 static int cfg_ok(int W, int H, const vsc_pipeline_cfg* c)
@@ -123,6 +124,8 @@ static int cfg_ok(int W, int H, const vsc_pipeline_cfg* c)
     if (c->overlap != 0 && c->overlap != 1) return set_error(VSC_E_INVALID, "vsc_pipeline: overlap must be 0 or 1");
     if (c->flow_batch < 1 || c->flow_batch > PIPE_MAX_BATCH)
         return set_error(VSC_E_INVALID, "vsc_pipeline: flow_batch must be in [1, 4]");
+    if (c->corr_dtype != VSC_F32 && c->corr_dtype != VSC_BF16)
+        return set_error(VSC_E_DTYPE, "vsc_pipeline: corr_dtype must be VSC_F32 or VSC_BF16");
     return VSC_OK;
 }
 
@@ -153,6 +156,7 @@ extern "C" int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg
     p->slots = cfg->flow_batch + 2;
     if ((rc = flownet_create_pairs(width, height, cfg->downscale, cfg->flow_dtype, cfg->seed, p->slots, 2 * cfg->flow_batch,
                                    &p->net)) ||
+        (rc = vsc_flownet_set_corr_dtype(p->net, cfg->flow_dtype == VSC_BF16 ? VSC_BF16 : cfg->corr_dtype)) ||
         (rc = vsc_flownet_flow_size(p->net, &p->Wd, &p->Hd))) {
         vsc_pipeline_destroy(p);
         return rc;
