@@ -10,8 +10,8 @@ the largest magnitude of the reference tensor.
 Tolerances: FP32 storage 2e-5 (float32 through ~30 conv layers; measured
 errors ~3e-6, printed); BF16 storage 4e-2 (bfloat16 activations, 2^-9 relative
 rounding per layer); FP32 storage with the cost volume's products on the BF16
-MFMA 4e-2 (the features rounded to bfloat16 before the dot products of each
-level, everything else FP32).
+MFMA 5e-3 (the features rounded to bfloat16 before the dot products of each
+level, everything else FP32; measured errors up to 1.2e-3).
 """
 import numpy as np
 import pytest
@@ -20,7 +20,7 @@ from oracle import pwcnet_light as pw
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"f32": 2e-5, "bf16": 4e-2, "f32+bf16corr": 4e-2}
+TOL = {"f32": 2e-5, "bf16": 4e-2, "f32+bf16corr": 5e-3}
 
 
 def _frames(vsc, W, H, seed=3):
