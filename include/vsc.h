@@ -391,12 +391,12 @@ int vsc_upsampler_set_params(vsc_upsampler* ups, const float* host);
  * size a workspace without knowing the kernel. */
 size_t vsc_upsampler_workspace_size(int width, int height);
 /* flow_lo: device HWC lo_w x lo_h x lo_ch (2|3); rgba: device RGBA8 width x
- * height; flow_out: device HWC width x height x out_ch (2|3, third channel 0);
+ * height (4-byte aligned, else VSC_E_INVALID); flow_out: device HWC width x height x out_ch (2|3, third channel 0);
  * scale: factor applied to the bilinear flow vectors (FLOWDOWNSCALE f with
  * vector scaling, 1 for the reference's unscaled vectors); gate: NULL or a
  * device float multiplying the learned residual (see vsc_flow_fb_gate);
  * workspace: vsc_upsampler_workspace_size bytes (may be NULL when that is 0).
- * One fused kernel: the three layers of a 16 x 32 tile in LDS.             */
+ * One fused kernel: the three layers of a 20 x 28 tile in LDS.             */
 int vsc_upsampler_run(vsc_upsampler* ups, const float* flow_lo, int lo_w, int lo_h, int lo_ch, const uint8_t* rgba,
                       int width, int height, float scale, float* flow_out, int out_ch, const float* gate,
                       void* workspace, vsc_hip_stream stream);

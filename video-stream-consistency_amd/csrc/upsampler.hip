@@ -26,11 +26,18 @@ constexpr int U_W2 = U_B1 + 16, U_B2 = U_W2 + 16 * 9 * 16;
 constexpr int U_W3 = U_B2 + 16, U_B3 = U_W3 + 2 * 9 * 16;
 constexpr int U_NP = U_B3 + 2;                            // 3202 floats
 constexpr int U_NREF = 16 * 3 * 9 + 16 + 16 * 16 * 9 + 16 + 2 * 16 * 9 + 2;  // torch layout: 3058
-// FP16 inference weights of the fused kernel, MFMA A-operand rows [co][K]:
-// layer 1 K = tap*4 + c (36 used of 64), layers 2 and 3 K = tap*16 + ci (144
-// used of 160); layer 3 rows 2..15 are zero
-constexpr int H_K1 = 64, H_K2 = 160;
-constexpr int H_W1 = 0, H_W2 = 16 * H_K1, H_W3 = H_W2 + 16 * H_K2, H_N = H_W3 + 16 * H_K2;
+// FP16 inference weights of the fused kernel: 14 MFMA A operands (16 rows co
+// x 32 K), each stored lane-major (lane = 16*lg + co holds K 8*lg .. 8*lg+7):
+// layer 1 fragment f holds tap rows dy = 2f + lg/2 (dy 3: zero), lane group
+// lg%2 taps dx 0 and 1 (4 channels each, channel 3 zero) or dx 2 and a zero
+// tap; layer 2 fragments 0..2 hold taps (dy, dx = lg/2), channels
+// 8*(lg%2) .. +7 (output channel co = row), fragments 3..6 tap dx 2 of two
+// consecutive input rows (lane group lg/2 the second), at tap rows (0, 1),
+// (2, -), (-, 0), (1, 2) (-: zero); layer 3 fragment st holds taps dx = 2*st
+// + lg/2 (dx 3: zero) of all three tap rows, row m = 4*co + dy (co < 2, dy <
+// 3; other rows zero)
+constexpr int H_FR1 = 0, H_FR2 = 2, H_FR3 = 9, H_NFR = 11;
+constexpr int H_N = H_NFR * 64 * 8;
 // flipped / transposed FP32 weights for the data gradients
 constexpr int T_W2 = 0, T_W3 = 16 * 9 * 16, T_N = T_W3 + 16 * 9 * 4;
 
@@ -176,16 +183,19 @@ __global__ void k_ups_derive(const float* __restrict__ p, uint16_t* w16, float* 
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < H_N) {
+        const int f = i / 512, lane = (i / 8) % 64, j = i % 8, co = lane % 16, lg = lane / 16;
         float v = 0.0f;
-        if (i < H_W2) {
-            const int co = i / H_K1, k = i % H_K1, tap = k / 4, c = k % 4;
-            v = (tap < 9 && c < 3) ? p[U_W1 + (co * 9 + tap) * U_C1 + c] : 0.0f;
-        } else if (i < H_W3) {
-            const int co = (i - H_W2) / H_K2, k = (i - H_W2) % H_K2;
-            v = k < 144 ? p[U_W2 + co * 144 + k] : 0.0f;
+        if (f < H_FR2) {
+            const int dy = 2 * f + (lg >> 1), dx = 2 * (lg & 1) + (j >> 2), c = j & 3;
+            if (dy < 3 && dx < 3 && c < 3) v = p[U_W1 + (co * 9 + dy * 3 + dx) * U_C1 + c];
+        } else if (f < H_FR3) {
+            const int g = f - H_FR2, h = lg >> 1, ci = 8 * (lg & 1) + j;
+            constexpr int pair_dy[4][2] = {{0, 1}, {2, -1}, {-1, 0}, {1, 2}};
+            const int dy = g < 3 ? g : pair_dy[g - 3][h], dx = g < 3 ? h : 2;
+            if (dy >= 0) v = p[U_W2 + (co * 9 + dy * 3 + dx) * 16 + ci];
         } else {
-            const int co = (i - H_W3) / H_K2, k = (i - H_W3) % H_K2;
-            v = (co < 2 && k < 144) ? p[U_W3 + co * 144 + k] : 0.0f;
+            const int m = co, oc = m >> 2, dy = m & 3, dx = 2 * (f - H_FR3) + (lg >> 1), ci = 8 * (lg & 1) + j;
+            if (oc < 2 && dy < 3 && dx < 3) v = p[U_W3 + (oc * 9 + dy * 3 + dx) * 16 + ci];
         }
         w16[i] = f32_to_f16(v);
     }
@@ -203,24 +213,42 @@ __global__ void k_ups_derive(const float* __restrict__ p, uint16_t* w16, float* 
 }
 
 // ---------------------------------------------------------------------------
-// Fused FP16 inference: the three convolutions of one 16 x 32 output tile in
-// one block, activations in LDS.  The input tile carries a 3-pixel halo, the
-// first activation a 2-pixel and the second a 1-pixel halo; activations of
-// pixels outside the frame are zero (each layer's zero padding).  Each layer
-// is an MFMA GEMM D[co][px] = W[co][K] * X[K][px] over 16-pixel groups, K =
-// (tap, channel) pairs gathered from LDS: layer 1 in two 16x16x32 steps (taps
-// 0..7, tap 8), layers 2 and 3 in five (tap pairs).  Weights stay in
-// registers for the whole block.
+// Fused FP16 inference: the three convolutions of one 20 x 28 output tile in
+// one block, activations in LDS.  The input tile (26 x 34 pixels, a 3-pixel
+// halo) holds (u, v, luma, 0) per pixel; layer 1 writes a 24 x 32 tile
+// (2-pixel halo), layer 2 a 22 x 32 tile (1-pixel halo and two spare
+// columns), layer 3 the 20 x 28 output.  Activations of pixels outside the
+// frame are zero (each layer's zero padding).
+//
+// Each wave computes a strip of R rows x 16 columns of a layer as MFMA GEMMs
+// D[co][px] = W[co][K] * X[K][px] (A = weights in registers, B = 16 pixels
+// of one row) and reads the R + 2 input rows of its strip once: the
+// fragments of input row rho feed output rows rho - 2 .. rho, each with the
+// weights of its tap row.  A 16-channel layer takes two K-steps per tap row
+// (taps dx 0 and 1; tap dx 2 beside a zero-weight column), layer 1 (4
+// channels per pixel) one K-step per two tap rows (rows dy 0 and 1; row dy
+// 2 beside a zero-weight row).  Every LDS address is a per-lane base plus a
+// compile-time offset, the accumulators start at the bias, and a row's
+// epilogue is two packed FP16 conversions and a packed ReLU.
 // ---------------------------------------------------------------------------
 typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 up_f16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float up_f32x4;
 typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t up_u32x4;
+typedef _Float16 up_h2 __attribute__((ext_vector_type(2)));
 
-constexpr int UF_TH = 16, UF_TW = 32;
-constexpr int UF_XH = UF_TH + 6, UF_XW = UF_TW + 6;      // input tile
-constexpr int UF_1H = UF_TH + 4, UF_1W = UF_TW + 4;      // layer-1 output tile
-constexpr int UF_2H = UF_TH + 2, UF_2W = UF_TW + 2;      // layer-2 output tile
-constexpr int UF_AS = 24;                                 // LDS pixel stride of activations (halfs): 48 B
+constexpr int UF_TH = 20, UF_TW = 28;                     // output tile
+constexpr int UF_XP = UF_TW + 6, UF_XR = UF_TH + 8;       // input tile pitch (pixels); rows 26, 27 are zero
+constexpr int UF_AP = 32;                                 // activation tile pitch (pixels)
+constexpr int UF_AS = 24;                                 // halfs per activation pixel: 16 channels + pad (48 B)
+constexpr int UF_ROWB = UF_AP * UF_AS * 2;                // bytes per activation row
+constexpr int UF_1R = UF_TH + 4, UF_2R = UF_TH + 2;       // rows of the layer-1 and layer-2 tiles
+// one zero row after each activation tile: the zero-weight column of a
+// strip's last row reads two pixels into it
+// (layer 1's also 128 bytes more: layer 2's paired tap dx 2 reads the first
+// two pixels of the row after it)
+constexpr int UF_S1B = (UF_1R + 1) * UF_ROWB + 128, UF_S2B = (UF_2R + 1) * UF_ROWB;
+static_assert(UF_XR * UF_XP * 8 <= UF_2R * UF_ROWB, "the input tile lies under layer 2's rows");
+static_assert(UF_TW + 4 == UF_AP, "layer 1 fills the activation pitch");
 
 __device__ inline up_f32x4 up_mma(const up_u32x4& a, const up_u32x4& b, up_f32x4 c)
 {
@@ -233,21 +261,107 @@ __device__ inline uint32_t pack_h2(float a, float b)
     return (uint32_t)f32_to_f16(a) | ((uint32_t)f32_to_f16(b) << 16);
 }
 
-// one 16-pixel group of a 16-channel layer: K = 144 (9 taps x 16) in five
-// steps; off[m] is the lane's LDS offset of tap 2m + (lg >> 1) relative to
-// the group pixel's top-left tap (negative: the tap does not exist)
-__device__ inline up_f32x4 ups_l16(const uint16_t* src, const up_u32x4 (&w)[5], int base, const int (&off)[5])
+// ReLU(x) of the lane's four channels as four FP16 values (round to nearest)
+__device__ inline uint2 relu_h4(const up_f32x4& x)
 {
-    up_f32x4 acc = up_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    up_u32x4 xb[5];
+    const up_h2 z = {(_Float16)0.0f, (_Float16)0.0f};
+    up_h2 a, b;
+    a.x = (_Float16)x[0];
+    a.y = (_Float16)x[1];
+    b.x = (_Float16)x[2];
+    b.y = (_Float16)x[3];
+    a = __builtin_elementwise_max(a, z);
+    b = __builtin_elementwise_max(b, z);
+    return make_uint2(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b));
+}
+
+// A strip of layer 2: rows 0 .. R-1 of the strip, put(r, acc) once row r
+// has all its terms.  The fragment of taps dx 0 and 1 of input row rho (src:
+// the lane's byte address of input row 0, pixel lg/2, channels 8*(lg%2)..)
+// feeds output rows rho - 2 .. rho with the weights of tap rows 2 .. 0; tap
+// dx 2 of input rows 2k and 2k+1 shares one K-step (src2: pixel 2 of input
+// row lg/2) and feeds output rows 2k-2 .. 2k+1 with the tap-row pairs (2, -),
+// (1, 2), (0, 1), (-, 0): five MFMAs per output row.
+template <int R, typename Put>
+__device__ inline void ups_strip16p(const uint8_t* src, const uint8_t* src2, const up_u32x4 (&A)[3],
+                                    const up_u32x4 (&P)[4], const up_f32x4& bias, Put put)
+{
+    up_f32x4 acc[R];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        xb[m] = up_u32x4{0u, 0u, 0u, 0u};
-        if (off[m] >= 0) xb[m] = *reinterpret_cast<const up_u32x4*>(&src[base + off[m]]);
+    for (int r = 0; r < R; ++r) acc[r] = bias;
+    up_u32x4 fa[2], fp[2];
+    fa[0] = *reinterpret_cast<const up_u32x4*>(src);
+    fp[0] = *reinterpret_cast<const up_u32x4*>(src2);
+#pragma unroll
+    for (int rho = 0; rho < R + 2; ++rho) {
+        if (rho + 1 < R + 2) fa[(rho + 1) & 1] = *reinterpret_cast<const up_u32x4*>(src + (rho + 1) * UF_ROWB);
+        if ((rho & 1) && rho + 1 < R + 2)
+            fp[((rho + 1) >> 1) & 1] = *reinterpret_cast<const up_u32x4*>(src2 + (rho + 1) * UF_ROWB);
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+            const int r = rho - dy;
+            if (r >= 0 && r < R) acc[r] = up_mma(A[dy], fa[rho & 1], acc[r]);
+        }
+        if (!(rho & 1)) {
+            const up_u32x4& f = fp[(rho >> 1) & 1];
+            if (rho - 2 >= 0 && rho - 2 < R) acc[rho - 2] = up_mma(P[1], f, acc[rho - 2]);
+            if (rho - 1 >= 0 && rho - 1 < R) acc[rho - 1] = up_mma(P[3], f, acc[rho - 1]);
+            if (rho < R) acc[rho] = up_mma(P[0], f, acc[rho]);
+            if (rho + 1 < R) acc[rho + 1] = up_mma(P[2], f, acc[rho + 1]);
+        }
+        if (rho >= 2) put(rho - 2, acc[rho - 2]);
     }
+}
+
+// A strip of layer 3: D row m = 4*co + dy holds, after input row rho, the
+// partial sum of output channel co of output row rho - dy, so each input row
+// takes two MFMAs for all three output rows it feeds; the accumulator shifts
+// by one element per input row and element 2 leaves complete.  put(r, v):
+// output row r, channel lg of the lane's pixel (lanes lg < 2).
+template <int R, typename Put>
+__device__ inline void ups_strip_out(const uint8_t* src, const up_u32x4 (&A)[2], float bias, Put put)
+{
+    up_f32x4 acc = up_f32x4{bias, bias, bias, bias};
+    up_u32x4 fa[2], fb[2];
+    fa[0] = *reinterpret_cast<const up_u32x4*>(src);
+    fb[0] = *reinterpret_cast<const up_u32x4*>(src + 2 * UF_AS * 2);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) acc = up_mma(w[m], xb[m], acc);
-    return acc;
+    for (int rho = 0; rho < R + 2; ++rho) {
+        if (rho + 1 < R + 2) {
+            fa[(rho + 1) & 1] = *reinterpret_cast<const up_u32x4*>(src + (rho + 1) * UF_ROWB);
+            fb[(rho + 1) & 1] = *reinterpret_cast<const up_u32x4*>(src + (rho + 1) * UF_ROWB + 2 * UF_AS * 2);
+        }
+        const up_f32x4 c = up_f32x4{bias, acc[0], acc[1], acc[2]};
+        acc = up_mma(A[0], fa[rho & 1], c);
+        acc = up_mma(A[1], fb[rho & 1], acc);
+        if (rho >= 2) put(rho - 2, acc[2]);
+    }
+}
+
+// A strip of layer 1 (src: the lane's byte address of input-tile row lg/2,
+// pixel 2*(lg%2) of the lane's column)
+template <int R, typename Put>
+__device__ inline void ups_strip4(const uint8_t* src, const up_u32x4 (&A)[2], const up_f32x4& bias, Put put)
+{
+    up_f32x4 acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = bias;
+    auto load = [&](int rho) {
+        const uint2 a = *reinterpret_cast<const uint2*>(src + rho * UF_XP * 8);
+        const uint2 b = *reinterpret_cast<const uint2*>(src + rho * UF_XP * 8 + 8);
+        return up_u32x4{a.x, a.y, b.x, b.y};
+    };
+    up_u32x4 f[2];
+    f[0] = load(0);
+#pragma unroll
+    for (int rho = 0; rho < R + 2; ++rho) {
+        if (rho + 1 < R + 2) f[(rho + 1) & 1] = load(rho + 1);
+        if (rho < R) acc[rho] = up_mma(A[0], f[rho & 1], acc[rho]);
+        if (rho >= 2) {
+            acc[rho - 2] = up_mma(A[1], f[rho & 1], acc[rho - 2]);
+            put(rho - 2, acc[rho - 2]);
+        }
+    }
 }
 
 // blockIdx.y selects the flow (up to two flows of one frame per launch)
@@ -256,164 +370,161 @@ struct UpsFlows { const float* lo[2]; float* out[2]; };
 __global__ __launch_bounds__(512) void k_ups_fused(UpsFlows flows, int lw, int lh, int lch,
                                                    const uint8_t* __restrict__ rgba, int W, int H, float scale,
                                                    const uint16_t* __restrict__ wf, const float* __restrict__ prm,
-                                                   const float* gate, int out_ch, int tiles_x)
+                                                   const float* gate, int out_ch, int tiles_x, bool vec2)
 {
     const float* __restrict__ lo = flows.lo[blockIdx.y];
     float* __restrict__ out = flows.out[blockIdx.y];
-    constexpr int NW = 8;   // waves per block; each keeps two 16-pixel groups in flight
-    __shared__ __attribute__((aligned(16))) uint2 sx[UF_XH * UF_XW];
-    __shared__ __attribute__((aligned(16))) uint16_t s1[UF_1H * UF_1W * UF_AS];
-    __shared__ __attribute__((aligned(16))) uint16_t s2[UF_2H * UF_2W * UF_AS];
-    __shared__ float2 sbase[UF_TH * UF_TW];          // FP32 bilinear flow of the output tile
-    __shared__ int tx0[UF_XW], tx1[UF_XW], ty0[UF_XH], ty1[UF_XH];
-    __shared__ float tfx[UF_XW], tfy[UF_XH];
+    constexpr int NT = 512;
+    __shared__ __attribute__((aligned(16))) uint8_t s1[UF_S1B];
+    __shared__ __attribute__((aligned(16))) uint8_t s2[UF_S2B];   // the input tile until layer 1 is done
+    __shared__ float2 sbase[UF_TH * UF_TW];                        // FP32 bilinear flow of the output tile
+    __shared__ int tx0[UF_XP], tx1[UF_XP], ty0[UF_XR], ty1[UF_XR];
+    __shared__ float tfx[UF_XP], tfy[UF_XR];
+    uint2* const sx = reinterpret_cast<uint2*>(s2);
     const int tid = threadIdx.x, q = tid / 64, lane = tid % 64, l16 = lane % 16, lg = lane / 16;
     const int y0 = (blockIdx.x / tiles_x) * UF_TH, x0 = (blockIdx.x % tiles_x) * UF_TW;
     // bilinear taps of the tile's columns and rows (kernel_bilinear mapping)
-    if (tid < UF_XW) {
+    if (tid < UF_XP) {
         const int ix = min(max(x0 - 3 + tid, 0), W - 1);
         const BiTap t = bilinear_tap(ix, 0, lw, lh, W, H);
         tx0[tid] = t.ix; tx1[tid] = t.ix1; tfx[tid] = t.fx;
-    } else if (tid >= 64 && tid < 64 + UF_XH) {
+    } else if (tid >= 64 && tid < 64 + UF_XR) {
         const int iy = min(max(y0 - 3 + tid - 64, 0), H - 1);
         const BiTap t = bilinear_tap(0, iy, lw, lh, W, H);
         ty0[tid - 64] = t.iy; ty1[tid - 64] = t.iy1; tfy[tid - 64] = t.fy;
     }
+    // the zero rows after the activation tiles (never written again)
+    if (tid < (UF_S1B - UF_1R * UF_ROWB) / 16)
+        reinterpret_cast<uint4*>(s1 + UF_1R * UF_ROWB)[tid] = make_uint4(0u, 0u, 0u, 0u);
+    if (tid < UF_ROWB / 16)
+        reinterpret_cast<uint4*>(s2 + UF_2R * UF_ROWB)[tid] = make_uint4(0u, 0u, 0u, 0u);
 
-    // weights (A operand rows co = l16), biases for this lane's 4 output rows,
-    // and the lane's tap offsets in the activation tiles
-    up_u32x4 w1[2], w2[5], w3[5];
+    // weights: MFMA A fragments (rows co = l16), 8 halfs per lane, lane-major
+    auto frag = [&](int f) { return *reinterpret_cast<const up_u32x4*>(wf + (f * 64 + lane) * 8); };
+    up_u32x4 a1[2], a2[3], a2p[4], a3[2];
 #pragma unroll
-    for (int m = 0; m < 2; ++m) w1[m] = *reinterpret_cast<const up_u32x4*>(wf + H_W1 + l16 * H_K1 + 32 * m + 8 * lg);
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        w2[m] = *reinterpret_cast<const up_u32x4*>(wf + H_W2 + l16 * H_K2 + 32 * m + 8 * lg);
-        w3[m] = *reinterpret_cast<const up_u32x4*>(wf + H_W3 + l16 * H_K2 + 32 * m + 8 * lg);
+    for (int f = 0; f < 2; ++f) {
+        a1[f] = frag(H_FR1 + f);
+        a3[f] = frag(H_FR3 + f);
     }
-    float b1[4], b2[4], b3[4];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) a2[f] = frag(H_FR2 + f);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) a2p[f] = frag(H_FR2 + 3 + f);
+    up_f32x4 b1, b2;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         b1[r] = prm[U_B1 + 4 * lg + r];
         b2[r] = prm[U_B2 + 4 * lg + r];
-        b3[r] = 4 * lg + r < 2 ? prm[U_B3 + 4 * lg + r] : 0.0f;
     }
-    int off1[5], off2[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        const int tap = 2 * m + (lg >> 1);
-        off1[m] = tap < 9 ? ((tap / 3) * UF_1W + tap % 3) * UF_AS + 8 * (lg & 1) : -1;
-        off2[m] = tap < 9 ? ((tap / 3) * UF_2W + tap % 3) * UF_AS + 8 * (lg & 1) : -1;
-    }
-    const int t0 = 2 * lg, t1 = 2 * lg + 1;
-    const int offx0 = (t0 / 3) * UF_XW + t0 % 3, offx1 = (t1 / 3) * UF_XW + t1 % 3, offx8 = 2 * UF_XW + 2;
+    const float b3 = lg < 2 ? prm[U_B3 + lg] : 0.0f;
+    const float gv = gate ? *gate : 1.0f;
     __syncthreads();
 
+    // wave q: column group cg (16 columns), strip s of rows
+    const int cg = q & 1, s = q >> 1, c = 16 * cg + l16;
     // Tiles whose input tile (with its 3-pixel halo) lies inside the frame
-    // skip every per-pixel frame test (block-uniform choice).
+    // skip every frame test (block-uniform choice).
     const bool interior = y0 >= 3 && y0 + UF_TH + 3 <= H && x0 >= 3 && x0 + UF_TW + 3 <= W;
     auto run = [&](auto interior_c) {
         constexpr bool INTERIOR = decltype(interior_c)::value;
-        // input tile: (u, v) = bilinear x scale, luma; zero outside the frame
-        for (int e = tid; e < UF_XH * UF_XW; e += 64 * NW) {
-            const int r = e / UF_XW, c = e % UF_XW;
-            const int iy = y0 - 3 + r, ix = x0 - 3 + c;
+        // input tile: (u, v) = bilinear x scale, luma; zero outside the frame.
+        // A thread's two pixels issue all their loads before either is
+        // computed (addresses clamped into the frame, values dropped after).
+        constexpr int NX = UF_XR * UF_XP, NE = (NX + NT - 1) / NT;
+        float2 v00[NE], v10[NE], v01[NE], v11[NE];
+        uint32_t pix[NE];
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            const int e = min(tid + NT * k, NX - 1), r = e / UF_XP, cc = e % UF_XP;
+            const int a0 = tx0[cc], a1_ = tx1[cc], r0 = ty0[r] * lw, r1 = ty1[r] * lw;
+            if (vec2) {
+                const float2* f2 = reinterpret_cast<const float2*>(lo);
+                v00[k] = f2[r0 + a0]; v10[k] = f2[r0 + a1_]; v01[k] = f2[r1 + a0]; v11[k] = f2[r1 + a1_];
+            } else {
+                v00[k] = make_float2(lo[(r0 + a0) * lch], lo[(r0 + a0) * lch + 1]);
+                v10[k] = make_float2(lo[(r0 + a1_) * lch], lo[(r0 + a1_) * lch + 1]);
+                v01[k] = make_float2(lo[(r1 + a0) * lch], lo[(r1 + a0) * lch + 1]);
+                v11[k] = make_float2(lo[(r1 + a1_) * lch], lo[(r1 + a1_) * lch + 1]);
+            }
+            const int iy = min(max(y0 - 3 + r, 0), H - 1), ix = min(max(x0 - 3 + cc, 0), W - 1);
+            pix[k] = *reinterpret_cast<const uint32_t*>(rgba + ((long long)iy * W + ix) * 4);
+        }
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            const int e = tid + NT * k, r = e / UF_XP, cc = e % UF_XP;
+            if (e >= NX) break;
+            const int iy = y0 - 3 + r, ix = x0 - 3 + cc;
             uint2 v = make_uint2(0u, 0u);
-            if (INTERIOR || (iy >= 0 && iy < H && ix >= 0 && ix < W)) {
-                // bilinear_sample's operation order, both channels from 8-byte loads
-                const int a0 = tx0[c], a1 = tx1[c], r0 = ty0[r] * lw, r1 = ty1[r] * lw;
-                const float fx = tfx[c], fy = tfy[r];
-                float2 v00, v10, v01, v11;
-                if (lch == 2) {
-                    const float2* f2 = reinterpret_cast<const float2*>(lo);
-                    v00 = f2[r0 + a0]; v10 = f2[r0 + a1]; v01 = f2[r1 + a0]; v11 = f2[r1 + a1];
-                } else {
-                    v00 = make_float2(lo[(r0 + a0) * lch], lo[(r0 + a0) * lch + 1]);
-                    v10 = make_float2(lo[(r0 + a1) * lch], lo[(r0 + a1) * lch + 1]);
-                    v01 = make_float2(lo[(r1 + a0) * lch], lo[(r1 + a0) * lch + 1]);
-                    v11 = make_float2(lo[(r1 + a1) * lch], lo[(r1 + a1) * lch + 1]);
-                }
-                const float gx = 1.0f - fx, gy = 1.0f - fy;
-                const float u = ((((v00.x * gx) * gy + (v10.x * fx) * gy) + (v01.x * gx) * fy) + (v11.x * fx) * fy) * scale;
-                const float w = ((((v00.y * gx) * gy + (v10.y * fx) * gy) + (v01.y * gx) * fy) + (v11.y * fx) * fy) * scale;
-                const uint8_t* px = rgba + ((long long)iy * W + ix) * 4;
-                const float luma = (0.299f * (float)px[0] + 0.587f * (float)px[1] + 0.114f * (float)px[2]) * (1.0f / 255.0f);
+            if (r < UF_TH + 6 && (INTERIOR || (iy >= 0 && iy < H && ix >= 0 && ix < W))) {
+                // bilinear_sample's operation order
+                const float fx = tfx[cc], fy = tfy[r], gx = 1.0f - fx, gy = 1.0f - fy;
+                const float u = ((((v00[k].x * gx) * gy + (v10[k].x * fx) * gy) + (v01[k].x * gx) * fy) +
+                                 (v11[k].x * fx) * fy) * scale;
+                const float w = ((((v00[k].y * gx) * gy + (v10[k].y * fx) * gy) + (v01[k].y * gx) * fy) +
+                                 (v11[k].y * fx) * fy) * scale;
+                const float luma = (0.299f * (float)(pix[k] & 0xffu) + 0.587f * (float)((pix[k] >> 8) & 0xffu) +
+                                    0.114f * (float)((pix[k] >> 16) & 0xffu)) * (1.0f / 255.0f);
                 v = make_uint2(pack_h2(u, w), pack_h2(luma, 0.0f));
-                if (r >= 3 && r < 3 + UF_TH && c >= 3 && c < 3 + UF_TW) sbase[(r - 3) * UF_TW + c - 3] = make_float2(u, w);
+                if (r >= 3 && r < 3 + UF_TH && cc >= 3 && cc < 3 + UF_TW)
+                    sbase[(r - 3) * UF_TW + cc - 3] = make_float2(u, w);
             }
             sx[e] = v;
         }
         __syncthreads();
 
-        // layer 1 on the 20 x 36 tile, ReLU, zero outside the frame
-        constexpr int N1 = UF_1H * UF_1W, G1 = (N1 + 15) / 16;
-        auto l1 = [&](int p) {
-            const int bx = (p / UF_1W) * UF_XW + p % UF_1W;
-            const uint2 a = sx[bx + offx0], b = sx[bx + offx1];
-            const uint2 e = lg == 0 ? sx[bx + offx8] : make_uint2(0u, 0u);
-            up_f32x4 acc = up_mma(w1[0], up_u32x4{a.x, a.y, b.x, b.y}, up_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-            return up_mma(w1[1], up_u32x4{e.x, e.y, 0u, 0u}, acc);
-        };
-        auto put1 = [&](int g, const up_f32x4& acc) {
-            const int p = 16 * g + l16;
-            if (p >= N1) return;
-            const int r = p / UF_1W, c = p % UF_1W, iy = y0 - 2 + r, ix = x0 - 2 + c;
-            const bool in = INTERIOR || (iy >= 0 && iy < H && ix >= 0 && ix < W);
-            float o[4];
-    #pragma unroll
-            for (int k = 0; k < 4; ++k) o[k] = in ? fmaxf(acc[k] + b1[k], 0.0f) : 0.0f;
-            *reinterpret_cast<uint2*>(&s1[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
-        };
-        for (int g = q; g < G1; g += 2 * NW) {
-            const int g2 = g + NW;
-            const up_f32x4 a0 = l1(min(16 * g + l16, N1 - 1));
-            const up_f32x4 a1 = l1(min(16 * g2 + l16, N1 - 1));
-            put1(g, a0);
-            if (g2 < G1) put1(g2, a1);
+        // layer 1: rows 6s .. 6s+5 of the 24 x 32 tile, ReLU, zero outside the frame
+        {
+            const int r0 = 6 * s;
+            const uint8_t* src = s2 + ((r0 + (lg >> 1)) * UF_XP + c + 2 * (lg & 1)) * 8;
+            uint8_t* dst = s1 + (r0 * UF_AP + c) * UF_AS * 2 + 8 * lg;
+            const bool col_in = INTERIOR || (x0 - 2 + c >= 0 && x0 - 2 + c < W);
+            ups_strip4<6>(src, a1, b1, [&](int r, const up_f32x4& acc) {
+                uint2 o = relu_h4(acc);
+                if (!INTERIOR) {
+                    const int iy = y0 - 2 + r0 + r;
+                    if (!(col_in && iy >= 0 && iy < H)) o = make_uint2(0u, 0u);
+                }
+                *reinterpret_cast<uint2*>(dst + r * UF_ROWB) = o;
+            });
         }
         __syncthreads();
 
-        // layer 2 on the 18 x 34 tile
-        constexpr int N2 = UF_2H * UF_2W, G2 = (N2 + 15) / 16;
-        auto put2 = [&](int g, const up_f32x4& acc) {
-            const int p = 16 * g + l16;
-            if (p >= N2) return;
-            const int r = p / UF_2W, c = p % UF_2W, iy = y0 - 1 + r, ix = x0 - 1 + c;
-            const bool in = INTERIOR || (iy >= 0 && iy < H && ix >= 0 && ix < W);
-            float o[4];
-    #pragma unroll
-            for (int k = 0; k < 4; ++k) o[k] = in ? fmaxf(acc[k] + b2[k], 0.0f) : 0.0f;
-            *reinterpret_cast<uint2*>(&s2[p * UF_AS + 4 * lg]) = make_uint2(pack_h2(o[0], o[1]), pack_h2(o[2], o[3]));
-        };
-        auto base2 = [&](int p) { return ((p / UF_2W) * UF_1W + p % UF_2W) * UF_AS; };
-        for (int g = q; g < G2; g += 2 * NW) {
-            const int g2 = g + NW;
-            const up_f32x4 a0 = ups_l16(s1, w2, base2(min(16 * g + l16, N2 - 1)), off1);
-            const up_f32x4 a1 = ups_l16(s1, w2, base2(min(16 * g2 + l16, N2 - 1)), off1);
-            put2(g, a0);
-            if (g2 < G2) put2(g2, a1);
+        // layer 2: strips of 6, 5, 6, 5 rows of the 22 x 32 tile
+        {
+            const int r0 = (s >> 1) * 11 + (s & 1) * 6;
+            const uint8_t* src = s1 + ((r0 * UF_AP + c + (lg >> 1)) * UF_AS + 8 * (lg & 1)) * 2;
+            const uint8_t* src2 = s1 + (((r0 + (lg >> 1)) * UF_AP + c + 2) * UF_AS + 8 * (lg & 1)) * 2;
+            uint8_t* dst = s2 + (r0 * UF_AP + c) * UF_AS * 2 + 8 * lg;
+            const bool col_in = INTERIOR || (x0 - 1 + c >= 0 && x0 - 1 + c < W);
+            auto put = [&](int r, const up_f32x4& acc) {
+                uint2 o = relu_h4(acc);
+                if (!INTERIOR) {
+                    const int iy = y0 - 1 + r0 + r;
+                    if (!(col_in && iy >= 0 && iy < H)) o = make_uint2(0u, 0u);
+                }
+                *reinterpret_cast<uint2*>(dst + r * UF_ROWB) = o;
+            };
+            if (s & 1) ups_strip16p<5>(src, src2, a2, a2p, b2, put);
+            else ups_strip16p<6>(src, src2, a2, a2p, b2, put);
         }
         __syncthreads();
 
-        // layer 3 on the 16 x 32 output tile, added to the FP32 bilinear flow
-        const float gv = gate ? *gate : 1.0f;
-        constexpr int G3 = UF_TH * UF_TW / 16;
-        auto put3 = [&](int g, const up_f32x4& acc) {
-            const int p = 16 * g + l16, r = p / UF_TW, c = p % UF_TW, iy = y0 + r, ix = x0 + c;
-            if (lg == 0 && (INTERIOR || (iy < H && ix < W))) {
-                const float2 bs = sbase[p];
-                float* o = out + ((long long)iy * W + ix) * out_ch;
-                o[0] = bs.x + gv * (acc[0] + b3[0]);
-                o[1] = bs.y + gv * (acc[1] + b3[1]);
-                if (out_ch == 3) o[2] = 0.0f;
-            }
-        };
-        auto base3 = [&](int p) { return ((p / UF_TW) * UF_2W + p % UF_TW) * UF_AS; };
-        for (int g = q; g < G3; g += 2 * NW) {
-            const int g2 = g + NW;
-            const up_f32x4 a0 = ups_l16(s2, w3, base3(16 * g + l16), off2);
-            const up_f32x4 a1 = ups_l16(s2, w3, base3(16 * min(g2, G3 - 1) + l16), off2);
-            put3(g, a0);
-            if (g2 < G3) put3(g2, a1);
+        // layer 3: rows 5s .. 5s+4 of the output tile, added to the FP32
+        // bilinear flow; lane group lg < 2 holds output channel lg
+        {
+            const int r0 = 5 * s;
+            const uint8_t* src = s2 + ((r0 * UF_AP + c + (lg >> 1)) * UF_AS + 8 * (lg & 1)) * 2;
+            const bool st = lg < 2 && c < UF_TW && (INTERIOR || x0 + c < W);
+            float* o = out + ((long long)(y0 + r0) * W + x0 + c) * out_ch + lg;
+            const float* bs = reinterpret_cast<const float*>(sbase) + (r0 * UF_TW + c) * 2 + lg;
+            ups_strip_out<5>(src, a3, b3, [&](int r, float v) {
+                if (st && (INTERIOR || y0 + r0 + r < H)) {
+                    float* p = o + (long long)r * W * out_ch;
+                    p[0] = bs[r * UF_TW * 2] + gv * v;
+                    if (out_ch == 3 && lg == 1) p[1] = 0.0f;
+                }
+            });
         }
     };
     if (interior) run(std::true_type{});
@@ -716,11 +827,18 @@ int upsampler_run_flows(vsc_upsampler* u, int n, const float* const flow_lo[2], 
     if (lo_w <= 0 || lo_h <= 0 || width <= 0 || height <= 0 || (lo_ch != 2 && lo_ch != 3) ||
         (out_ch != 2 && out_ch != 3))
         return set_error(VSC_E_INVALID, "vsc_upsampler_run: bad shape");
+    if (reinterpret_cast<uintptr_t>(rgba) % 4 != 0)
+        return set_error(VSC_E_INVALID, "vsc_upsampler_run: rgba must be 4-byte aligned");
     UpsFlows f{};
-    for (int i = 0; i < n; ++i) { f.lo[i] = flow_lo[i]; f.out[i] = flow_out[i]; }
+    bool vec2 = lo_ch == 2;   // both flow channels of a pixel from one 8-byte load
+    for (int i = 0; i < n; ++i) {
+        f.lo[i] = flow_lo[i];
+        f.out[i] = flow_out[i];
+        vec2 = vec2 && reinterpret_cast<uintptr_t>(flow_lo[i]) % 8 == 0;
+    }
     const int tx = (width + UF_TW - 1) / UF_TW, ty = (height + UF_TH - 1) / UF_TH;
     hipLaunchKernelGGL(k_ups_fused, dim3(tx * ty, n), dim3(512), 0, s, f, lo_w, lo_h, lo_ch, rgba, width, height,
-                       scale, u->w16, u->p, gate, out_ch, tx);
+                       scale, u->w16, u->p, gate, out_ch, tx, vec2);
     return check_launch("upsampler fused");
 }
 
