@@ -69,6 +69,39 @@ def test_upsampler_inference_matches_oracle(vsc, cuda, size):
         assert np.allclose(b, base[0].permute(1, 2, 0).numpy(), rtol=0, atol=1e-5)
 
 
+@pytest.mark.parametrize("lw,lh,lch,shift,scale", [(29, 37, 2, 0, 1.7), (48, 48, 3, 0, 2.0), (48, 48, 2, 1, 2.0),
+                                                   (61, 45, 3, 1, 0.5)])
+def test_upsampler_inference_input_forms(vsc, cuda, lw, lh, lch, shift, scale):
+    """The kernel's input forms: the exact 2x form (output twice the low-res
+    size: each bilinear weight 0 or 1/2) and the general one (any ratio), both
+    flow channels from one 8-byte load or per channel (three-channel flows,
+    flows at an address that is not 8-byte aligned).  96 x 96 frames: every
+    tile touches the frame edge and the general ratios tile it unevenly."""
+    import torch
+    ut, (fr, gt, _) = _data(vsc, 1, 96, seed=11)
+    rng = np.random.default_rng(lw * 100 + lh)
+    lo = rng.uniform(-3.0, 3.0, (1, lh, lw, 2)).astype(np.float32)
+    p = _random_params(vsc, 3)
+    ups = vsc.Upsampler(seed=0, params=p)
+    p16 = p.astype(np.float16).astype(np.float32)
+    full = np.zeros((lh, lw, lch), np.float32)
+    full[..., :2] = lo[0]
+    flat = torch.zeros(full.size + shift, dtype=torch.float32, device=cuda)
+    flat[shift:] = torch.from_numpy(full.ravel()).to(cuda)
+    lt = flat[shift:].view(lh, lw, lch)
+    assert (lt.data_ptr() % 8 != 0) == (shift == 1)
+    got = ups.run(lt, torch.from_numpy(fr[0]).to(cuda), scale=scale).cpu().numpy()
+    x, base = ou.prep(lo, fr[0:1], scale)
+    ref = ou.forward(ou.unpack(p16), x, base)[0].permute(1, 2, 0).numpy()
+    err = np.abs(got - ref)
+    print(f"{lw}x{lh} ch{lch} shift{shift}: max err {err.max():.3e}")
+    assert np.all(err <= 2e-2 + 2.0 ** -10 * np.abs(ref))
+    g0 = torch.zeros(1, device=cuda)
+    b = ups.run(lt, torch.from_numpy(fr[0]).to(cuda), scale=scale, gate=g0).cpu().numpy()
+    assert np.allclose(b, base[0].permute(1, 2, 0).numpy(), rtol=0, atol=1e-5)
+    ups.close()
+
+
 def test_trained_upsampler_beats_bilinear(vsc, cuda):
     """A short training run on the device lowers the end-point error below
     the bilinear upsampler's on held-out pairs."""
