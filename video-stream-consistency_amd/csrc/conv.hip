@@ -108,10 +108,11 @@ __global__ __launch_bounds__(64 * NW) void k_conv3x3(ConvArgs a)
     constexpr int VEC = E::VEC, KC = KCX, KV = KC / VEC;
     constexpr int TH = NW * MR, TW = 16;
     constexpr int PR = (TH - 1) * S + 3, PC = (TW - 1) * S + 3;
-    // LDS pixel stride: the chunk and one 16-byte pad (FP32), two for 16-bit
-    // operands (96-byte stride for 32-channel chunks: 5% faster than 80 on the
+    // LDS pixel stride: the chunk alone for FP32 (64 bytes: the plain form
+    // 1.3% faster than with a 16-byte pad), the chunk and two 16-byte pads for
+    // 16-bit operands (96 bytes for 32-channel chunks: 5% faster than 80 on the
     // 1080p level-3 layers, 64 and 112 no better, 144 slower)
-    constexpr int ROW = KC + (EK == EK_F32 ? 1 : 2) * VEC;
+    constexpr int ROW = KC + (EK == EK_F32 ? 0 : 2) * VEC;
     constexpr int NP = PR * PC * KV, NWU = NT * 9 * KV;
     constexpr int RP = (NP + NTH - 1) / NTH, RW = (NWU + NTH - 1) / NTH;
     __shared__ __attribute__((aligned(16))) T sP[PR * PC * ROW];
