@@ -224,12 +224,15 @@ __global__ void k_ups_derive(const float* __restrict__ p, uint16_t* w16, float* 
 // D[co][px] = W[co][K] * X[K][px] (A = weights in registers, B = 16 pixels
 // of one row) and reads the R + 2 input rows of its strip once: the
 // fragments of input row rho feed output rows rho - 2 .. rho, each with the
-// weights of its tap row.  A 16-channel layer takes two K-steps per tap row
-// (taps dx 0 and 1; tap dx 2 beside a zero-weight column), layer 1 (4
-// channels per pixel) one K-step per two tap rows (rows dy 0 and 1; row dy
-// 2 beside a zero-weight row).  Every LDS address is a per-lane base plus a
-// compile-time offset, the accumulators start at the bias, and a row's
-// epilogue is two packed FP16 conversions and a packed ReLU.
+// weights of its tap row.  Layer 1 (4 channels per pixel) takes one K-step
+// per two tap rows (rows dy 0 and 1; row dy 2 beside a zero-weight row);
+// layer 2 one K-step per tap row for taps dx 0 and 1 and one per two input
+// rows for tap dx 2 (ups_strip16p); layer 3, whose 2 output channels would
+// leave 14 of the MFMA's 16 rows idle, holds (channel, tap row) in the rows
+// instead and rolls its accumulator from input row to input row
+// (ups_strip_out).  Every LDS address is a per-lane base plus a compile-time
+// offset, the accumulators start at the bias, and a row's epilogue is two
+// packed FP16 conversions and a packed ReLU.
 // ---------------------------------------------------------------------------
 typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 up_f16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float up_f32x4;
