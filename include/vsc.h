@@ -329,7 +329,10 @@ int vsc_conv3x3_nhwc(const void* in, int in_cs, int in_c0, int N, int H, int W, 
  * frames by nearest sampling to round(W/f) x round(H/f) before the network;
  * the flow is then produced at that size (flowmodel.cpp:126-133).
  * dtype: VSC_F32 (plain form) or VSC_BF16 (BF16 storage and MFMA operands,
- * FP32 accumulation).                                                       */
+ * FP32 accumulation).  The extractor and the decoders have scratch of their
+ * own: an extraction and a decode may run at once on two streams when the
+ * extraction writes none of the slots the decode reads (two extractions, or
+ * two decodes, of one network may not).                                    */
 typedef struct vsc_flownet vsc_flownet;
 
 int vsc_flownet_create(int frame_width, int frame_height, int downscale, int dtype, unsigned long long seed, int slots,

@@ -97,3 +97,34 @@ def test_flow_batch_refusals(vsc, cuda):
     for bad in (0, 5):
         with pytest.raises(vsc.VscError):
             vsc.Pipeline(W, H, form="plain", flow_batch=bad)
+
+
+def test_flow_batch_overlap_back_to_back_with_jump(vsc, cuda):
+    """Frame-pipelined batches submitted back to back (no sync between calls,
+    so a batch's feature extraction runs on its own stream beside the previous
+    batch's decode), with short batches and a jump in the frame index (the
+    jump re-extracts into slots the decode before it may still read), equal
+    the one-frame-per-call pipeline frame for frame."""
+    import torch
+    W, H, n = 320, 192, 15
+    orig, proc = _video(vsc, W, H, n, seed=37)
+    o = torch.from_numpy(orig).to(cuda)
+    p = torch.from_numpy(proc).to(cuda)
+    groups = [[1, 2], [3, 4], [5], [8, 9], [10, 11], [12], [3, 4], [5, 6]]
+    frames = [t for g in groups for t in g]
+    ref, ref_gates = _single(vsc, cuda, o, p, W, H, "chosen", frames)
+    pipe = vsc.Pipeline(W, H, form="chosen", seed=7, flow_head_scale=0.05, overlap=True, flow_batch=2)
+    pipe.set_last_rgba(p[2])
+    got = []
+    for g in groups:
+        of = [torch.empty((H, W, 3), dtype=torch.float32, device=cuda) for _ in g]
+        rs = pipe.frames([_window(o, p, t) for t in g], g[0], out_float=of)
+        got += list(zip(rs, of))
+    pipe.sync()
+    gate = pipe.gate()
+    torch.cuda.synchronize()
+    pipe.close()
+    for i, ((ra, fa), (rb, fb)) in enumerate(zip(ref, got)):
+        assert torch.equal(ra, rb), frames[i]
+        assert torch.equal(fa, fb), frames[i]
+    assert gate == ref_gates[-1]

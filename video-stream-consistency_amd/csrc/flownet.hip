@@ -86,7 +86,9 @@ struct vsc_flownet {
     float* flow[7] = {};                  // per level [pairs][h][w][2]
     float* refined = nullptr;             // [pairs][h3][w3][2]
     void* rbuf[2] = {nullptr, nullptr};   // refiner ping-pong [pairs][h3][w3][128]
-    float* partial = nullptr;             // split-channel partial sums of the small layers
+    float* partial = nullptr;             // split-channel partial sums of the small decoder layers
+    float* partial_x = nullptr;           // the same for the extractor ([frames] items): extraction and
+                                          // decoding may run on two streams at once
     std::vector<void*> owned;
     int last_pairs = 0;
 };
@@ -222,7 +224,7 @@ Sizes sizes_of(const vsc_flownet* n)
     z.weights = 0;
     for (const Layer& l : n->L)
         z.weights += (l.k == 4 ? l.w.size() * 4 : (size_t)l.cout * 9 * l.cin_pad * eb) + l.b.size() * 4 + 512;
-    z.weights += kPartialFloatsPerItem * n->pairs * 4;
+    z.weights += kPartialFloatsPerItem * (n->pairs + n->frames) * 4;
     return z;
 }
 
@@ -267,6 +269,7 @@ int alloc_all(vsc_flownet* n)
     n->rbuf[0] = dalloc(n, z.rbuf, rc);
     n->rbuf[1] = dalloc(n, z.rbuf, rc);
     n->partial = static_cast<float*>(dalloc(n, kPartialFloatsPerItem * n->pairs * 4, rc));
+    n->partial_x = static_cast<float*>(dalloc(n, kPartialFloatsPerItem * n->frames * 4, rc));
     for (Layer& l : n->L) {
         l.dw = dalloc(n, l.k == 4 ? l.w.size() * 4 : (size_t)l.cout * 9 * l.cin_pad * n->eb, rc);
         l.db = static_cast<float*>(dalloc(n, l.b.size() * 4, rc));
@@ -279,7 +282,8 @@ void* feat_ptr(vsc_flownet* n, int slot, int lvl)
     return static_cast<char*>(n->feat[lvl]) + (size_t)slot * (n->Hp >> lvl) * (n->Wp >> lvl) * cv_of(n, lvl) * n->eb;
 }
 
-int conv(vsc_flownet* n, const Layer& l, Nhwc in, Nhwc out, Nhwc add, int ek_out, hipStream_t s, Nhwc out2 = Nhwc{})
+int conv(vsc_flownet* n, const Layer& l, Nhwc in, Nhwc out, Nhwc add, int ek_out, hipStream_t s, Nhwc out2 = Nhwc{},
+         float* partial = nullptr)
 {
     // the device weights are [cout][3][3][cin_pad]: the slices must match them exactly
     if (in.c != l.cin_pad || out.c != l.cout || (add.p && add.c != l.cout))
@@ -289,7 +293,7 @@ int conv(vsc_flownet* n, const Layer& l, Nhwc in, Nhwc out, Nhwc add, int ek_out
     d.wt = l.dw; d.bias = l.db;
     d.stride = l.stride; d.dil = l.dil; d.pad = l.dil; d.act = l.act;
     d.ek_in = n->ek; d.ek_out = ek_out;
-    d.partial = n->partial;
+    d.partial = partial ? partial : n->partial;
     d.partial_cap = kPartialFloatsPerItem;
     return launch_conv3x3(d, s);
 }
@@ -318,9 +322,12 @@ int extract(vsc_flownet* n, const uint8_t* const* rgba, int m, int slot, hipStre
         // outputs carry the layer's channel count; as inputs the slices include
         // the zero padding channels up to the 16-byte multiple
         const int co = kFeatC[k];
-        if ((rc = conv(n, l[0], x, T(a, m, h, w, co, cv, 0), Nhwc{}, n->ek, s))) return rc;
-        if ((rc = conv(n, l[1], T(a, m, h, w, cv, cv, 0), T(b, m, h, w, co, cv, 0), Nhwc{}, n->ek, s))) return rc;
-        if ((rc = conv(n, l[2], T(b, m, h, w, cv, cv, 0), T(c, m, h, w, co, cv, 0), Nhwc{}, n->ek, s))) return rc;
+        float* px = n->partial_x;
+        if ((rc = conv(n, l[0], x, T(a, m, h, w, co, cv, 0), Nhwc{}, n->ek, s, Nhwc{}, px))) return rc;
+        if ((rc = conv(n, l[1], T(a, m, h, w, cv, cv, 0), T(b, m, h, w, co, cv, 0), Nhwc{}, n->ek, s, Nhwc{}, px)))
+            return rc;
+        if ((rc = conv(n, l[2], T(b, m, h, w, cv, cv, 0), T(c, m, h, w, co, cv, 0), Nhwc{}, n->ek, s, Nhwc{}, px)))
+            return rc;
         x = T(c, m, h, w, cv, cv, 0);
     }
     return VSC_OK;

@@ -12,22 +12,24 @@
 // All buffers are allocated at creation; a frame allocates nothing.  The
 // flows of frame t (and its fallback gate) live in buffer set t mod 2B (B =
 // cfg.flow_batch), so that with overlap the flows of the next frame or batch
-// (internal stream sf: features, decode, consistency gate) are computed while
-// the current one is upsampled and stabilized (internal stream ss).  A batch
-// of n <= B frames (vsc_pipeline_frames) decodes the forward and backward
-// flows of all its frames in one network batch of 2n pairs.
+// (internal stream sf: decode, consistency gate) are computed while the
+// current one is upsampled and stabilized (internal stream ss), and the new
+// frames' features of a batch (internal stream se) while the previous batch
+// decodes.  A batch of n <= B frames (vsc_pipeline_frames) decodes the
+// forward and backward flows of all its frames in one network batch of 2n
+// pairs.
 #include "vsc_internal.h"
 
 #include <algorithm>
 #include <cmath>
 #include <new>
 
-constexpr int PIPE_MAX_BATCH = 4, PIPE_MAX_SETS = 2 * PIPE_MAX_BATCH, PIPE_MAX_SLOTS = PIPE_MAX_BATCH + 2;
+constexpr int PIPE_MAX_BATCH = 4, PIPE_MAX_SETS = 2 * PIPE_MAX_BATCH, PIPE_MAX_SLOTS = 2 * PIPE_MAX_BATCH + 2;
 
 struct vsc_pipeline {
     int W, H, Wd, Hd;
     vsc_pipeline_cfg cfg;
-    int sets = 2, slots = 3;                       // flow buffer sets (2B), feature slots (B + 2)
+    int sets = 2, slots = 3;                       // flow buffer sets (2B), feature slots (B + 2; overlap 2B + 2)
     vsc_flownet* net = nullptr;
     vsc_upsampler* ups = nullptr;
     vsc_stream* stab = nullptr;
@@ -35,7 +37,7 @@ struct vsc_pipeline {
     float* hi[PIPE_MAX_SETS][2] = {};              // [frame mod sets][fwd, bwd] at frame size when upsampling here
     uint32_t* count = nullptr;                     // [set] inconsistent-pixel counts (64 B apart)
     float* gate = nullptr;                         // [set] fallback gates (64 B apart)
-    long long slot_frame[PIPE_MAX_SLOTS] = {-1, -1, -1, -1, -1, -1};   // frame index held by each feature slot
+    long long slot_frame[PIPE_MAX_SLOTS];           // frame index held by each feature slot (-1: none)
     long long prev_fwd_frame = -2;                 // last frame whose flows were computed
     // overlap: flows on sf, stabilization on ss; ev_s[k] = the stabilization
     // that last read buffer set k, ev_f = the last flow work
@@ -43,6 +45,15 @@ struct vsc_pipeline {
     hipEvent_t ev_in = nullptr, ev_f = nullptr, ev_s[PIPE_MAX_SETS] = {};
     bool f_rec = false, s_rec[PIPE_MAX_SETS] = {};
     int last_set = 0;                              // buffer set of the last frame
+    // overlap: features on se.  ev_e = the last extraction; ev_d[c & 1] = the
+    // decode of overlapped call c; slot_read[j] = the call whose decode last
+    // read feature slot j (-1: none).  With 2B + 2 slots the slots a batch's
+    // extraction overwrites were last read two calls before, so it runs
+    // beside the previous call's decode.
+    hipStream_t se = nullptr;
+    hipEvent_t ev_e = nullptr, ev_d[2] = {};
+    bool e_rec = false, d_rec[2] = {};
+    long long calls = 0, slot_read[PIPE_MAX_SLOTS];
 };
 
 namespace vsc {
@@ -86,6 +97,9 @@ int join_all(vsc_pipeline* p, hipStream_t s)
 {
     if (!p->cfg.overlap) return VSC_OK;
     if (p->f_rec && check_hip(hipStreamWaitEvent(s, p->ev_f, 0), "vsc_pipeline join")) return VSC_E_HIP;
+    if (p->e_rec && check_hip(hipStreamWaitEvent(s, p->ev_e, 0), "vsc_pipeline join")) return VSC_E_HIP;
+    for (int d = 0; d < 2; ++d)
+        if (p->d_rec[d] && check_hip(hipStreamWaitEvent(s, p->ev_d[d], 0), "vsc_pipeline join")) return VSC_E_HIP;
     for (int k = 0; k < p->sets; ++k)
         if (p->s_rec[k] && check_hip(hipStreamWaitEvent(s, p->ev_s[k], 0), "vsc_pipeline join")) return VSC_E_HIP;
     return VSC_OK;
@@ -134,7 +148,8 @@ extern "C" size_t vsc_pipeline_footprint(int width, int height, const vsc_pipeli
     if (cfg_ok(width, height, cfg)) return 0;
     const int Wd = (int)std::lround((double)width / cfg->downscale), Hd = (int)std::lround((double)height / cfg->downscale);
     const int B = cfg->flow_batch;
-    size_t t = flownet_footprint_pairs(width, height, cfg->downscale, cfg->flow_dtype, B + 2, 2 * B);
+    size_t t = flownet_footprint_pairs(width, height, cfg->downscale, cfg->flow_dtype, cfg->overlap ? 2 * B + 2 : B + 2,
+                                       2 * B);
     const bool here = upsample_here(*cfg);
     t += vsc_stream_footprint(width, height, here ? width : Wd, here ? height : Hd);
     t += 2 * 2 * B * (size_t)Wd * Hd * 2 * 4;
@@ -153,7 +168,8 @@ extern "C" int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg
     if (!p) return set_error(VSC_E_NOMEM, "vsc_pipeline_create: host memory");
     p->W = width; p->H = height; p->cfg = *cfg;
     p->sets = 2 * cfg->flow_batch;
-    p->slots = cfg->flow_batch + 2;
+    p->slots = cfg->overlap ? 2 * cfg->flow_batch + 2 : cfg->flow_batch + 2;
+    for (int i = 0; i < PIPE_MAX_SLOTS; ++i) p->slot_frame[i] = p->slot_read[i] = -1;
     if ((rc = flownet_create_pairs(width, height, cfg->downscale, cfg->flow_dtype, cfg->seed, p->slots, 2 * cfg->flow_batch,
                                    &p->net)) ||
         (rc = vsc_flownet_set_corr_dtype(p->net, cfg->flow_dtype == VSC_BF16 ? VSC_BF16 : cfg->corr_dtype)) ||
@@ -174,8 +190,12 @@ extern "C" int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg
         // default-priority streams; events without timing
         if (check_hip(hipStreamCreateWithFlags(&p->sf, hipStreamNonBlocking), "vsc_pipeline_create") ||
             check_hip(hipStreamCreateWithFlags(&p->ss, hipStreamNonBlocking), "vsc_pipeline_create") ||
+            check_hip(hipStreamCreateWithFlags(&p->se, hipStreamNonBlocking), "vsc_pipeline_create") ||
             check_hip(hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming), "vsc_pipeline_create") ||
-            check_hip(hipEventCreateWithFlags(&p->ev_f, hipEventDisableTiming), "vsc_pipeline_create"))
+            check_hip(hipEventCreateWithFlags(&p->ev_f, hipEventDisableTiming), "vsc_pipeline_create") ||
+            check_hip(hipEventCreateWithFlags(&p->ev_e, hipEventDisableTiming), "vsc_pipeline_create") ||
+            check_hip(hipEventCreateWithFlags(&p->ev_d[0], hipEventDisableTiming), "vsc_pipeline_create") ||
+            check_hip(hipEventCreateWithFlags(&p->ev_d[1], hipEventDisableTiming), "vsc_pipeline_create"))
             rc = VSC_E_HIP;
         for (int k = 0; k < p->sets && !rc; ++k)
             if (check_hip(hipEventCreateWithFlags(&p->ev_s[k], hipEventDisableTiming), "vsc_pipeline_create"))
@@ -202,7 +222,7 @@ extern "C" int vsc_pipeline_create(int width, int height, const vsc_pipeline_cfg
 extern "C" int vsc_pipeline_destroy(vsc_pipeline* p)
 {
     if (!p) return VSC_OK;
-    for (hipStream_t h : {p->sf, p->ss})
+    for (hipStream_t h : {p->sf, p->ss, p->se})
         if (h) (void)hipStreamSynchronize(h);
     if (p->net) vsc_flownet_destroy(p->net);
     if (p->ups) vsc_upsampler_destroy(p->ups);
@@ -214,11 +234,11 @@ extern "C" int vsc_pipeline_destroy(vsc_pipeline* p)
         }
     for (void* b : {static_cast<void*>(p->count), static_cast<void*>(p->gate)})
         if (b) (void)hipFree(b);
-    for (hipEvent_t e : {p->ev_in, p->ev_f})
+    for (hipEvent_t e : {p->ev_in, p->ev_f, p->ev_e, p->ev_d[0], p->ev_d[1]})
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : p->ev_s)
         if (e) (void)hipEventDestroy(e);
-    for (hipStream_t h : {p->sf, p->ss})
+    for (hipStream_t h : {p->sf, p->ss, p->se})
         if (h) (void)hipStreamDestroy(h);
     delete p;
     return VSC_OK;
@@ -259,11 +279,16 @@ static void frame_pairs(const vsc_pipeline* p, long long frame, int* first, int*
 
 // the network part for the n consecutive windows of frames first .. first+n-1:
 // features of frames first-1 .. first+n (each extracted once), then one
-// batched decode of their 2n flow pairs into outs (fwd, bwd per frame)
+// batched decode of their 2n flow pairs into outs (fwd, bwd per frame) on s.
+// es: the extraction stream (overlap: p->se, which the caller has ordered
+// after its inputs; else s).
 static int flows_batch_on(vsc_pipeline* p, const vsc_window* w, int n, long long first, float* const* outs,
-                          hipStream_t s)
+                          hipStream_t s, hipStream_t es)
 {
     int rc;
+    const bool split = es != s;
+    const long long call = p->calls;
+    bool extracted = false, waited[2] = {false, false};
     // frames first-1 .. first+n: the ones not cached are extracted in batches
     // of consecutive frames whose slots do not wrap
     const uint8_t* rgba[PIPE_MAX_BATCH + 2];
@@ -279,20 +304,44 @@ static int flows_batch_on(vsc_pipeline* p, const vsc_window* w, int n, long long
         int m = 1;
         while (j + m < n + 2 && pmod(f + m, p->slots) != 0 && p->slot_frame[pmod(f + m, p->slots)] != f + m) ++m;
         m = std::min(m, p->cfg.flow_batch);
-        if ((rc = flownet_extract_frames(p->net, rgba + j, m, pmod(f, p->slots), s))) return rc;
+        if (split) {
+            // the slots overwritten must be done with: wait for the decode
+            // that last read each (the latest record of its event is that
+            // decode or a later one on the same stream)
+            for (int i = 0; i < m; ++i) {
+                const long long r = p->slot_read[pmod(f + i, p->slots)];
+                if (r < 0 || waited[r & 1] || !p->d_rec[r & 1]) continue;
+                if (check_hip(hipStreamWaitEvent(es, p->ev_d[r & 1], 0), "vsc_pipeline_frame")) return VSC_E_HIP;
+                waited[r & 1] = true;
+            }
+        }
+        if ((rc = flownet_extract_frames(p->net, rgba + j, m, pmod(f, p->slots), es))) return rc;
         for (int i = 0; i < m; ++i) p->slot_frame[pmod(f + i, p->slots)] = f + i;
+        extracted = true;
         j += m;
+    }
+    if (split && extracted) {
+        if (check_hip(hipEventRecord(p->ev_e, es), "vsc_pipeline_frame")) return VSC_E_HIP;
+        p->e_rec = true;
+        if (check_hip(hipStreamWaitEvent(s, p->ev_e, 0), "vsc_pipeline_frame")) return VSC_E_HIP;
     }
     int a[2 * PIPE_MAX_BATCH], b[2 * PIPE_MAX_BATCH];
     for (int i = 0; i < n; ++i) frame_pairs(p, first + i, a + 2 * i, b + 2 * i);
-    return vsc_flownet_flow(p->net, 2 * n, a, b, outs, 2, reinterpret_cast<vsc_hip_stream>(s));
+    if ((rc = vsc_flownet_flow(p->net, 2 * n, a, b, outs, 2, reinterpret_cast<vsc_hip_stream>(s)))) return rc;
+    if (split) {
+        if (check_hip(hipEventRecord(p->ev_d[call & 1], s), "vsc_pipeline_frame")) return VSC_E_HIP;
+        p->d_rec[call & 1] = true;
+        for (int i = 0; i < 2 * n; ++i) p->slot_read[a[i]] = p->slot_read[b[i]] = call;
+        p->calls = call + 1;
+    }
+    return VSC_OK;
 }
 
 static int flows_on(vsc_pipeline* p, const vsc_window* w, long long frame, float* fwd_out, float* bwd_out,
                     hipStream_t s)
 {
     float* outs[2] = {fwd_out, bwd_out};
-    return flows_batch_on(p, w, 1, frame, outs, s);
+    return flows_batch_on(p, w, 1, frame, outs, s, s);
 }
 
 extern "C" int vsc_pipeline_flows(vsc_pipeline* p, const vsc_window* w, long long frame, float* fwd_out,
@@ -460,7 +509,8 @@ extern "C" int vsc_pipeline_frames(vsc_pipeline* p, const vsc_window* w, int n, 
         // flows wait for the caller's inputs and for the stabilizations that
         // last read the batch's buffer sets (frames first - 2B ..)
         if (check_hip(hipEventRecord(p->ev_in, cs), "vsc_pipeline_frame") ||
-            check_hip(hipStreamWaitEvent(fs, p->ev_in, 0), "vsc_pipeline_frame"))
+            check_hip(hipStreamWaitEvent(fs, p->ev_in, 0), "vsc_pipeline_frame") ||
+            check_hip(hipStreamWaitEvent(p->se, p->ev_in, 0), "vsc_pipeline_frame"))
             return fail(VSC_E_HIP);
         for (int i = 0; i < n; ++i) {
             const int k = pmod(first + i, p->sets);
@@ -474,7 +524,7 @@ extern "C" int vsc_pipeline_frames(vsc_pipeline* p, const vsc_window* w, int n, 
         outs[2 * i] = p->lo[k][0];
         outs[2 * i + 1] = p->lo[k][1];
     }
-    if ((rc = flows_batch_on(p, w, n, first, outs, fs))) return fail(rc);
+    if ((rc = flows_batch_on(p, w, n, first, outs, fs, ov ? p->se : fs))) return fail(rc);
     for (int i = 0; i < n; ++i)
         if ((rc = gate_on(p, first + i, fs))) return fail(rc);
     // a batch of the BF16 network's flows is upsampled on the flow stream,
