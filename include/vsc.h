@@ -430,8 +430,8 @@ int vsc_flow_fb_gate(const float* fwd, const float* bwd, int width, int height, 
  * 167-265) including retrieveOpticalFlow (:267-279) through the flow network,
  * FLOWDOWNSCALE (videostabilizer.cpp:74-94, flowmodel.cpp:126-133,158-161)
  * and the learned upsampler.  Owns a vsc_flownet (flow_batch + 2 feature
- * slots), the flow buffers, an upsampler and a vsc_stream; a frame allocates
- * nothing.                                                                   */
+ * slots; 2 flow_batch + 2 with overlap), the flow buffers, an upsampler and
+ * a vsc_stream; a frame allocates nothing.                                   */
 typedef struct vsc_pipeline vsc_pipeline;
 
 typedef struct {
@@ -445,8 +445,9 @@ typedef struct {
     unsigned long long seed;    /* flow network (and untrained upsampler) initialisation */
     int solver;                 /* stabilization solver: 0 reference op order (VSC_FRAME_FUSED),
                                    1 FMA form (VSC_FRAME_FMA, <= 1 LSB from it) */
-    int overlap;                /* 1: frame-pipelined execution on two internal HIP streams: the flows
-                                   of a frame run while the previous frame is stabilized.  Frame t's
+    int overlap;                /* 1: frame-pipelined execution on three internal HIP streams: the flows
+                                   of a frame run while the previous frame is stabilized, and the new
+                                   frames' features of a call while the previous call decodes.  Frame t's
                                    outputs are then ordered on the caller's stream by the call for
                                    frame t+1 (or vsc_pipeline_sync), and the window frames of call t
                                    must stay unchanged until then.  Same results as 0. */

@@ -527,12 +527,13 @@ extern "C" int vsc_pipeline_frames(vsc_pipeline* p, const vsc_window* w, int n, 
     if ((rc = flows_batch_on(p, w, n, first, outs, fs, ov ? p->se : fs))) return fail(rc);
     for (int i = 0; i < n; ++i)
         if ((rc = gate_on(p, first + i, fs))) return fail(rc);
-    // a batch of the BF16 network's flows is upsampled on the flow stream,
-    // overlapping the previous batch's stabilization (1080p batches of two:
-    // 774 vs 758 frames/s); with one frame, or the longer flow work of an FP32
-    // network, the upsampling would delay the next flows instead (722 vs 683,
-    // 474 vs 499 frames/s) and runs on the stabilization stream
-    const bool ups_on_flows = n >= 2 && p->cfg.flow_dtype == VSC_BF16;
+    // the BF16 network's flows are upsampled on the flow stream, overlapping
+    // the previous frames' stabilization (1080p: batches of two 774 vs 758
+    // frames/s; one frame per call 809 vs 805 with the features extracted on
+    // their own stream); the longer flow work of an FP32 network would delay
+    // the next flows instead (474 vs 499 frames/s), so its flows are
+    // upsampled on the stabilization stream
+    const bool ups_on_flows = p->cfg.flow_dtype == VSC_BF16;
     const float* ffs[PIPE_MAX_BATCH] = {};
     const float* fbs[PIPE_MAX_BATCH] = {};
     if (ups_on_flows)
