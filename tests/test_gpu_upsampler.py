@@ -70,13 +70,15 @@ def test_upsampler_inference_matches_oracle(vsc, cuda, size):
 
 
 @pytest.mark.parametrize("lw,lh,lch,shift,scale", [(29, 37, 2, 0, 1.7), (48, 48, 3, 0, 2.0), (48, 48, 2, 1, 2.0),
-                                                   (61, 45, 3, 1, 0.5)])
+                                                   (61, 45, 3, 1, 0.5), (250, 250, 2, 0, 1.0),
+                                                   (251, 243, 3, 1, 0.5)])
 def test_upsampler_inference_input_forms(vsc, cuda, lw, lh, lch, shift, scale):
     """The kernel's input forms: the exact 2x form (output twice the low-res
-    size: each bilinear weight 0 or 1/2) and the general one (any ratio), both
-    flow channels from one 8-byte load or per channel (three-channel flows,
-    flows at an address that is not 8-byte aligned).  96 x 96 frames: every
-    tile touches the frame edge and the general ratios tile it unevenly."""
+    size: each bilinear weight 0 or 1/2) and the general one (any ratio, also
+    flows larger than the frame), both flow channels from one 8-byte load or
+    per channel (three-channel flows, flows at an address that is not 8-byte
+    aligned).  96 x 96 frames: every tile touches the frame edge and the
+    general ratios tile it unevenly."""
     import torch
     ut, (fr, gt, _) = _data(vsc, 1, 96, seed=11)
     rng = np.random.default_rng(lw * 100 + lh)
