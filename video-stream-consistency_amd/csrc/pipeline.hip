@@ -360,7 +360,10 @@ namespace {
 
 // Host state of a pipeline that a frame's enqueueing changes; restored when
 // enqueueing (or capturing) fails part-way, so a later call recomputes what
-// the failed one did not finish.
+// the failed one did not finish.  A feature slot the failed call began to
+// fill is marked empty rather than given back its old frame: an eager copy
+// into it may already have been enqueued (a captured one is discarded, and
+// the slot is then merely extracted again).
 struct HostState {
     long long slot_frame[PIPE_MAX_SLOTS];
     long long prev_fwd;
@@ -371,7 +374,7 @@ struct HostState {
     }
     void restore(vsc_pipeline* p) const
     {
-        for (int i = 0; i < PIPE_MAX_SLOTS; ++i) p->slot_frame[i] = slot_frame[i];
+        for (int i = 0; i < PIPE_MAX_SLOTS; ++i) p->slot_frame[i] = p->slot_frame[i] == slot_frame[i] ? slot_frame[i] : -1;
         p->prev_fwd_frame = prev_fwd;
         p->last_set = set;
     }

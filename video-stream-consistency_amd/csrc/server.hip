@@ -195,7 +195,8 @@ int frame_alone(vsc_server* sv, Stream& st, long long c)
         if ((rc = pipeline_advance(st.pipe, c))) return rc;
         if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
     } else if (sv->cfg.use_graphs && c >= 3) {
-        // a failed capture leaves the pipeline's host state as it was
+        // a failed capture leaves the pipeline's host state as it was, but
+        // for the feature slot it began to fill, which reads as empty
         // (vsc_pipeline_frame restores it), so the frame is retried whole
         if ((rc = capture(st.hs, &st.ge[ph], enqueue))) return rc;
         if (check_hip(hipGraphLaunch(st.ge[ph], st.hs), "vsc_server_tick graph launch")) return VSC_E_HIP;
