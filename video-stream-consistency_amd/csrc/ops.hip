@@ -28,6 +28,7 @@ namespace vsc {
 // 8*QX pixels is written contiguously.
 // ---------------------------------------------------------------------------
 static constexpr int CF_TH = 8, CF_HALO = 4;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
 
 template <int MD, bool LEGACY, int QX, int CC>
 __global__ __launch_bounds__(576) void k_corr_f32(const float* __restrict__ f1, const float* __restrict__ f2,
@@ -114,6 +115,19 @@ __global__ __launch_bounds__(576) void k_corr_f32(const float* __restrict__ f1, 
     const int h = h0 + row;
     if (d >= P || h >= H) return;
     float* o = out + ((long long)n * P * P + (long long)d * P) * HW + (long long)h * W + w0 + quad * QX;
+    // four pixels of a plane row as one 16-byte store when they are in the
+    // image and 16-byte aligned (a 16-byte aligned output and W a multiple of
+    // 4: every plane row then starts aligned)
+    if (QX == 4 && (W & 3) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0 && w0 + quad * QX + 3 < W) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            float v[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) v[x] = LEGACY ? acc[x % QX][k] / (float)C : acc[x % QX][k];
+            __builtin_nontemporal_store(*reinterpret_cast<const f32x4_t*>(v), reinterpret_cast<f32x4_t*>(o + k * HW));
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < P; ++k)
 #pragma unroll
