@@ -251,13 +251,22 @@ __global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v,
                 }
             }
     __syncthreads();
-    // each group of 32 lanes writes one 128-byte row segment of one plane;
+    // each group of 8 lanes writes one 128-byte row segment of one plane, 16
+    // bytes per lane where the 4 pixels are in the image and the output rows
+    // are 16-byte aligned (a 16-byte aligned output, W a multiple of 4);
     // the cost volume is not re-read by this kernel: non-temporal stores
-    for (int e = tid; e < CM_ROWS * P * P * CM_PX; e += 512) {
-        const int x = e % CM_PX, pl = (e / CM_PX) % (P * P), r = e / (CM_PX * P * P);
+    const bool vec = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+    for (int e = tid; e < CM_ROWS * P * P * CM_PX / 4; e += 512) {
+        const int x = 4 * (e % (CM_PX / 4)), pl = (e / (CM_PX / 4)) % (P * P), r = e / (CM_PX / 4 * P * P);
         const int h = h0 + r, w = w0 + x;
-        if (h < H && w < W)
-            __builtin_nontemporal_store(stage[r][pl][x], out + ((long long)n * P * P + pl) * HW + (long long)h * W + w);
+        if (h >= H) continue;
+        float* o = out + ((long long)n * P * P + pl) * HW + (long long)h * W + w;
+        if (vec && w + 3 < W) {
+            __builtin_nontemporal_store(*reinterpret_cast<const f32x4*>(&stage[r][pl][x]), reinterpret_cast<f32x4*>(o));
+        } else {
+            for (int j = 0; j < 4; ++j)
+                if (w + j < W) __builtin_nontemporal_store(stage[r][pl][x + j], o + j);
+        }
     }
 }
 

@@ -174,13 +174,16 @@ static void launch_corr_f32_q(const float* f1, const float* f2, float* out, int 
         hipLaunchKernelGGL((k_corr_f32<MD, false, QX, CC>), grid, dim3(576), 0, s, f1, f2, out, C, H, W);
 }
 
-// 4 pixels per thread when that still gives a full machine of blocks
+// 4 pixels per thread when that gives at least half a machine of blocks
+// (1920x1088 pyramid, with the 16-byte stores: at 240x136, 136 blocks of the
+// 4-pixel tile take 29.8 us against 40.2 us for 510 of the 1-pixel tile; at
+// 120x68, 36 against 135 blocks, 40 against 24 us)
 template <int MD>
 static void launch_corr_f32(const float* f1, const float* f2, float* out, int N, int C, int H, int W, bool legacy,
                             hipStream_t s)
 {
     const long long blocks4 = (long long)((W + 31) / 32) * ((H + CF_TH - 1) / CF_TH) * N;
-    if (blocks4 >= 256) launch_corr_f32_q<MD, 4, 8>(f1, f2, out, N, C, H, W, legacy, s);
+    if (blocks4 >= 128) launch_corr_f32_q<MD, 4, 8>(f1, f2, out, N, C, H, W, legacy, s);
     else launch_corr_f32_q<MD, 1, 32>(f1, f2, out, N, C, H, W, legacy, s);
 }
 
