@@ -37,10 +37,14 @@ __global__ __launch_bounds__(576) void k_corr_f32(const float* __restrict__ f1, 
     constexpr int P = 2 * MD + 1;
     constexpr int TW = 8 * QX;
     constexpr int SW = TW + 2 * CF_HALO, SH = CF_TH + 2 * CF_HALO;
+    // f2 row pitch: the 1-pixel tile's lanes read 8 rows x 8 consecutive
+    // pixels at once; a pitch of 24 floats puts those rows on distinct banks
+    // (16 would stack them on the same 16)
+    constexpr int SWP = QX == 1 ? 24 : SW;
     constexpr int N1 = CC * CF_TH * TW, N2 = CC * SH * SW;
     constexpr int L1 = (N1 + 575) / 576, L2 = (N2 + 575) / 576;
     __shared__ __attribute__((aligned(16))) float s1[CC][CF_TH][TW];
-    __shared__ __attribute__((aligned(16))) float s2[CC][SH][SW];
+    __shared__ __attribute__((aligned(16))) float s2[CC][SH][SWP];
 
     const int n = blockIdx.z;
     const int tid = threadIdx.x;
@@ -92,7 +96,10 @@ __global__ __launch_bounds__(576) void k_corr_f32(const float* __restrict__ f1, 
 #pragma unroll
         for (int k = 0; k < L2; ++k) {
             const int e = tid + 576 * k;
-            if (e < N2) (&s2[0][0][0])[e] = r2[k];
+            if (e < N2) {
+                if constexpr (SWP == SW) (&s2[0][0][0])[e] = r2[k];
+                else s2[e / (SW * SH)][(e / SW) % SH][e % SW] = r2[k];
+            }
         }
         __syncthreads();
         if (c0 + CC < C) load_chunk(c0 + CC);   // next chunk in flight during the compute
