@@ -12,13 +12,16 @@ namespace vsc {
 // Correlation, FP32, sequential-in-c accumulation (the order of the reference
 // CPU op, correlation.cc:163-177: *dst = 0; *dst += v1*v2 for c = 0..C-1).
 //
-// A 576-thread block owns a tile of 8 rows x 8*QX pixels of one image.  Wave
-// d (9 waves) handles vertical displacement dy = d - MD; its lane (row, quad)
+// A block owns a tile of 8 rows x 8*QX pixels of one image and DG of the
+// vertical displacements (a 576-thread block all 9; the small levels' tiles
+// spread them over 3 blocks of 3 waves).  Wave d handles vertical
+// displacement dy = d - MD; its lane (row, quad)
 // keeps QX pixels x (2*MD+1) horizontal displacements in registers (QX = 4
 // for large images: 12 f2 loads from LDS feed 36 multiply-adds; QX = 1 for
 // small pyramid levels, where parallelism matters more than reuse).  Channels
-// stream through LDS in chunks (8 channels for QX = 4, 32 for QX = 1): the f1 tile and the f2 tile with a
-// 4-pixel halo, zeros outside the image (an out-of-range term of the
+// stream through LDS in chunks (8 channels for QX = 4, 32 for QX = 1): the f1 tile and the f2 tile (a
+// 4-pixel halo in x, the rows the block's displacements reach), zeros
+// outside the image (an out-of-range term of the
 // reference contributes nothing, and acc + (+-0) == acc for the accumulator
 // values that occur, so zero terms leave every bit unchanged).  Every thread
 // issues all its global loads of the next chunk before computing the current
@@ -28,27 +31,36 @@ namespace vsc {
 // 8*QX pixels is written contiguously.
 // ---------------------------------------------------------------------------
 static constexpr int CF_TH = 8, CF_HALO = 4;
+// vertical displacements per block of the 1-pixel tile (md = 4: three blocks
+// of 3 waves per tile; 1920x1088 pyramid 134.0 us against 141.3 us for one
+// block of 9 waves and 162.7 us for nine blocks of one wave)
+static constexpr int CF_DG1 = 3;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
 
-template <int MD, bool LEGACY, int QX, int CC>
-__global__ __launch_bounds__(576) void k_corr_f32(const float* __restrict__ f1, const float* __restrict__ f2,
-                                                  float* __restrict__ out, int C, int H, int W)
+// DG: vertical displacements (waves) per block; the P of them are split over
+// P / DG blocks (blockIdx.z = item * (P / DG) + group), each staging only the
+// CF_TH + DG - 1 f2 rows its displacements read.
+template <int MD, bool LEGACY, int QX, int CC, int DG>
+__global__ __launch_bounds__(64 * DG) void k_corr_f32(const float* __restrict__ f1, const float* __restrict__ f2,
+                                                      float* __restrict__ out, int C, int H, int W)
 {
     constexpr int P = 2 * MD + 1;
+    constexpr int NTH = 64 * DG, G = DG >= P ? 1 : P / DG;
+    static_assert(DG >= P || P % DG == 0, "displacement groups of equal size");
     constexpr int TW = 8 * QX;
-    constexpr int SW = TW + 2 * CF_HALO, SH = CF_TH + 2 * CF_HALO;
+    constexpr int SW = TW + 2 * CF_HALO, SH = CF_TH + DG - 1;
     // f2 row pitch: the 1-pixel tile's lanes read 8 rows x 8 consecutive
     // pixels at once; a pitch of 24 floats puts those rows on distinct banks
     // (16 would stack them on the same 16)
     constexpr int SWP = QX == 1 ? 24 : SW;
     constexpr int N1 = CC * CF_TH * TW, N2 = CC * SH * SW;
-    constexpr int L1 = (N1 + 575) / 576, L2 = (N2 + 575) / 576;
+    constexpr int L1 = (N1 + NTH - 1) / NTH, L2 = (N2 + NTH - 1) / NTH;
     __shared__ __attribute__((aligned(16))) float s1[CC][CF_TH][TW];
     __shared__ __attribute__((aligned(16))) float s2[CC][SH][SWP];
 
-    const int n = blockIdx.z;
+    const int n = blockIdx.z / G, dg = blockIdx.z % G;
     const int tid = threadIdx.x;
-    const int d = tid / 64, rem = tid % 64;
+    const int dl = tid / 64, d = dg * DG + dl, rem = tid % 64;   // local and global displacement row
     const int row = rem / 8, quad = rem % 8;
     const int w0 = blockIdx.x * TW, h0 = blockIdx.y * CF_TH;
     const long long HW = (long long)H * W;
@@ -67,7 +79,7 @@ __global__ __launch_bounds__(576) void k_corr_f32(const float* __restrict__ f1, 
     auto load_chunk = [&](int c0) {
 #pragma unroll
         for (int k = 0; k < L1; ++k) {
-            const int e = tid + 576 * k;
+            const int e = tid + NTH * k;
             const int x = e % TW, r = (e / TW) % CF_TH, c = e / (TW * CF_TH);
             const int gh = min(h0 + r, H - 1), gw = min(w0 + x, W - 1), gc = min(c0 + c, C - 1);
             const bool ok = e < N1 && c0 + c < C && h0 + r < H && w0 + x < W;
@@ -76,9 +88,9 @@ __global__ __launch_bounds__(576) void k_corr_f32(const float* __restrict__ f1, 
         }
 #pragma unroll
         for (int k = 0; k < L2; ++k) {
-            const int e = tid + 576 * k;
+            const int e = tid + NTH * k;
             const int x = e % SW, r = (e / SW) % SH, c = e / (SW * SH);
-            const int ghr = h0 - CF_HALO + r, gwr = w0 - CF_HALO + x;
+            const int ghr = h0 - MD + dg * DG + r, gwr = w0 - CF_HALO + x;
             const int gh = min(max(ghr, 0), H - 1), gw = min(max(gwr, 0), W - 1), gc = min(c0 + c, C - 1);
             const bool ok = e < N2 && c0 + c < C && ghr >= 0 && ghr < H && gwr >= 0 && gwr < W;
             const float v = b[gc * HW + (long long)gh * W + gw];
@@ -90,12 +102,12 @@ __global__ __launch_bounds__(576) void k_corr_f32(const float* __restrict__ f1, 
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < L1; ++k) {
-            const int e = tid + 576 * k;
+            const int e = tid + NTH * k;
             if (e < N1) (&s1[0][0][0])[e] = r1[k];
         }
 #pragma unroll
         for (int k = 0; k < L2; ++k) {
-            const int e = tid + 576 * k;
+            const int e = tid + NTH * k;
             if (e < N2) {
                 if constexpr (SWP == SW) (&s2[0][0][0])[e] = r2[k];
                 else s2[e / (SW * SH)][(e / SW) % SH][e % SW] = r2[k];
@@ -105,7 +117,7 @@ __global__ __launch_bounds__(576) void k_corr_f32(const float* __restrict__ f1, 
         if (c0 + CC < C) load_chunk(c0 + CC);   // next chunk in flight during the compute
         if (d < P) {
             const int cn = min(CC, C - c0);
-            const int br = row + d - MD + CF_HALO;
+            const int br = row + dl;
             for (int cc = 0; cc < cn; ++cc) {
                 float va[QX], vb[QX + 2 * MD];
 #pragma unroll
@@ -170,15 +182,16 @@ __global__ __launch_bounds__(256) void k_corr_generic(const T* __restrict__ f1, 
     out[i] = LEGACY ? acc / (T)C : acc;
 }
 
-template <int MD, int QX, int CC>
+template <int MD, int QX, int CC, int DG>
 static void launch_corr_f32_q(const float* f1, const float* f2, float* out, int N, int C, int H, int W, bool legacy,
                               hipStream_t s)
 {
-    dim3 grid((W + 8 * QX - 1) / (8 * QX), (H + CF_TH - 1) / CF_TH, N);
+    constexpr int P = 2 * MD + 1, G = DG >= P ? 1 : P / DG;
+    dim3 grid((W + 8 * QX - 1) / (8 * QX), (H + CF_TH - 1) / CF_TH, N * G);
     if (legacy)
-        hipLaunchKernelGGL((k_corr_f32<MD, true, QX, CC>), grid, dim3(576), 0, s, f1, f2, out, C, H, W);
+        hipLaunchKernelGGL((k_corr_f32<MD, true, QX, CC, DG>), grid, dim3(64 * DG), 0, s, f1, f2, out, C, H, W);
     else
-        hipLaunchKernelGGL((k_corr_f32<MD, false, QX, CC>), grid, dim3(576), 0, s, f1, f2, out, C, H, W);
+        hipLaunchKernelGGL((k_corr_f32<MD, false, QX, CC, DG>), grid, dim3(64 * DG), 0, s, f1, f2, out, C, H, W);
 }
 
 // 4 pixels per thread when that gives at least half a machine of blocks
@@ -190,8 +203,9 @@ static void launch_corr_f32(const float* f1, const float* f2, float* out, int N,
                             hipStream_t s)
 {
     const long long blocks4 = (long long)((W + 31) / 32) * ((H + CF_TH - 1) / CF_TH) * N;
-    if (blocks4 >= 128) launch_corr_f32_q<MD, 4, 8>(f1, f2, out, N, C, H, W, legacy, s);
-    else launch_corr_f32_q<MD, 1, 32>(f1, f2, out, N, C, H, W, legacy, s);
+    if (blocks4 >= 128) launch_corr_f32_q<MD, 4, 8, 9>(f1, f2, out, N, C, H, W, legacy, s);
+    else launch_corr_f32_q<MD, 1, 32, ((2 * MD + 1) % CF_DG1 == 0 ? CF_DG1 : 2 * MD + 1)>(f1, f2, out, N, C, H, W,
+                                                                                           legacy, s);
 }
 
 // ---------------------------------------------------------------------------
