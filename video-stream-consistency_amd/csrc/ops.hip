@@ -260,19 +260,33 @@ __global__ __launch_bounds__(256) void k_warp(const T* __restrict__ in, const T*
         for (int c = c0; c < c1; ++c) o[c * HW] = (T)0;
         return;
     }
-    const int iL = (int)xL, iR = (int)xR, iT = (int)yT, iB = (int)yB;
+    // All 4 x WG_CH gathers are issued before any is used: from addresses
+    // clamped into the image and into the group's channels, so every load is
+    // valid; a corner outside the image is still left out of the sum below,
+    // which keeps the reference's terms and order.
+    const int iL = min(max((int)xL, 0), W - 1), iR = min(max((int)xR, 0), W - 1);
+    const int iT = min(max((int)yT, 0), H - 1), iB = min(max((int)yB, 0), H - 1);
     const long long oTL = (long long)iT * W + iL, oTR = (long long)iT * W + iR;
     const long long oBL = (long long)iB * W + iL, oBR = (long long)iB * W + iR;
     const T* src = in + base;
+    T vTL[WG_CH], vTR[WG_CH], vBL[WG_CH], vBR[WG_CH];
 #pragma unroll
-    for (int c = c0; c < c1; ++c) {
-        const T* pl = src + c * HW;
+    for (int j = 0; j < WG_CH; ++j) {
+        const T* pl = src + (long long)min(c0 + j, c1 - 1) * HW;
+        vTL[j] = pl[oTL];
+        vTR[j] = pl[oTR];
+        vBL[j] = pl[oBL];
+        vBR[j] = pl[oBR];
+    }
+#pragma unroll
+    for (int j = 0; j < WG_CH; ++j) {
+        if (c0 + j >= c1) break;
         T v = (T)0;
-        if (mT && mL) v = v + wTL * pl[oTL];
-        if (mT && mR) v = v + wTR * pl[oTR];
-        if (mB && mL) v = v + wBL * pl[oBL];
-        if (mB && mR) v = v + wBR * pl[oBR];
-        o[c * HW] = v;
+        if (mT && mL) v = v + wTL * vTL[j];
+        if (mT && mR) v = v + wTR * vTR[j];
+        if (mB && mL) v = v + wBL * vBL[j];
+        if (mB && mR) v = v + wBR * vBR[j];
+        o[(long long)(c0 + j) * HW] = v;
     }
 }
 
