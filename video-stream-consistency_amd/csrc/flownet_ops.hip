@@ -296,10 +296,17 @@ __global__ __launch_bounds__(256) void k_warp_nhwc(WarpArgs a)
     for (int k = 0; k < VEC; ++k) v[k] = 0.0f;
     if (mask >= 0.999f) {
         const T* src = static_cast<const T*>(a.src[n]) + g * VEC;
-        const int iL = (int)xL, iR = (int)xR, iT = (int)yT, iB = (int)yB;
-        auto acc = [&](bool m, int ix, int iy, float w) {
+        // the four corners are loaded before any is used, from addresses
+        // clamped into the image (every load valid); a corner outside it is
+        // still left out of the sum
+        const int iL = min(max((int)xL, 0), a.W - 1), iR = min(max((int)xR, 0), a.W - 1);
+        const int iT = min(max((int)yT, 0), a.H - 1), iB = min(max((int)yB, 0), a.H - 1);
+        auto ld = [&](int ix, int iy) {
+            return *reinterpret_cast<const fo_u32x4*>(src + ((long long)iy * a.W + ix) * a.scs);
+        };
+        const fo_u32x4 rTL = ld(iL, iT), rTR = ld(iR, iT), rBL = ld(iL, iB), rBR = ld(iR, iB);
+        auto acc = [&](bool m, const fo_u32x4& raw, float w) {
             if (!m) return;
-            const fo_u32x4 raw = *reinterpret_cast<const fo_u32x4*>(src + ((long long)iy * a.W + ix) * a.scs);
             if constexpr (EK == EK_F32) {
                 const fo_f32x4 f = __builtin_bit_cast(fo_f32x4, raw);
 #pragma unroll
@@ -312,10 +319,10 @@ __global__ __launch_bounds__(256) void k_warp_nhwc(WarpArgs a)
                 }
             }
         };
-        acc(mT && mL, iL, iT, wTL);
-        acc(mT && mR, iR, iT, wTR);
-        acc(mB && mL, iL, iB, wBL);
-        acc(mB && mR, iR, iB, wBR);
+        acc(mT && mL, rTL, wTL);
+        acc(mT && mR, rTR, wTR);
+        acc(mB && mL, rBL, wBL);
+        acc(mB && mR, rBR, wBR);
     }
     T* dst = static_cast<T*>(a.out) + ((long long)n * npx + p) * a.ocs + g * VEC;
     if constexpr (EK == EK_F32) {
