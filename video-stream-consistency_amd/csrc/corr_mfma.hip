@@ -84,9 +84,9 @@ __device__ inline uint2 nchw_px8(const uint32_t (&raw)[8], int j)
 // conversion pass runs before the kernel.  Otherwise the operands are
 // channels-last copies [n][y][x][CP].
 template <int KIND, int MD, bool LEGACY, bool NCHW>
-__global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v, const void* __restrict__ f2v,
-                                                   float* __restrict__ out, int C, int CP, int H, int W,
-                                                   const float* scale)
+__device__ __forceinline__ void corr_mfma_tile(const void* __restrict__ f1v, const void* __restrict__ f2v,
+                                               float* __restrict__ out, int C, int CP, int H, int W,
+                                               const float* scale)
 {
     constexpr int EB = (KIND == CM_FP8) ? 1 : 2;          // bytes per element
     constexpr int ST = CmLayout<EB>::STRIDE;              // LDS pixel stride in elements
@@ -270,19 +270,45 @@ __global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v,
     }
 }
 
+// The kernel as is (152 VGPRs for md = 4: one 512-thread block per CU), and
+// the same tile capped at 128 VGPRs for two blocks per CU (13 of them spill
+// to scratch at md = 4).  Grids of at least 256 blocks take the capped form
+// (1920x1088 pyramid, BF16: 32.9 vs 38.4 us at 480x272, 19.7 vs 21.5 us at
+// 240x136); smaller grids lose to the spills (120x68: 15.7 vs 13.9 us).
+template <int KIND, int MD, bool LEGACY, bool NCHW>
+__global__ __launch_bounds__(512) void k_corr_mfma(const void* __restrict__ f1v, const void* __restrict__ f2v,
+                                                   float* __restrict__ out, int C, int CP, int H, int W,
+                                                   const float* scale)
+{
+    corr_mfma_tile<KIND, MD, LEGACY, NCHW>(f1v, f2v, out, C, CP, H, W, scale);
+}
+
+template <int KIND, int MD, bool LEGACY, bool NCHW>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_corr_mfma2(
+    const void* __restrict__ f1v, const void* __restrict__ f2v, float* __restrict__ out, int C, int CP, int H, int W,
+    const float* scale)
+{
+    corr_mfma_tile<KIND, MD, LEGACY, NCHW>(f1v, f2v, out, C, CP, H, W, scale);
+}
+
 template <int KIND, int MD>
 static void launch_mfma(const void* f1, const void* f2, float* out, int N, int C, int CP, int H, int W, bool legacy,
                         bool nchw, const float* scale, hipStream_t s)
 {
     dim3 grid((W + CM_PX - 1) / CM_PX, (H + CM_ROWS - 1) / CM_ROWS, N);
+    const bool two = (long long)grid.x * grid.y * grid.z >= 256;
     if (legacy && nchw)
-        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, true, true>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
+        hipLaunchKernelGGL(two ? (k_corr_mfma2<KIND, MD, true, true>) : (k_corr_mfma<KIND, MD, true, true>), grid,
+                           dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
     else if (legacy)
-        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, true, false>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
+        hipLaunchKernelGGL(two ? (k_corr_mfma2<KIND, MD, true, false>) : (k_corr_mfma<KIND, MD, true, false>), grid,
+                           dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
     else if (nchw)
-        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, false, true>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
+        hipLaunchKernelGGL(two ? (k_corr_mfma2<KIND, MD, false, true>) : (k_corr_mfma<KIND, MD, false, true>), grid,
+                           dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
     else
-        hipLaunchKernelGGL((k_corr_mfma<KIND, MD, false, false>), grid, dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
+        hipLaunchKernelGGL(two ? (k_corr_mfma2<KIND, MD, false, false>) : (k_corr_mfma<KIND, MD, false, false>), grid,
+                           dim3(512), 0, s, f1, f2, out, C, CP, H, W, scale);
 }
 
 template <int KIND>
