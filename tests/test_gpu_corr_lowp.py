@@ -197,3 +197,30 @@ def test_corr_nchw_staging_equals_channels_last_copy(vsc, cuda, kind, shape):
             copy = vsc.correlation(ma, mb, 4, legacy)
         torch.cuda.synchronize()
         assert torch.equal(direct, copy), (kind, shape, legacy)
+
+
+@pytest.mark.parametrize("kind", ["bf16", "fp8"])
+@pytest.mark.parametrize("shape", [(1, 32, 136, 256), (1, 40, 17, 28)])
+def test_corr_lowp_unaligned_output(vsc, cuda, kind, shape):
+    """The MFMA kernel writes its staged planes with 16-byte stores when the
+    output is 16-byte aligned and W a multiple of 4; an output 4 bytes off
+    that alignment takes per-pixel stores with the same bits.  The 136 x 256
+    shape (272 blocks) runs the two-blocks-per-CU form, the 17 x 28 one the
+    other."""
+    import torch
+    rng = np.random.default_rng(sum(shape))
+    a = _t(rng.standard_normal(shape).astype(np.float32), cuda)
+    b = _t(rng.standard_normal(shape).astype(np.float32), cuda)
+    if kind == "fp8":
+        q1, q2, sc = vsc.quantize_fp8_pair(a, b)
+        run = lambda out=None: vsc.correlation_fp8(q1, q2, sc, 4, 0, out=out)
+    else:
+        ca, cb = vsc.convert(a, kind), vsc.convert(b, kind)
+        run = lambda out=None: vsc.correlation(ca, cb, 4, 0, out=out)
+    ref = run()
+    buf = torch.empty(ref.numel() + 1, dtype=torch.float32, device=cuda)
+    out = buf[1:].view(ref.shape)
+    assert out.data_ptr() % 16 == 4
+    run(out)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)

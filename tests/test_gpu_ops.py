@@ -128,3 +128,23 @@ def test_warp_f64(vsc, orc, cuda):
     fl = rng.uniform(-3, 3, (1, 2, 15, 21))
     got = vsc.warp(_t(x, cuda), _t(fl, cuda)).cpu().numpy()
     np.testing.assert_allclose(got, orc.warp(x, fl, dtype=np.float64), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("shape,legacy", [((1, 8, 136, 240), 0), ((1, 8, 136, 240), 1), ((1, 5, 17, 30), 0)])
+def test_correlation_f32_unaligned_output(vsc, cuda, shape, legacy):
+    """The FP32 kernel writes 4 outputs of a plane row with one 16-byte store
+    when the output is 16-byte aligned and W a multiple of 4; an output
+    4 bytes off that alignment takes the per-pixel stores, with the same bits
+    (the 240-pixel shape uses the 4-pixel tile, the 30-pixel one the
+    1-pixel tile)."""
+    import torch
+    rng = np.random.default_rng(sum(shape) + legacy)
+    a = _t(rng.standard_normal(shape).astype(np.float32), cuda)
+    b = _t(rng.standard_normal(shape).astype(np.float32), cuda)
+    ref = vsc.correlation(a, b, 4, legacy)
+    buf = torch.empty(ref.numel() + 1, dtype=torch.float32, device=cuda)
+    out = buf[1:].view(ref.shape)
+    assert out.data_ptr() % 16 == 4
+    vsc.correlation(a, b, 4, legacy, out=out)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)

@@ -563,13 +563,15 @@ def quantize_fp8_pair(f1, f2, workspace=None):
     return q1, q2, sc
 
 
-def correlation_fp8(q1, q2, scale, max_displacement=4, legacy=0, workspace=None):
-    """Correlation of e4m3 operands given as uint8 bit patterns."""
+def correlation_fp8(q1, q2, scale, max_displacement=4, legacy=0, workspace=None, out=None):
+    """Correlation of e4m3 operands given as uint8 bit patterns (out: an
+    optional float32 device tensor of the output shape)."""
     torch = _torch()
     N, C, H, W = q1.shape
     P = 2 * int(max_displacement) + 1
     shape = (N, P * P, H, W) if legacy else (N, P, P, H, W)
-    out = torch.empty(shape, dtype=torch.float32, device=q1.device)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=q1.device)
     if workspace is not None:
         check(lib().vsc_correlation_forward_ws(_ptr(q1.contiguous()), _ptr(q2.contiguous()), _ptr(out), N, C, H, W,
                                                int(max_displacement), int(bool(legacy)), VSC_FP8E4M3,
