@@ -4,8 +4,9 @@ with flow_batch B stabilizes B consecutive windows per call, their forward
 and backward flows decoded in one network batch of 2B pairs.  Each flow has
 the same bits in any batch (the convolution split depends on one item's
 geometry only), so every output frame, and the learned-upsampler gate, equals
-the one-frame-per-call pipeline's, in both network precisions, with frame
-pipelining, and with a short last batch."""
+the one-frame-per-call pipeline's, in both network precisions and with the FP32
+network's cost volume on the BF16 MFMA, with frame pipelining, and with a
+short last batch."""
 import numpy as np
 import pytest
 
@@ -22,9 +23,18 @@ def _window(o, p, t):
     return (o[t - 1], o[t], o[t + 1], p[t - 1], p[t], p[t + 1])
 
 
+# the two named forms, and the FP32 network at half resolution with the cost
+# volume's products in BF16 (learned upsampling)
+FORM_KW = {
+    "chosen": dict(form="chosen"),
+    "plain": dict(form="plain"),
+    "bf16corr": dict(flow_dtype="f32", downscale=2, upsample="learned", solver="fma", corr_dtype="bf16"),
+}
+
+
 def _single(vsc, cuda, o, p, W, H, form, frames, overlap=False):
     import torch
-    pipe = vsc.Pipeline(W, H, form=form, seed=7, flow_head_scale=0.05, overlap=overlap)
+    pipe = vsc.Pipeline(W, H, seed=7, flow_head_scale=0.05, overlap=overlap, **FORM_KW[form])
     pipe.set_last_rgba(p[2])
     outs, gates = [], []
     for t in frames:
@@ -40,7 +50,7 @@ def _single(vsc, cuda, o, p, W, H, form, frames, overlap=False):
 
 def _batched(vsc, cuda, o, p, W, H, form, frames, B, overlap=False):
     import torch
-    pipe = vsc.Pipeline(W, H, form=form, seed=7, flow_head_scale=0.05, overlap=overlap, flow_batch=B)
+    pipe = vsc.Pipeline(W, H, seed=7, flow_head_scale=0.05, overlap=overlap, flow_batch=B, **FORM_KW[form])
     pipe.set_last_rgba(p[2])
     outs, gates = [], []
     for i in range(0, len(frames), B):
@@ -55,10 +65,11 @@ def _batched(vsc, cuda, o, p, W, H, form, frames, B, overlap=False):
     return outs, gates
 
 
-@pytest.mark.parametrize("form,B,overlap", [("chosen", 4, False), ("chosen", 3, True), ("plain", 2, False)])
+@pytest.mark.parametrize("form,B,overlap", [("chosen", 4, False), ("chosen", 3, True), ("plain", 2, False),
+                                             ("bf16corr", 2, True)])
 def test_flow_batch_equals_one_frame_per_call(vsc, cuda, form, B, overlap):
     import torch
-    W, H = (320, 192) if form == "chosen" else (192, 128)
+    W, H = (192, 128) if form == "plain" else (320, 192)
     n = 12
     print(f"flow batch {form} B={B} overlap={overlap} {W}x{H}", flush=True)
     orig, proc = _video(vsc, W, H, n, seed=31)
