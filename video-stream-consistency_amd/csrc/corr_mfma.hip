@@ -94,14 +94,17 @@ __device__ __forceinline__ void corr_mfma_tile(const void* __restrict__ f1v, con
     constexpr int BR = CM_ROWS + 2 * CM_HALO, BC = CM_PX + 2 * CM_HALO;
     using E = typename std::conditional<EB == 2, uint16_t, uint8_t>::type;
     // LDS: the operand tiles during the channel loop, then the whole output
-    // tile [row][plane][32 px] (the operand tiles are dead by then)
+    // tile [row][plane][32 px] at a plane pitch of 36 floats (the operand
+    // tiles are dead by then): the 16 lanes of a K group write 16 planes at
+    // once, which a 32-float pitch puts on the same banks
+    constexpr int SP = CM_PX + 4;
     constexpr int A_BYTES = CM_ROWS * CM_PX * ST * EB, B_BYTES = BR * BC * ST * EB;
-    constexpr int OUT_BYTES = CM_ROWS * P * P * CM_PX * 4;
+    constexpr int OUT_BYTES = CM_ROWS * P * P * SP * 4;
     constexpr int LDS_BYTES = (A_BYTES + B_BYTES) > OUT_BYTES ? (A_BYTES + B_BYTES) : OUT_BYTES;
     __shared__ __attribute__((aligned(16))) unsigned char lds[LDS_BYTES];
     E (*sA)[CM_PX][ST] = reinterpret_cast<E (*)[CM_PX][ST]>(lds);
     E (*sB)[BC][ST] = reinterpret_cast<E (*)[BC][ST]>(lds + A_BYTES);
-    float (*stage)[P * P][CM_PX] = reinterpret_cast<float (*)[P * P][CM_PX]>(lds);
+    float (*stage)[P * P][SP] = reinterpret_cast<float (*)[P * P][SP]>(lds);
 
     // operands are channels-last copies [n][y][x][CP] (CP: C padded to a
     // multiple of 32 with zeros), so a chunk of 32 channels of one pixel is
